@@ -1,0 +1,167 @@
+// ============================================================================
+// afmm_internal.h -- device-side data layout and the host entry points of the
+// kernel files (scan_sort.hip, tree.hip, fmm.hip). Not part of the C ABI.
+//
+// HBM layout (all struct-of-arrays, grow-only pools owned by the context):
+//   bodies     N: sorted x, y, z, m (FP64), orig index (int32), leaf of body.
+//              Input AoS positions are read once by the key kernel and once
+//              by the gather.
+//   boxes      B, breadth-first: level-major, Morton order inside a level, so
+//              the children of a box are contiguous (child_first, child mask).
+//              seg_first/seg_count index the sorted bodies.
+//   lists      colleague (27 slots/box), V (189 slots/box) and the dense M2L
+//              source table src_at[B][343]; U, W (per leaf) and X (per box)
+//              in CSR.
+//   expansions Mhat[B][S], Lhat[B][S] (FP64, scaled real basis).
+// ============================================================================
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace afmm {
+
+constexpr int kWave = 64;
+constexpr int kMaxDepth = 24;   // AdaptiveOctree::maxDepth (AdaptiveOctree.hpp:49)
+constexpr int kCollSlots = 27;  // same-level neighbours incl. self
+constexpr int kVSlots = 189;    // 6^3 - 3^3 separated children of the parent's colleagues
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+#define AFMM_CHECK(call)                                                                  \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::afmm::HipError(e_, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Grow-only named device allocations.
+class Pool {
+public:
+    ~Pool() { release(); }
+    void release() {
+        for (auto& kv : slots_) (void)hipFree(kv.second.p);
+        slots_.clear();
+    }
+    // Buffer of at least `count` elements; contents are undefined after growth
+    // unless keep is set (then the old prefix is copied on `s`).
+    template <class T>
+    T* get(const std::string& name, size_t count, hipStream_t s = nullptr, bool keep = false) {
+        const size_t bytes = (count ? count : 1) * sizeof(T);
+        Slot& sl = slots_[name];
+        if (sl.bytes < bytes) {
+            size_t nb = bytes + bytes / 4 + 256;
+            void* np = nullptr;
+            hipError_t e = hipMalloc(&np, nb);
+            if (e != hipSuccess) throw HipError(e, "hipMalloc(" + name + ")");
+            if (keep && sl.p) AFMM_CHECK(hipMemcpyAsync(np, sl.p, sl.bytes, hipMemcpyDeviceToDevice, s));
+            if (sl.p) {
+                if (keep) AFMM_CHECK(hipStreamSynchronize(s));
+                AFMM_CHECK(hipFree(sl.p));
+            }
+            sl.p = np;
+            sl.bytes = nb;
+        }
+        return static_cast<T*>(sl.p);
+    }
+    size_t total_bytes() const {
+        size_t t = 0;
+        for (const auto& kv : slots_) t += kv.second.bytes;
+        return t;
+    }
+
+private:
+    struct Slot {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Slot> slots_;
+};
+
+// Root cube and sizing, computed on the device from the bodies.
+struct RootInfo {
+    double c[3];      // 0.5*(lo+hi) per axis (AdaptiveOctree.hpp:77)
+    double half;      // after *(1+1e-9)      (:78-80)
+    double tol;       // 1e-9*half            (:81)
+    int key_levels;   // octant digits needed: levels that may split under the eps floor
+    int nonfinite;    // count of NaN/Inf inputs
+};
+
+// Device pointers of the built tree (breadth-first box ids).
+struct TreeView {
+    int n = 0, nb = 0, nleaves = 0, max_level = 0;
+    double tol = 0;
+    double *x, *y, *z, *m;
+    int* orig;
+    int* leaf_of;        // [n] BFS leaf id of each sorted body
+    int *seg_first, *seg_count, *level, *parent, *octant, *child_first, *nchild;
+    unsigned char* child_mask;
+    int* refid;          // [B] reference (DFS-allocation) box id
+    double* cfp;         // [3B] reference floating-point centres
+    double* cdd;         // [6B] exact centres, (hi, lo) per axis
+    double* half;        // [B]
+    double* invh;        // [B]
+    int* ic;             // [3B] integer coordinates at the box's level
+    int* leaf_ids;       // [nleaves] BFS ids of the leaves
+    // lists
+    int *coll, *ncoll;   // [27B], [B]
+    int *vl, *nv;        // [189B], [B]
+    int* src_at;         // [343B] source box per M2L offset, -1 if none
+    int *u_off, *u_idx;  // per leaf (index into leaf_ids)
+    int *w_off, *w_idx;  // per leaf
+    int *x_off, *x_idx;  // per box
+    int64_t n_u = 0, n_w = 0, n_x = 0;
+};
+
+// Host-side shape of the built tree: boxes of level l are the breadth-first
+// ids [level_off[l], level_off[l+1]).
+struct TreeMeta {
+    std::vector<int> level_off;
+};
+
+// ---- scan_sort.hip -----------------------------------------------------------
+int scan_tmp_ints(int n);
+// Exclusive prefix sum of n int32 values (in place allowed); total to *d_total
+// when non-null.
+void scan_exclusive(const int* d_in, int* d_out, int n, int* d_total, int* d_tmp, hipStream_t s);
+
+struct SortBufs {
+    uint64_t* khi[2];
+    uint32_t* klo[2];
+    int* val[2];
+    int* hist;
+    int* scan_tmp;
+};
+int sort_blocks(int n);
+// Stable LSD radix sort of (khi, klo, val) on 9-bit digits of the 72-bit key
+// (khi << 9) | klo: digit g is bits [9g, 9g+9). Sorts digits
+// first..first+count-1 in that order; returns the buffer set holding the result.
+int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s);
+
+// ---- tree.hip ----------------------------------------------------------------
+// Builds the adaptive octree and all interaction lists for n bodies given as
+// device AoS positions and masses. Reproduces AdaptiveOctree::build and
+// buildLists (AdaptiveOctree.hpp:65-189) exactly.
+void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int n, int cap,
+                double eps_floor, TreeView& tv, TreeMeta& meta, RootInfo& root);
+
+// ---- fmm.hip -----------------------------------------------------------------
+struct FmmTimes {
+    float upward = 0, m2l = 0, downward = 0, leaf = 0;
+};
+struct FmmTables {
+    const double *m2m, *l2l, *m2l;  // device, column-major S x S each
+};
+// Runs the FMM passes on a built tree and writes G*(far + near) to d_acc in
+// input order (AoS n x 3).
+void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
+             const FmmTables& tab, double G, double eps, double* d_acc, hipEvent_t* ev /* 5 or null */);
+
+}  // namespace afmm

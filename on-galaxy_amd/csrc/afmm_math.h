@@ -1,0 +1,235 @@
+// ============================================================================
+// afmm_math.h -- per-point expansion math shared by the gfx950 kernels and
+// the host-side table builder / CPU model tests (all __host__ __device__).
+//
+// Convention: the reference's Greengard-Rokhlin normalisation, no
+// Condon-Shortley phase (FmmExpansions3D.hpp:12-38):
+//   Y_n^m = sqrt((n-|m|)!/(n+|m|)!) P_n^|m|(cos t) e^{i m phi}, Y_n^{-m} = conj
+//   regular  R_n^m(v) = |v|^n Y_n^m,   singular S_n^m(v) = Y_n^m / |v|^{n+1}
+//   P2M  M_n^m = sum q R_n^{-m}(d)           (FmmExpansions3D.hpp:148-163)
+//   P2L  L_n^m = sum q S_n^{-m}(d)           (AdaptiveFMM3D.hpp:117-128)
+//   potential sum M S(x-c)  /  sum L R(x-c)  (:166-177, :203-213)
+//
+// Storage: real sources give M_n^{-m} = conj(M_n^m) (same for L), so an
+// expansion is (p+1)^2 reals, degree-major: [Re a_n^0, Re a_n^1, Im a_n^1, ...].
+// Scaling: a box of half-width h stores Mhat_n^m = M_n^m / h^n and
+// Lhat_n^m = L_n^m h^{n+1}; point vectors enter divided by h. Every
+// translation operator is then independent of the level.
+//
+// Harmonics come from the Cartesian recurrences of the seminormalised
+// Legendre functions (no trigonometry): with r2 = |v|^2,
+//   R_0^0 = 1, R_a^a = sqrt((2a-1)/(2a)) (x+iy) R_{a-1}^{a-1},
+//   R_{a+1}^a = sqrt(2a+1) z R_a^a,
+//   R_n^a = ((2n-1) z R_{n-1}^a - sqrt((n-1)^2-a^2) r2 R_{n-2}^a)/sqrt(n^2-a^2)
+// and the same for S with z -> z/r2, r2 -> 1/r2, x+iy -> (x+iy)/r2, S_0^0 = 1/|v|.
+// They are streamed (a outer, n inner) so only three complex values are live.
+// ============================================================================
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define AFMM_HD __host__ __device__ __forceinline__
+
+namespace afmm {
+
+constexpr int kMaxOrder = 12;
+
+// Real-storage offset of Re a_n^a (Im a_n^a follows it for a > 0).
+AFMM_HD constexpr int rid(int n, int a) { return n * n + (a == 0 ? 0 : 2 * a - 1); }
+AFMM_HD constexpr int nterms(int p) { return (p + 1) * (p + 1); }
+
+// ---- streamed solid harmonics ----------------------------------------------
+// f(n, a, re, im) is called for every 0 <= a <= n <= L.
+template <int L, class F>
+AFMM_HD void regular_stream(double x, double y, double z, F&& f) {
+    const double r2 = x * x + y * y + z * z;
+    double dre = 1.0, dim = 0.0;  // R_a^a
+#pragma unroll
+    for (int a = 0; a <= L; ++a) {
+        if (a > 0) {
+            const double c = __builtin_sqrt((2.0 * a - 1.0) / (2.0 * a));
+            const double tre = c * (x * dre - y * dim);
+            const double tim = c * (x * dim + y * dre);
+            dre = tre;
+            dim = tim;
+        }
+        f(a, a, dre, dim);
+        if (a + 1 <= L) {
+            const double c1 = __builtin_sqrt(2.0 * a + 1.0) * z;
+            double p2re = dre, p2im = dim;
+            double p1re = c1 * dre, p1im = c1 * dim;
+            f(a + 1, a, p1re, p1im);
+#pragma unroll
+            for (int n = a + 2; n <= L; ++n) {
+                const double inv = 1.0 / __builtin_sqrt(double(n * n - a * a));
+                const double ca = (2.0 * n - 1.0) * inv;
+                const double cb = __builtin_sqrt(double((n - 1) * (n - 1) - a * a)) * inv;
+                const double zz = ca * z, rr = cb * r2;
+                const double qre = zz * p1re - rr * p2re;
+                const double qim = zz * p1im - rr * p2im;
+                f(n, a, qre, qim);
+                p2re = p1re, p2im = p1im;
+                p1re = qre, p1im = qim;
+            }
+        }
+    }
+}
+
+// f(n, a, re, im) for every 0 <= a <= n <= L of S_n^a(v); v must be nonzero.
+template <int L, class F>
+AFMM_HD void singular_stream(double x, double y, double z, F&& f) {
+    const double r2 = x * x + y * y + z * z;
+    const double ir2 = 1.0 / r2;
+    const double ir = __builtin_sqrt(ir2);
+    const double ux = x * ir2, uy = y * ir2, uz = z * ir2;
+    double dre = ir, dim = 0.0;  // S_a^a
+#pragma unroll
+    for (int a = 0; a <= L; ++a) {
+        if (a > 0) {
+            const double c = __builtin_sqrt((2.0 * a - 1.0) / (2.0 * a));
+            const double tre = c * (ux * dre - uy * dim);
+            const double tim = c * (ux * dim + uy * dre);
+            dre = tre;
+            dim = tim;
+        }
+        f(a, a, dre, dim);
+        if (a + 1 <= L) {
+            const double c1 = __builtin_sqrt(2.0 * a + 1.0) * uz;
+            double p2re = dre, p2im = dim;
+            double p1re = c1 * dre, p1im = c1 * dim;
+            f(a + 1, a, p1re, p1im);
+#pragma unroll
+            for (int n = a + 2; n <= L; ++n) {
+                const double inv = 1.0 / __builtin_sqrt(double(n * n - a * a));
+                const double ca = (2.0 * n - 1.0) * inv;
+                const double cb = __builtin_sqrt(double((n - 1) * (n - 1) - a * a)) * inv;
+                const double zz = ca * uz, rr = cb * ir2;
+                const double qre = zz * p1re - rr * p2re;
+                const double qim = zz * p1im - rr * p2im;
+                f(n, a, qre, qim);
+                p2re = p1re, p2im = p1im;
+                p1re = qre, p1im = qim;
+            }
+        }
+    }
+}
+
+// ---- per-point operators ----------------------------------------------------
+// P2M contribution of one body (mass q, scaled offset d from the box centre):
+// Mhat_n^a += q conj(R_n^a(d)).
+template <int P, class Acc>
+AFMM_HD void p2m_point(double q, double dx, double dy, double dz, Acc&& add) {
+    regular_stream<P>(dx, dy, dz, [&](int n, int a, double re, double im) {
+        add(rid(n, a), q * re);
+        if (a) add(rid(n, a) + 1, -q * im);
+    });
+}
+
+// P2L contribution of one source body: Lhat_n^a += q conj(S_n^a(d)).
+template <int P, class Acc>
+AFMM_HD void p2l_point(double q, double dx, double dy, double dz, Acc&& add) {
+    singular_stream<P>(dx, dy, dz, [&](int n, int a, double re, double im) {
+        add(rid(n, a), q * re);
+        if (a) add(rid(n, a) + 1, -q * im);
+    });
+}
+
+// Gradient of sum_{n<=P} L_n^m R_n^m at the scaled point d. This is the
+// reference's localField (FmmExpansions3D.hpp:215-235) folded onto m >= 0:
+// with G+ = sum L_n^m c+_R(n,m) R_{n-1}^{m+1} the minus-branch sum equals
+// conj(G+), so grad = (Re G+, Im G+, sum L c_z R).
+template <int P, class Coef>
+AFMM_HD void local_grad(Coef L, double dx, double dy, double dz, double& gx, double& gy,
+                        double& gz) {
+    double pr = 0.0, pi = 0.0, zz = 0.0;
+    if constexpr (P >= 1) {
+        regular_stream<P - 1>(dx, dy, dz, [&](int np, int a, double re, double im) {
+            const int n = np + 1;
+            {
+                const double lr = L(rid(n, a)), li = a ? L(rid(n, a) + 1) : 0.0;
+                const double w = (a ? 2.0 : 1.0) * __builtin_sqrt(double(n * n - a * a));
+                zz += w * (re * lr - im * li);
+            }
+            if (a >= 1) {
+                const double lr = L(rid(n, a - 1)), li = (a - 1) ? L(rid(n, a - 1) + 1) : 0.0;
+                const double c = -__builtin_sqrt(double((n - a + 1) * (n - a)));
+                pr += c * (re * lr - im * li);
+                pi += c * (re * li + im * lr);
+            }
+            {
+                const double lr = L(rid(n, a + 1)), li = L(rid(n, a + 1) + 1);
+                const double c = __builtin_sqrt(double((n + a + 1) * (n + a)));
+                pr += c * (re * lr - im * li);
+                pi -= c * (re * li + im * lr);
+            }
+        });
+    }
+    gx = pr;
+    gy = pi;
+    gz = zz;
+}
+
+// Gradient of sum_{n<=P} M_n^m S_n^m at the scaled point d: the reference's
+// multipoleField (FmmExpansions3D.hpp:180-200) folded onto m >= 0 the same way.
+template <int P, class Coef>
+AFMM_HD void multipole_grad(Coef M, double dx, double dy, double dz, double& gx, double& gy,
+                            double& gz) {
+    double pr = 0.0, pi = 0.0, zz = 0.0;
+    singular_stream<P + 1>(dx, dy, dz, [&](int n2, int a, double re, double im) {
+        if (n2 == 0) return;
+        const int n = n2 - 1;
+        if (a <= n) {
+            const double mr = M(rid(n, a)), mi = a ? M(rid(n, a) + 1) : 0.0;
+            const double w = -(a ? 2.0 : 1.0) * __builtin_sqrt(double((n + 1) * (n + 1) - a * a));
+            zz += w * (re * mr - im * mi);
+        }
+        if (a >= 1) {
+            const double mr = M(rid(n, a - 1)), mi = (a - 1) ? M(rid(n, a - 1) + 1) : 0.0;
+            const double c = -__builtin_sqrt(double((n + a) * (n + a + 1)));
+            pr += c * (re * mr - im * mi);
+            pi += c * (re * mi + im * mr);
+        }
+        if (a + 1 <= n) {
+            const double mr = M(rid(n, a + 1)), mi = M(rid(n, a + 1) + 1);
+            const double c = __builtin_sqrt(double((n - a) * (n - a + 1)));
+            pr += c * (re * mr - im * mi);
+            pi -= c * (re * mi + im * mr);
+        }
+    });
+    gx = pr;
+    gy = pi;
+    gz = zz;
+}
+
+// ---- exact box geometry -----------------------------------------------------
+// A box at level l with integer coordinate i (0 <= i < 2^l) along an axis has
+// the exact centre c_root + h0 * (2i + 1 - 2^l) / 2^l. It is kept as an
+// unevaluated double pair (hi + lo) so that body offsets x - c are accurate to
+// an ulp of the offset, independent of |c|.
+struct Centre {
+    double hi, lo;
+};
+
+AFMM_HD Centre exact_centre(double croot, double h0, int level, int i) {
+    const double g = double(2 * (long long)i + 1 - (1LL << level));
+    const double ph = h0 * g;
+    const double pl = __builtin_fma(h0, g, -ph);  // h0*g = ph + pl exactly
+    const double s = __builtin_ldexp(1.0, -level);
+    const double a = croot, b = ph * s;
+    const double sh = a + b;
+    const double bb = sh - a;
+    const double sl = (a - (sh - bb)) + (b - bb) + pl * s;
+    const double hi = sh + sl;
+    return Centre{hi, sl - (hi - sh)};
+}
+
+AFMM_HD double offset_from(double x, Centre c) { return (x - c.hi) - c.lo; }
+
+// Offset index of an M2L pair: target minus source in units of the box edge,
+// each component in [-3, 3].
+AFMM_HD constexpr int offset_index(int dx, int dy, int dz) {
+    return (dx + 3) * 49 + (dy + 3) * 7 + (dz + 3);
+}
+constexpr int kNumOffsets = 343;
+
+}  // namespace afmm

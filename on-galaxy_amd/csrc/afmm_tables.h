@@ -1,0 +1,29 @@
+// ============================================================================
+// afmm_tables.h -- level-independent translation matrices in the scaled real
+// basis of afmm_math.h, built on the host in long double.
+//
+//   m2m[o]  child (octant o) -> parent   (FmmExpansions3D.hpp:242-265)
+//   l2l[o]  parent -> child (octant o)   (FmmExpansions3D.hpp:307-330)
+//   m2l[k]  source -> target, k = offset_index(target - source)
+//                                        (FmmExpansions3D.hpp:273-299)
+// Each matrix is S x S (S = (p+1)^2) stored column-major: entry (row r, col c)
+// at [c*S + r], so out[r] = sum_c T[c*S + r] in[c]. Offsets with all
+// components in [-1, 1] are adjacent pairs and are left zero.
+// ============================================================================
+#pragma once
+
+#include <vector>
+
+namespace afmm {
+
+struct Tables {
+    int p = 0, S = 0;
+    std::vector<double> m2m;  // 8 * S * S
+    std::vector<double> l2l;  // 8 * S * S
+    std::vector<double> m2l;  // 343 * S * S
+};
+
+// Builds (and caches per p for the life of the process) the tables of order p.
+const Tables& tables_for_order(int p);
+
+}  // namespace afmm

@@ -1,0 +1,362 @@
+// ============================================================================
+// fmm.hip -- FP64 FMM passes on a built tree (AdaptiveFMM3D.hpp:44-108):
+//   P2M at leaves -> M2M bottom-up -> M2L (V) -> P2L (X) -> L2L top-down
+//   -> L2P + M2P (W) per body -> P2P (U) per leaf, acc = G*(far + near).
+// M2L and P2L only read multipoles, so all levels run in one launch each;
+// L2L then folds parents into children level by level, which gives the same
+// sum as the reference's per-box order (parents first, AdaptiveFMM3D.hpp:74).
+// Expansions use the scaled real basis of afmm_math.h; translations use the
+// level-independent tables of afmm_tables.cpp (column-major S x S).
+// ============================================================================
+#include <algorithm>
+
+#include "afmm_internal.h"
+#include "afmm_math.h"
+
+namespace afmm {
+namespace {
+
+constexpr int kT = 256;
+inline int blocks(long long n, int t = kT) { return (int)std::max<long long>(1, (n + t - 1) / t); }
+
+// out[r] (+)= sum_c T[c*S + r] * in[c] for one wave; in[] is broadcast.
+template <int S>
+__device__ __forceinline__ void wave_matvec_acc(const double* __restrict__ T, const double* __restrict__ in,
+                                                double (&acc)[(S + 63) / 64], int lane) {
+    for (int c = 0; c < S; ++c) {
+        const double v = in[c];
+        const double* col = T + (size_t)c * S;
+#pragma unroll
+        for (int k = 0; k < (S + 63) / 64; ++k) {
+            const int r = lane + 64 * k;
+            if (r < S) acc[k] += col[r] * v;
+        }
+    }
+}
+
+// Rows of per-body coefficient vectors staged in LDS (kept within 64 KB).
+template <int S>
+constexpr int lds_rows() { return 64 * S * 8 <= 65536 ? 64 : 32; }
+
+// ---- P2M: one wave per leaf; rows of q*conj(R) in LDS, column sums per lane.
+template <int P>
+__global__ void __launch_bounds__(64) k_p2m(TreeView t, double* __restrict__ M) {
+    constexpr int S = nterms(P);
+    constexpr int K = (S + 63) / 64;
+    constexpr int R = lds_rows<S>();
+    __shared__ double rows[R * S];
+    const int b = t.leaf_ids[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int f = t.seg_first[b], cnt = t.seg_count[b];
+    const double ch[6] = {t.cdd[6 * (size_t)b], t.cdd[6 * (size_t)b + 1], t.cdd[6 * (size_t)b + 2],
+                          t.cdd[6 * (size_t)b + 3], t.cdd[6 * (size_t)b + 4], t.cdd[6 * (size_t)b + 5]};
+    const double ih = t.invh[b];
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int base = 0; base < cnt; base += R) {
+        double* row = rows + lane * S;
+        const int i = f + base + lane;
+        if (lane < R && base + lane < cnt) {
+            const double dx = offset_from(t.x[i], Centre{ch[0], ch[1]}) * ih;
+            const double dy = offset_from(t.y[i], Centre{ch[2], ch[3]}) * ih;
+            const double dz = offset_from(t.z[i], Centre{ch[4], ch[5]}) * ih;
+            p2m_point<P>(t.m[i], dx, dy, dz, [&](int k, double v) { row[k] = v; });
+        }
+        __syncthreads();
+        const int rows_used = min(R, cnt - base);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int c = lane + 64 * k;
+            if (c < S)
+                for (int r = 0; r < rows_used; ++r) acc[k] += rows[r * S + c];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int c = lane + 64 * k;
+        if (c < S) M[(size_t)b * S + c] = acc[k];
+    }
+}
+
+// ---- M2M: one wave per parent of level l, 8 child-octant matrices.
+template <int P>
+__global__ void __launch_bounds__(64) k_m2m(TreeView t, int off, int cnt, const double* __restrict__ T,
+                                           double* __restrict__ M) {
+    constexpr int S = nterms(P);
+    constexpr int K = (S + 63) / 64;
+    const int b = off + blockIdx.x;
+    const int nch = t.nchild[b];
+    if (nch == 0) return;
+    const int lane = threadIdx.x;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int r = 0; r < nch; ++r) {
+        const int c = t.child_first[b] + r;
+        wave_matvec_acc<S>(T + (size_t)t.octant[c] * S * S, M + (size_t)c * S, acc, lane);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int r = lane + 64 * k;
+        if (r < S) M[(size_t)b * S + r] = acc[k];
+    }
+}
+
+// ---- M2L: one wave per target box, V list through the offset tables.
+template <int P>
+__global__ void __launch_bounds__(64) k_m2l(TreeView t, const double* __restrict__ T,
+                                           const double* __restrict__ M, double* __restrict__ L) {
+    constexpr int S = nterms(P);
+    constexpr int K = (S + 63) / 64;
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    const int nv = t.nv[b];
+    for (int e = 0; e < nv; ++e) {
+        const int c = t.vl[kVSlots * (size_t)b + e];
+        const int o = offset_index(t.ic[3 * (size_t)b] - t.ic[3 * (size_t)c], t.ic[3 * (size_t)b + 1] - t.ic[3 * (size_t)c + 1],
+                                   t.ic[3 * (size_t)b + 2] - t.ic[3 * (size_t)c + 2]);
+        wave_matvec_acc<S>(T + (size_t)o * S * S, M + (size_t)c * S, acc, lane);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int r = lane + 64 * k;
+        if (r < S) L[(size_t)b * S + r] = acc[k];
+    }
+}
+
+// ---- P2L: one wave per box with an X list; rows of q*conj(S) in LDS.
+template <int P>
+__global__ void __launch_bounds__(64) k_p2l(TreeView t, double* __restrict__ L) {
+    constexpr int S = nterms(P);
+    constexpr int K = (S + 63) / 64;
+    constexpr int R = lds_rows<S>();
+    __shared__ double rows[R * S];
+    const int b = blockIdx.x;
+    const int xa = t.x_off[b], xe = t.x_off[b + 1];
+    if (xa == xe) return;
+    const int lane = threadIdx.x;
+    const double ch[6] = {t.cdd[6 * (size_t)b], t.cdd[6 * (size_t)b + 1], t.cdd[6 * (size_t)b + 2],
+                          t.cdd[6 * (size_t)b + 3], t.cdd[6 * (size_t)b + 4], t.cdd[6 * (size_t)b + 5]};
+    const double ih = t.invh[b];
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int e = xa; e < xe; ++e) {
+        const int c = t.x_idx[e];
+        const int f = t.seg_first[c], cnt = t.seg_count[c];
+        for (int base = 0; base < cnt; base += R) {
+            double* row = rows + lane * S;
+            const int i = f + base + lane;
+            if (lane < R && base + lane < cnt) {
+                const double dx = offset_from(t.x[i], Centre{ch[0], ch[1]}) * ih;
+                const double dy = offset_from(t.y[i], Centre{ch[2], ch[3]}) * ih;
+                const double dz = offset_from(t.z[i], Centre{ch[4], ch[5]}) * ih;
+                p2l_point<P>(t.m[i], dx, dy, dz, [&](int k, double v) { row[k] = v; });
+            }
+            __syncthreads();
+            const int rows_used = min(R, cnt - base);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int cc = lane + 64 * k;
+                if (cc < S)
+                    for (int r = 0; r < rows_used; ++r) acc[k] += rows[r * S + cc];
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int r = lane + 64 * k;
+        if (r < S) L[(size_t)b * S + r] += acc[k];
+    }
+}
+
+// ---- L2L: one wave per box of level l, parent -> child octant matrix.
+template <int P>
+__global__ void __launch_bounds__(64) k_l2l(TreeView t, int off, const double* __restrict__ T,
+                                           double* __restrict__ L) {
+    constexpr int S = nterms(P);
+    constexpr int K = (S + 63) / 64;
+    const int b = off + blockIdx.x;
+    const int lane = threadIdx.x;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int r = lane + 64 * k;
+        acc[k] = (r < S) ? L[(size_t)b * S + r] : 0.0;
+    }
+    wave_matvec_acc<S>(T + (size_t)t.octant[b] * S * S, L + (size_t)t.parent[b] * S, acc, lane);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int r = lane + 64 * k;
+        if (r < S) L[(size_t)b * S + r] = acc[k];
+    }
+}
+
+// ---- L2P: one thread per sorted body (far field from its leaf's local).
+template <int P>
+__global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict__ L, double* __restrict__ far) {
+    constexpr int S = nterms(P);
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= t.n) return;
+    const int b = t.leaf_of[i];
+    const double xi = t.x[i], yi = t.y[i], zi = t.z[i];
+    auto dhat = [&](int box, double& dx, double& dy, double& dz) {
+        const double* c = t.cdd + 6 * (size_t)box;
+        const double ih = t.invh[box];
+        dx = offset_from(xi, Centre{c[0], c[1]}) * ih;
+        dy = offset_from(yi, Centre{c[2], c[3]}) * ih;
+        dz = offset_from(zi, Centre{c[4], c[5]}) * ih;
+    };
+    double dx, dy, dz, gx, gy, gz;
+    dhat(b, dx, dy, dz);
+    const double* lb = L + (size_t)b * S;
+    (void)S;
+    local_grad<P>([&](int k) { return lb[k]; }, dx, dy, dz, gx, gy, gz);
+    const double s2 = t.invh[b] * t.invh[b];
+    far[3 * (size_t)i] = gx * s2;
+    far[3 * (size_t)i + 1] = gy * s2;
+    far[3 * (size_t)i + 2] = gz * s2;
+}
+
+// ---- M2P over W: adds each W box's multipole gradient (W is per leaf rank).
+template <int P>
+__global__ void __launch_bounds__(kT) k_far_w(TreeView t, const int* __restrict__ leaf_rank,
+                                             const double* __restrict__ M, double* __restrict__ far) {
+    constexpr int S = nterms(P);
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= t.n) return;
+    const int li = leaf_rank[t.leaf_of[i]];
+    const int wa = t.w_off[li], we = t.w_off[li + 1];
+    if (wa == we) return;
+    const double xi = t.x[i], yi = t.y[i], zi = t.z[i];
+    double fx = far[3 * (size_t)i], fy = far[3 * (size_t)i + 1], fz = far[3 * (size_t)i + 2];
+    for (int e = wa; e < we; ++e) {
+        const int w = t.w_idx[e];
+        const double* c = t.cdd + 6 * (size_t)w;
+        const double ih = t.invh[w];
+        const double dx = offset_from(xi, Centre{c[0], c[1]}) * ih;
+        const double dy = offset_from(yi, Centre{c[2], c[3]}) * ih;
+        const double dz = offset_from(zi, Centre{c[4], c[5]}) * ih;
+        const double* mw = M + (size_t)w * S;
+        double gx, gy, gz;
+        multipole_grad<P>([&](int k) { return mw[k]; }, dx, dy, dz, gx, gy, gz);
+        const double s2 = ih * ih;
+        fx += gx * s2;
+        fy += gy * s2;
+        fz += gz * s2;
+    }
+    far[3 * (size_t)i] = fx;
+    far[3 * (size_t)i + 1] = fy;
+    far[3 * (size_t)i + 2] = fz;
+}
+
+// ---- P2P over U + final combine: one 64-thread block per target leaf,
+// 64-body source tiles staged in LDS; the self pair is skipped by index.
+__global__ void __launch_bounds__(64) k_p2p(TreeView t, double G, double eps2, const double* __restrict__ far,
+                                           double* __restrict__ acc) {
+    __shared__ double sx[64], sy[64], sz[64], sm[64];
+    const int li = blockIdx.x;
+    const int b = t.leaf_ids[li];
+    const int f = t.seg_first[b], cnt = t.seg_count[b];
+    const int lane = threadIdx.x;
+    const int ua = t.u_off[li], ue = t.u_off[li + 1];
+    for (int tb = 0; tb < cnt; tb += 64) {
+        const int i = f + tb + lane;
+        const bool act = tb + lane < cnt;
+        const double xi = act ? t.x[i] : 0.0, yi = act ? t.y[i] : 0.0, zi = act ? t.z[i] : 0.0;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int e = ua; e < ue; ++e) {
+            const int c = t.u_idx[e];
+            const int cf = t.seg_first[c], cc = t.seg_count[c];
+            for (int sbase = 0; sbase < cc; sbase += 64) {
+                const int j = cf + sbase + lane;
+                if (sbase + lane < cc) {
+                    sx[lane] = t.x[j];
+                    sy[lane] = t.y[j];
+                    sz[lane] = t.z[j];
+                    sm[lane] = t.m[j];
+                }
+                __syncthreads();
+                const int ns = min(64, cc - sbase);
+                for (int k = 0; k < ns; ++k) {
+                    const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+                    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                    const double s = sm[k] / (r2 * sqrt(r2));
+                    if (cf + sbase + k != i) {
+                        ax += s * dx;
+                        ay += s * dy;
+                        az += s * dz;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (act) {
+            const int o = t.orig[i];
+            acc[3 * (size_t)o] = G * (far[3 * (size_t)i] + ax);
+            acc[3 * (size_t)o + 1] = G * (far[3 * (size_t)i + 1] + ay);
+            acc[3 * (size_t)o + 2] = G * (far[3 * (size_t)i + 2] + az);
+        }
+    }
+}
+
+__global__ void k_leaf_rank(int nleaves, const int* leaf_ids, int* rank) {
+    const int li = blockIdx.x * kT + threadIdx.x;
+    if (li < nleaves) rank[leaf_ids[li]] = li;
+}
+
+template <int P>
+void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, double G,
+         double eps, double* acc, hipEvent_t* ev) {
+    constexpr int S = nterms(P);
+    const int nb = t.nb;
+    double* M = pool.get<double>("Mexp", (size_t)nb * S);
+    double* L = pool.get<double>("Lexp", (size_t)nb * S);
+    double* far = pool.get<double>("far", 3 * (size_t)t.n);
+    int* leaf_rank = pool.get<int>("leaf_rank", nb);
+    if (ev) AFMM_CHECK(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_p2m<P>, dim3(t.nleaves), dim3(64), 0, s, t, M);
+    for (int l = t.max_level - 1; l >= 0; --l) {
+        const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
+        hipLaunchKernelGGL(k_m2m<P>, dim3(cnt), dim3(64), 0, s, t, off, cnt, tab.m2m, M);
+    }
+    if (ev) AFMM_CHECK(hipEventRecord(ev[1], s));
+    hipLaunchKernelGGL(k_m2l<P>, dim3(nb), dim3(64), 0, s, t, tab.m2l, M, L);
+    if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
+    if (t.n_x > 0) hipLaunchKernelGGL(k_p2l<P>, dim3(nb), dim3(64), 0, s, t, L);
+    for (int l = 1; l <= t.max_level; ++l) {
+        const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
+        hipLaunchKernelGGL(k_l2l<P>, dim3(cnt), dim3(64), 0, s, t, off, tab.l2l, L);
+    }
+    if (ev) AFMM_CHECK(hipEventRecord(ev[3], s));
+    hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far);
+    if (t.n_w > 0) {
+        hipLaunchKernelGGL(k_leaf_rank, dim3(blocks(t.nleaves)), dim3(kT), 0, s, t.nleaves, t.leaf_ids, leaf_rank);
+        hipLaunchKernelGGL(k_far_w<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, leaf_rank, M, far);
+    }
+    hipLaunchKernelGGL(k_p2p, dim3(t.nleaves), dim3(64), 0, s, t, G, eps * eps, far, acc);
+    if (ev) AFMM_CHECK(hipEventRecord(ev[4], s));
+    AFMM_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
+             double G, double eps, double* d_acc, hipEvent_t* ev) {
+    switch (p) {
+#define AFMM_CASE(PP) \
+    case PP: run<PP>(pool, s, tv, meta, tab, G, eps, d_acc, ev); return;
+        AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
+        AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+#undef AFMM_CASE
+    }
+    throw std::runtime_error("unsupported expansion order");
+}
+
+}  // namespace afmm

@@ -1,0 +1,201 @@
+// ============================================================================
+// scan_sort.hip -- device-wide exclusive scan and a stable LSD radix sort with
+// 9-bit digits (three octree levels per pass), written for wave64.
+//
+// Sort pass = three kernels: per-tile digit histograms (LDS atomics on
+// integer counters), one exclusive scan over the digit-major histogram, and
+// a scatter that ranks every item stably inside its tile. In-tile ranks come
+// from wave ballots over the nine digit bits (lanes with the same digit form
+// a match mask; rank = popcount of lower lanes), then a per-digit prefix over
+// the four waves of the block. Everything is integer arithmetic: the result
+// is deterministic and identical to a sequential stable counting sort.
+// ============================================================================
+#include "afmm_internal.h"
+
+namespace afmm {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total) {
+    // lds: kScanThreads ints
+    const int tid = threadIdx.x;
+    lds[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int t = (tid >= off) ? lds[tid - off] : 0;
+        __syncthreads();
+        lds[tid] += t;
+        __syncthreads();
+    }
+    total = lds[kScanThreads - 1];
+    const int incl = lds[tid];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_tile_sums(const int* in, int n, int* sums) {
+    __shared__ int lds[kScanThreads];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += in[base + k];
+    int total;
+    block_exclusive_scan(s, lds, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One block scans `m` tile sums in chunks with a running carry.
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int* sums, int m, int* total_out) {
+    __shared__ int lds[kScanThreads];
+    int carry = 0;
+    for (int base = 0; base < m; base += kScanThreads) {
+        const int i = base + threadIdx.x;
+        const int v = (i < m) ? sums[i] : 0;
+        int total;
+        const int ex = block_exclusive_scan(v, lds, total);
+        if (i < m) sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int* in, int n, const int* sums,
+                                                             int* out) {
+    __shared__ int lds[kScanThreads];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int total;
+    int run = block_exclusive_scan(s, lds, total) + sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+// ---- radix sort -----------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr int kSortRounds = 8;
+constexpr int kSortTile = kSortThreads * kSortRounds;  // 2048 items per block
+constexpr int kBuckets = 512;
+
+__device__ __forceinline__ int digit_of(uint64_t khi, uint32_t klo, int g) {
+    return g == 0 ? int(klo & 511u) : int((khi >> (9 * (g - 1))) & 511u);
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* khi, const uint32_t* klo,
+                                                            int n, int g, int* hist, int nblocks) {
+    __shared__ int h[kBuckets];
+    for (int d = threadIdx.x; d < kBuckets; d += kSortThreads) h[d] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[digit_of(khi[i], klo[i], g)], 1);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kBuckets; d += kSortThreads) hist[d * nblocks + blockIdx.x] = h[d];
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* khi, const uint32_t* klo,
+                                                               const int* val, int n, int g,
+                                                               const int* offs, int nblocks,
+                                                               uint64_t* okhi, uint32_t* oklo, int* oval) {
+    __shared__ int run[kBuckets];
+    __shared__ int cnt[kSortWaves][kBuckets];
+    for (int d = threadIdx.x; d < kBuckets; d += kSortThreads) {
+        run[d] = offs[d * nblocks + blockIdx.x];
+        for (int w = 0; w < kSortWaves; ++w) cnt[w][d] = 0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int i = base + r * kSortThreads + threadIdx.x;
+        const bool act = i < n;
+        uint64_t kh = 0;
+        uint32_t kl = 0;
+        int v = 0, d = 0;
+        if (act) {
+            kh = khi[i];
+            kl = klo[i];
+            v = val[i];
+            d = digit_of(kh, kl, g);
+        }
+        uint64_t match = __ballot(act);
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t vote = __ballot(bit);
+            match &= bit ? vote : ~vote;
+        }
+        const int rank = __popcll(match & lt);
+        if (act && rank == 0) cnt[wave][d] = __popcll(match);
+        __syncthreads();
+        if (act) {
+            int pos = run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += cnt[w][d];
+            okhi[pos] = kh;
+            oklo[pos] = kl;
+            oval[pos] = v;
+        }
+        __syncthreads();
+        for (int dd = threadIdx.x; dd < kBuckets; dd += kSortThreads) {
+            int s = 0;
+            for (int w = 0; w < kSortWaves; ++w) {
+                s += cnt[w][dd];
+                cnt[w][dd] = 0;
+            }
+            run[dd] += s;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+int scan_tmp_ints(int n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void scan_exclusive(const int* d_in, int* d_out, int n, int* d_total, int* d_tmp, hipStream_t s) {
+    const int tiles = (n + kScanTile - 1) / kScanTile;
+    if (n <= 0) {
+        if (d_total) hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, 0, d_total);
+        return;
+    }
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, tiles, d_total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp, d_out);
+}
+
+int sort_blocks(int n) { return (n + kSortTile - 1) / kSortTile; }
+
+int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s) {
+    int cur = 0;
+    if (n <= 1) return 0;
+    const int nblocks = sort_blocks(n);
+    for (int k = 0; k < ndigits; ++k) {
+        const int g = first_digit + k;
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, s, b.khi[cur], b.klo[cur],
+                           n, g, b.hist, nblocks);
+        scan_exclusive(b.hist, b.hist, kBuckets * nblocks, nullptr, b.scan_tmp, s);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, b.khi[cur], b.klo[cur],
+                           b.val[cur], n, g, b.hist, nblocks, b.khi[1 - cur], b.klo[1 - cur], b.val[1 - cur]);
+        cur = 1 - cur;
+    }
+    return cur;
+}
+
+}  // namespace afmm

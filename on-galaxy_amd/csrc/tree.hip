@@ -1,0 +1,704 @@
+// ============================================================================
+// tree.hip -- device build of the adaptive octree and its CGR interaction
+// lists, bit-identical to AdaptiveOctree::build / buildLists
+// (AdaptiveOctree.hpp:65-189).
+//
+// Why it matches the recursive reference exactly:
+//  * Box centres are produced by the same IEEE operations along each path
+//    (c_child = c_parent +- 0.5*half, AdaptiveOctree.hpp:121-127), so a body
+//    that replays its own path from the root sees the very centres the
+//    reference compares it with (x >= cx, :109-111). The path is kept as a
+//    72-bit key of octant digits, level 1 most significant.
+//  * The reference's stable counting sort per split leaves the bodies of every
+//    box sorted by octant path, ties by input index. Pass 1 sorts by the key
+//    (stable LSD radix); the topology then follows from the sorted keys
+//    level by level (a box splits iff count > cap, 2*half > eps floor and
+//    level < 24, :102-104). Inside a leaf the reference keeps input order, so
+//    pass 2 re-sorts stably by leaf start.
+//  * Reference box ids come from its allocation order: children are numbered
+//    as a block when their parent is visited in DFS pre-order. With
+//    pre(b) = DFS pre-order rank, id(child r of P) = 1 + sum_{pre(Q) < pre(P)}
+//    nchild(Q) + r.
+//  * Lists use the reference's adjacency test on those centres
+//    (|dc| <= half_a + half_b + tol per axis, :58-62), and emit in the
+//    reference orders: colleagues and V ascending id (= Morton order in a
+//    level), U and W in DFS order, X ascending leaf id.
+// Internally boxes are numbered breadth-first (level-major, Morton inside a
+// level): children of a box are contiguous and each level is one range.
+// ============================================================================
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "afmm_internal.h"
+#include "afmm_math.h"
+
+#pragma clang fp contract(off)
+
+namespace afmm {
+namespace {
+
+constexpr int kT = 256;
+inline int blocks(long long n, int t = kT) { return (int)std::max<long long>(1, (n + t - 1) / t); }
+
+// ---- bounding cube ----------------------------------------------------------
+// Sequential std::min/std::max keep the first of equal values (only a strict
+// comparison replaces), which matters for the sign of a zero coordinate.
+struct Ext {
+    double v;
+    int i;
+};
+__device__ __forceinline__ void take_lo(Ext& a, Ext b) {
+    if (b.v < a.v || (b.v == a.v && b.i < a.i)) a = b;
+}
+__device__ __forceinline__ void take_hi(Ext& a, Ext b) {
+    if (b.v > a.v || (b.v == a.v && b.i < a.i)) a = b;
+}
+
+constexpr int kBBoxBlocks = 512;
+
+__global__ void __launch_bounds__(kT) k_bbox_partial(const double* pos, const double* mass, int n,
+                                                    double* pv, int* pi, int* pbad) {
+    Ext e[6];
+    for (int a = 0; a < 3; ++a) {
+        e[a] = Ext{1e300, -1};
+        e[3 + a] = Ext{-1e300, -1};
+    }
+    int bad = 0;
+    for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
+        for (int a = 0; a < 3; ++a) {
+            const double v = pos[3 * (size_t)i + a];
+            bad += !isfinite(v);
+            take_lo(e[a], Ext{v, i});
+            take_hi(e[3 + a], Ext{v, i});
+        }
+        bad += !isfinite(mass[i]);
+    }
+    __shared__ double sv[6][kT];
+    __shared__ int si[6][kT];
+    __shared__ int sb[kT];
+    for (int k = 0; k < 6; ++k) sv[k][threadIdx.x] = e[k].v, si[k][threadIdx.x] = e[k].i;
+    sb[threadIdx.x] = bad;
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const int o = threadIdx.x + off;
+            for (int k = 0; k < 6; ++k) {
+                Ext a{sv[k][threadIdx.x], si[k][threadIdx.x]}, b{sv[k][o], si[k][o]};
+                if (k < 3) take_lo(a, b);
+                else take_hi(a, b);
+                sv[k][threadIdx.x] = a.v, si[k][threadIdx.x] = a.i;
+            }
+            sb[threadIdx.x] += sb[o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; ++k) pv[k * gridDim.x + blockIdx.x] = sv[k][0], pi[k * gridDim.x + blockIdx.x] = si[k][0];
+        pbad[blockIdx.x] = sb[0];
+    }
+}
+
+// AdaptiveOctree::build (:71-81) from the partials; also the key depth.
+__global__ void k_bbox_final(const double* pv, const int* pi, const int* pbad, int nparts,
+                             double eps_floor, RootInfo* root) {
+    if (threadIdx.x != 0) return;
+    Ext e[6];
+    for (int a = 0; a < 3; ++a) e[a] = Ext{1e300, -1}, e[3 + a] = Ext{-1e300, -1};
+    int bad = 0;
+    for (int b = 0; b < nparts; ++b) {
+        for (int k = 0; k < 6; ++k) {
+            const Ext x{pv[k * nparts + b], pi[k * nparts + b]};
+            if (k < 3) take_lo(e[k], x);
+            else take_hi(e[k], x);
+        }
+        bad += pbad[b];
+    }
+    RootInfo r;
+    for (int a = 0; a < 3; ++a) r.c[a] = 0.5 * (e[a].v + e[3 + a].v);
+    double ext = e[3].v - e[0].v;
+    if (ext < e[4].v - e[1].v) ext = e[4].v - e[1].v;
+    if (ext < e[5].v - e[2].v) ext = e[5].v - e[2].v;
+    double half = 0.5 * ext;
+    if (!(half > 0)) half = 1.0;
+    half *= 1.0 + 1e-9;
+    r.half = half;
+    r.tol = 1e-9 * half;
+    // Levels 0..L-1 may split (2*half_l > eps floor, level < 24): keys carry L digits.
+    int L = 0;
+    double h = half;
+    while (L < kMaxDepth && 2.0 * h > eps_floor) {
+        ++L;
+        h = 0.5 * h;
+    }
+    r.key_levels = L;
+    r.nonfinite = bad;
+    *root = r;
+}
+
+// ---- path keys ----------------------------------------------------------------
+// digit of level l (1..24): levels 1..21 in khi bits [63-3l, 66-3l), 22..24 in klo.
+__device__ __forceinline__ int level_digit(uint64_t khi, uint32_t klo, int l) {
+    return l <= 21 ? int((khi >> (63 - 3 * l)) & 7u) : int((klo >> (3 * (24 - l))) & 7u);
+}
+
+__global__ void __launch_bounds__(kT) k_keys(const double* pos, int n, const RootInfo* rootp,
+                                            uint64_t* khi, uint32_t* klo, int* val) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    const RootInfo r = *rootp;
+    const double x = pos[3 * (size_t)i], y = pos[3 * (size_t)i + 1], z = pos[3 * (size_t)i + 2];
+    double cx = r.c[0], cy = r.c[1], cz = r.c[2], h = r.half;
+    uint64_t hi = 0;
+    uint32_t lo = 0;
+    for (int l = 1; l <= r.key_levels; ++l) {
+        const int o = (x >= cx ? 1 : 0) | (y >= cy ? 2 : 0) | (z >= cz ? 4 : 0);
+        h = 0.5 * h;
+        cx = cx + ((o & 1) ? h : -h);
+        cy = cy + ((o & 2) ? h : -h);
+        cz = cz + ((o & 4) ? h : -h);
+        if (l <= 21) hi |= uint64_t(o) << (63 - 3 * l);
+        else lo |= uint32_t(o) << (3 * (24 - l));
+    }
+    khi[i] = hi;
+    klo[i] = lo;
+    val[i] = i;
+}
+
+// ---- topology, one level at a time --------------------------------------------
+// For each box of level l: split decision and child ranges by binary search on
+// the level-(l+1) digit inside its (sorted) segment.
+__global__ void __launch_bounds__(kT) k_split(int off, int cnt, int l, int cap, double eps_floor,
+                                             const RootInfo* rootp, const int* seg_first,
+                                             const int* seg_count, const uint64_t* khi, const uint32_t* klo,
+                                             int* bnd, int* nch) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= cnt) return;
+    const int b = off + i;
+    const int f = seg_first[b], c = seg_count[b];
+    const double half = ldexp(rootp->half, -l);
+    int* bd = bnd + 9 * (size_t)i;
+    const bool split = c > cap && !(2.0 * half <= eps_floor) && l < kMaxDepth;
+    if (!split) {
+        nch[i] = 0;
+        return;
+    }
+    bd[0] = f;
+    bd[8] = f + c;
+    for (int o = 1; o < 8; ++o) {  // first index with digit >= o
+        int lo = bd[o - 1], hi = f + c;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (level_digit(khi[mid], klo[mid], l + 1) < o) lo = mid + 1;
+            else hi = mid;
+        }
+        bd[o] = lo;
+    }
+    int k = 0;
+    for (int o = 0; o < 8; ++o) k += bd[o + 1] > bd[o];
+    nch[i] = k;
+}
+
+__global__ void __launch_bounds__(kT) k_emit(int off, int cnt, int l, int next_off, const int* bnd,
+                                            const int* nch, const int* cbase, int* seg_first,
+                                            int* seg_count, int* level, int* parent, int* octant,
+                                            int* child_first, int* nchild, unsigned char* mask) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= cnt) return;
+    const int b = off + i;
+    const int k = nch[i];
+    nchild[b] = k;
+    if (k == 0) {
+        child_first[b] = -1;
+        mask[b] = 0;
+        return;
+    }
+    const int* bd = bnd + 9 * (size_t)i;
+    const int first = next_off + cbase[i];
+    child_first[b] = first;
+    unsigned char mk = 0;
+    int r = 0;
+    for (int o = 0; o < 8; ++o) {
+        if (bd[o + 1] <= bd[o]) continue;
+        const int c = first + r++;
+        mk |= (unsigned char)(1u << o);
+        seg_first[c] = bd[o];
+        seg_count[c] = bd[o + 1] - bd[o];
+        level[c] = l + 1;
+        parent[c] = b;
+        octant[c] = o;
+    }
+    mask[b] = mk;
+}
+
+__global__ void k_init_root(int n, int* seg_first, int* seg_count, int* level, int* parent, int* octant) {
+    if (threadIdx.x != 0) return;
+    seg_first[0] = 0;
+    seg_count[0] = n;
+    level[0] = 0;
+    parent[0] = -1;
+    octant[0] = 0;
+}
+
+// ---- reference numbering -------------------------------------------------------
+__global__ void k_fill(int* a, int n, int v) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(kT) k_count_starts(int nb, const int* seg_first, const int* level,
+                                                    int* cnt, int* minlev) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    atomicAdd(&cnt[seg_first[b]], 1);
+    atomicMin(&minlev[seg_first[b]], level[b]);
+}
+
+__global__ void __launch_bounds__(kT) k_preorder(int nb, const int* seg_first, const int* level,
+                                                const int* start_scan, const int* minlev,
+                                                const int* nchild, int* pre, int* nch_by_pre) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    const int s = seg_first[b];
+    const int p = start_scan[s] + level[b] - minlev[s];
+    pre[b] = p;
+    nch_by_pre[p] = nchild[b];
+}
+
+__global__ void __launch_bounds__(kT) k_refid(int nb, const int* pre, const int* base_by_pre,
+                                             const int* child_first, const int* nchild, int* refid) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    if (b == 0) refid[0] = 0;
+    const int k = nchild[b];
+    const int first = 1 + base_by_pre[pre[b]];
+    for (int r = 0; r < k; ++r) refid[child_first[b] + r] = first + r;
+}
+
+// ---- leaves and the in-leaf input order ------------------------------------------
+__global__ void __launch_bounds__(kT) k_leaf_flags(int nb, const int* nchild, int* flag) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b < nb) flag[b] = nchild[b] == 0;
+}
+__global__ void __launch_bounds__(kT) k_leaf_ids(int nb, const int* nchild, const int* scan, int* ids) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b < nb && nchild[b] == 0) ids[scan[b]] = b;
+}
+// key2[input index] = first sorted slot of its leaf; leaf_of[sorted slot] = leaf box.
+__global__ void __launch_bounds__(kT) k_leaf_keys(int nleaves, const int* leaf_ids, const int* seg_first,
+                                                 const int* seg_count, const int* perm1, uint64_t* key2,
+                                                 uint32_t* klo2, int* val2, int* leaf_of) {
+    const int li = blockIdx.x * kT + threadIdx.x;
+    if (li >= nleaves) return;
+    const int b = leaf_ids[li];
+    const int f = seg_first[b], c = seg_count[b];
+    for (int k = f; k < f + c; ++k) {
+        const int j = perm1[k];
+        key2[j] = (uint64_t)f;
+        klo2[j] = 0;
+        val2[j] = j;
+        leaf_of[k] = b;
+    }
+}
+
+__global__ void __launch_bounds__(kT) k_gather(int n, const int* perm, const double* pos, const double* mass,
+                                              double* x, double* y, double* z, double* m, int* orig) {
+    const int k = blockIdx.x * kT + threadIdx.x;
+    if (k >= n) return;
+    const int j = perm[k];
+    x[k] = pos[3 * (size_t)j];
+    y[k] = pos[3 * (size_t)j + 1];
+    z[k] = pos[3 * (size_t)j + 2];
+    m[k] = mass[j];
+    orig[k] = j;
+}
+
+// ---- box geometry -----------------------------------------------------------------
+// Replays the reference's centre arithmetic along the box's path (digits of
+// any of its bodies' keys), and derives the exact centre and integer coords.
+__global__ void __launch_bounds__(kT) k_geometry(int nb, const RootInfo* rootp, const int* seg_first,
+                                                const int* level, const uint64_t* khi, const uint32_t* klo,
+                                                double* cfp, double* cdd, double* half, double* invh, int* ic) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    const RootInfo r = *rootp;
+    const int L = level[b];
+    const uint64_t kh = khi[seg_first[b]];
+    const uint32_t kl = klo[seg_first[b]];
+    double c[3] = {r.c[0], r.c[1], r.c[2]}, h = r.half;
+    int ii[3] = {0, 0, 0};
+    for (int l = 1; l <= L; ++l) {
+        const int o = level_digit(kh, kl, l);
+        h = 0.5 * h;
+        for (int a = 0; a < 3; ++a) {
+            const int bit = (o >> a) & 1;
+            c[a] = c[a] + (bit ? h : -h);
+            ii[a] = 2 * ii[a] + bit;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        cfp[3 * (size_t)b + a] = c[a];
+        const Centre e = exact_centre(r.c[a], r.half, L, ii[a]);
+        cdd[6 * (size_t)b + 2 * a] = e.hi;
+        cdd[6 * (size_t)b + 2 * a + 1] = e.lo;
+        ic[3 * (size_t)b + a] = ii[a];
+    }
+    half[b] = h;
+    invh[b] = ldexp(1.0 / r.half, L);
+}
+
+// ---- lists ------------------------------------------------------------------------
+__device__ __forceinline__ bool adjacent(const double* cfp, const double* half, double tol, int a, int b) {
+    const double reach = half[a] + half[b] + tol;
+    return fabs(cfp[3 * (size_t)a] - cfp[3 * (size_t)b]) <= reach &&
+           fabs(cfp[3 * (size_t)a + 1] - cfp[3 * (size_t)b + 1]) <= reach &&
+           fabs(cfp[3 * (size_t)a + 2] - cfp[3 * (size_t)b + 2]) <= reach;
+}
+
+// Colleagues and V of every box of one level (AdaptiveOctree.hpp:143-161):
+// children of the parent's colleagues, colleague order then octant order.
+__global__ void __launch_bounds__(kT) k_coll_v(int off, int cnt, double tol, const int* parent,
+                                              const int* child_first, const unsigned char* mask,
+                                              const double* cfp, const double* half, const int* ic,
+                                              int* coll, int* ncoll, int* vl, int* nv, int* src_at,
+                                              int* overflow) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= cnt) return;
+    const int b = off + i;
+    if (b == 0) {
+        coll[0] = 0;
+        ncoll[0] = 1;
+        nv[0] = 0;
+        return;
+    }
+    const int par = parent[b];
+    int kc = 0, kv = 0;
+    for (int s = 0; s < ncoll[par]; ++s) {
+        const int pc = coll[kCollSlots * (size_t)par + s];
+        const unsigned mk = mask[pc];
+        int r = 0;
+        for (int o = 0; o < 8; ++o) {
+            if (!((mk >> o) & 1)) continue;
+            const int c = child_first[pc] + r++;
+            if (adjacent(cfp, half, tol, b, c)) {
+                if (kc < kCollSlots) coll[kCollSlots * (size_t)b + kc] = c;
+                else atomicOr(overflow, 1);
+                ++kc;
+            } else {
+                if (kv < kVSlots) {
+                    vl[kVSlots * (size_t)b + kv] = c;
+                    const int dx = ic[3 * (size_t)b] - ic[3 * (size_t)c];
+                    const int dy = ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1];
+                    const int dz = ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2];
+                    if (dx < -3 || dx > 3 || dy < -3 || dy > 3 || dz < -3 || dz > 3) atomicOr(overflow, 2);
+                    else src_at[kNumOffsets * (size_t)b + offset_index(dx, dy, dz)] = c;
+                } else {
+                    atomicOr(overflow, 1);
+                }
+                ++kv;
+            }
+        }
+    }
+    ncoll[b] = kc < kCollSlots ? kc : kCollSlots;
+    nv[b] = kv < kVSlots ? kv : kVSlots;
+}
+
+// Next box in DFS order after finishing `node`'s subtree, or -1 on reaching `stop`.
+__device__ __forceinline__ int dfs_next(int node, int stop, const int* parent, const int* child_first,
+                                        const int* nchild) {
+    while (node != stop) {
+        const int par = parent[node];
+        if (node + 1 < child_first[par] + nchild[par]) return node + 1;
+        node = par;
+    }
+    return -1;
+}
+
+// U(b): non-empty leaves adjacent to b, DFS from the root (:164, :176-180).
+// mode 0 counts, mode 1 writes.
+__global__ void __launch_bounds__(kT) k_ulist(int nleaves, const int* leaf_ids, double tol, const int* parent,
+                                             const int* child_first, const int* nchild, const double* cfp,
+                                             const double* half, int mode, int* cnt, const int* off, int* out) {
+    const int li = blockIdx.x * kT + threadIdx.x;
+    if (li >= nleaves) return;
+    const int b = leaf_ids[li];
+    int k = 0;
+    int* dst = mode ? out + off[li] : nullptr;
+    int node = 0;
+    while (node >= 0) {
+        if (adjacent(cfp, half, tol, node, b)) {
+            if (nchild[node] == 0) {
+                if (mode) dst[k] = node;
+                ++k;
+                node = dfs_next(node, 0, parent, child_first, nchild);
+            } else {
+                node = child_first[node];
+            }
+        } else {
+            node = dfs_next(node, 0, parent, child_first, nchild);
+        }
+    }
+    if (!mode) cnt[li] = k;
+}
+
+// W(b): for each non-leaf colleague c != b, DFS through adjacent non-leaf
+// boxes of c's subtree, emitting separated boxes (:167-169, :181-189).
+__global__ void __launch_bounds__(kT) k_wlist(int nleaves, const int* leaf_ids, double tol, const int* parent,
+                                             const int* child_first, const int* nchild, const double* cfp,
+                                             const double* half, const int* coll, const int* ncoll, int mode,
+                                             int* cnt, const int* off, int* out) {
+    const int li = blockIdx.x * kT + threadIdx.x;
+    if (li >= nleaves) return;
+    const int b = leaf_ids[li];
+    int k = 0;
+    int* dst = mode ? out + off[li] : nullptr;
+    for (int s = 0; s < ncoll[b]; ++s) {
+        const int c = coll[kCollSlots * (size_t)b + s];
+        if (c == b || nchild[c] == 0) continue;
+        int node = child_first[c];
+        while (node >= 0) {
+            if (!adjacent(cfp, half, tol, node, b)) {
+                if (mode) dst[k] = node;
+                ++k;
+                node = dfs_next(node, c, parent, child_first, nchild);
+            } else if (nchild[node] > 0) {
+                node = child_first[node];
+            } else {
+                node = dfs_next(node, c, parent, child_first, nchild);
+            }
+        }
+    }
+    if (!mode) cnt[li] = k;
+}
+
+// X = transpose of W (:172-173).
+__global__ void __launch_bounds__(kT) k_xcount(int nleaves, const int* w_off, const int* w_idx, int* xcnt) {
+    const int li = blockIdx.x * kT + threadIdx.x;
+    if (li >= nleaves) return;
+    for (int e = w_off[li]; e < w_off[li + 1]; ++e) atomicAdd(&xcnt[w_idx[e]], 1);
+}
+__global__ void __launch_bounds__(kT) k_xfill(int nleaves, const int* leaf_ids, const int* w_off,
+                                             const int* w_idx, const int* x_off, int* xcur, int* x_idx) {
+    const int li = blockIdx.x * kT + threadIdx.x;
+    if (li >= nleaves) return;
+    const int b = leaf_ids[li];
+    for (int e = w_off[li]; e < w_off[li + 1]; ++e) {
+        const int w = w_idx[e];
+        x_idx[x_off[w] + atomicAdd(&xcur[w], 1)] = b;
+    }
+}
+// Each X list in ascending reference id of its leaves.
+__global__ void __launch_bounds__(kT) k_xsort(int nb, const int* x_off, const int* refid, int* x_idx) {
+    const int w = blockIdx.x * kT + threadIdx.x;
+    if (w >= nb) return;
+    const int a = x_off[w], e = x_off[w + 1];
+    for (int i = a + 1; i < e; ++i) {
+        const int v = x_idx[i], key = refid[v];
+        int j = i - 1;
+        while (j >= a && refid[x_idx[j]] > key) {
+            x_idx[j + 1] = x_idx[j];
+            --j;
+        }
+        x_idx[j + 1] = v;
+    }
+}
+
+int read_int(const int* d, hipStream_t s) {
+    int h = 0;
+    AFMM_CHECK(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, s));
+    AFMM_CHECK(hipStreamSynchronize(s));
+    return h;
+}
+
+}  // namespace
+
+void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int n, int cap,
+                double eps_floor, TreeView& tv, TreeMeta& meta, RootInfo& root) {
+    // ---- root cube
+    double* pv = pool.get<double>("bbox_v", 6 * kBBoxBlocks);
+    int* pi = pool.get<int>("bbox_i", 6 * kBBoxBlocks);
+    int* pbad = pool.get<int>("bbox_bad", kBBoxBlocks);
+    RootInfo* d_root = pool.get<RootInfo>("root", 1);
+    const int nbb = std::min(kBBoxBlocks, blocks(n));
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kT), 0, s, d_pos, d_mass, n, pv, pi, pbad);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, pv, pi, pbad, nbb, eps_floor, d_root);
+    AFMM_CHECK(hipMemcpyAsync(&root, d_root, sizeof(RootInfo), hipMemcpyDeviceToHost, s));
+    AFMM_CHECK(hipStreamSynchronize(s));
+    if (root.nonfinite) return;
+    tv.n = n;
+    tv.tol = root.tol;
+
+    // ---- pass 1: sort bodies by octant path
+    const int nsb = sort_blocks(n);
+    SortBufs sb;
+    for (int k = 0; k < 2; ++k) {
+        sb.khi[k] = pool.get<uint64_t>(k ? "khi1" : "khi0", n);
+        sb.klo[k] = pool.get<uint32_t>(k ? "klo1" : "klo0", n);
+        sb.val[k] = pool.get<int>(k ? "val1" : "val0", n);
+    }
+    sb.hist = pool.get<int>("sort_hist", 512 * (size_t)nsb);
+    const int scan_cap = std::max(n, 512 * nsb) + 1;
+    int* scan_tmp = pool.get<int>("scan_tmp", scan_tmp_ints(scan_cap) + 16);
+    sb.scan_tmp = scan_tmp;
+    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kT), 0, s, d_pos, n, d_root, sb.khi[0], sb.klo[0], sb.val[0]);
+    const int L = root.key_levels;
+    const int ndig = (L + 2) / 3;
+    const int r1 = radix_sort(sb, n, 8 - ndig, ndig, s);
+    // keep pass-1 keys/perm: copy into stable names
+    uint64_t* khi = pool.get<uint64_t>("key1_hi", n);
+    uint32_t* klo = pool.get<uint32_t>("key1_lo", n);
+    int* perm1 = pool.get<int>("perm1", n);
+    AFMM_CHECK(hipMemcpyAsync(khi, sb.khi[r1], sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
+    AFMM_CHECK(hipMemcpyAsync(klo, sb.klo[r1], sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    AFMM_CHECK(hipMemcpyAsync(perm1, sb.val[r1], sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+
+    // ---- topology, level by level
+    size_t cap_boxes = std::max<size_t>(1024, (size_t)n / 4 + 64);
+    auto box_arrays = [&](size_t c, bool keep) {
+        tv.seg_first = pool.get<int>("seg_first", c, s, keep);
+        tv.seg_count = pool.get<int>("seg_count", c, s, keep);
+        tv.level = pool.get<int>("level", c, s, keep);
+        tv.parent = pool.get<int>("parent", c, s, keep);
+        tv.octant = pool.get<int>("octant", c, s, keep);
+        tv.child_first = pool.get<int>("child_first", c, s, keep);
+        tv.nchild = pool.get<int>("nchild", c, s, keep);
+        tv.child_mask = pool.get<unsigned char>("child_mask", c, s, keep);
+    };
+    box_arrays(cap_boxes, false);
+    hipLaunchKernelGGL(k_init_root, dim3(1), dim3(64), 0, s, n, tv.seg_first, tv.seg_count, tv.level, tv.parent,
+                       tv.octant);
+    int* d_total = pool.get<int>("total", 4);
+    meta.level_off.assign(1, 0);
+    meta.level_off.push_back(1);
+    int l = 0;
+    for (;; ++l) {
+        const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
+        int* bnd = pool.get<int>("split_bnd", 9 * (size_t)cnt);
+        int* nch = pool.get<int>("split_nch", cnt);
+        int* cbase = pool.get<int>("split_base", cnt);
+        hipLaunchKernelGGL(k_split, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, l, cap, eps_floor, d_root,
+                           tv.seg_first, tv.seg_count, khi, klo, bnd, nch);
+        scan_exclusive(nch, cbase, cnt, d_total, scan_tmp, s);
+        const int next = read_int(d_total, s);
+        const int next_off = off + cnt;
+        if ((size_t)(next_off + next) > cap_boxes) {
+            cap_boxes = std::max(cap_boxes * 2, (size_t)(next_off + next) + 1024);
+            box_arrays(cap_boxes, true);
+        }
+        hipLaunchKernelGGL(k_emit, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, l, next_off, bnd, nch, cbase,
+                           tv.seg_first, tv.seg_count, tv.level, tv.parent, tv.octant, tv.child_first,
+                           tv.nchild, tv.child_mask);
+        if (next == 0) break;
+        meta.level_off.push_back(next_off + next);
+    }
+    tv.max_level = l;
+    const int nb = meta.level_off.back();
+    tv.nb = nb;
+
+    // ---- reference box ids
+    int* cnt_start = pool.get<int>("cnt_start", n + 1);
+    int* minlev = pool.get<int>("minlev", n + 1);
+    int* start_scan = pool.get<int>("start_scan", n + 1);
+    int* pre = pool.get<int>("pre", nb);
+    int* nch_pre = pool.get<int>("nch_pre", nb);
+    int* base_pre = pool.get<int>("base_pre", nb);
+    tv.refid = pool.get<int>("refid", nb);
+    hipLaunchKernelGGL(k_fill, dim3(blocks(n)), dim3(kT), 0, s, cnt_start, n, 0);
+    hipLaunchKernelGGL(k_fill, dim3(blocks(n)), dim3(kT), 0, s, minlev, n, INT_MAX);
+    hipLaunchKernelGGL(k_count_starts, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.seg_first, tv.level, cnt_start,
+                       minlev);
+    scan_exclusive(cnt_start, start_scan, n, nullptr, scan_tmp, s);
+    hipLaunchKernelGGL(k_preorder, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.seg_first, tv.level, start_scan,
+                       minlev, tv.nchild, pre, nch_pre);
+    scan_exclusive(nch_pre, base_pre, nb, nullptr, scan_tmp, s);
+    hipLaunchKernelGGL(k_refid, dim3(blocks(nb)), dim3(kT), 0, s, nb, pre, base_pre, tv.child_first, tv.nchild,
+                       tv.refid);
+
+    // ---- leaves, pass 2 (input order inside each leaf), gather
+    int* flags = pool.get<int>("leaf_flag", nb);
+    int* fscan = pool.get<int>("leaf_scan", nb);
+    hipLaunchKernelGGL(k_leaf_flags, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, flags);
+    scan_exclusive(flags, fscan, nb, d_total, scan_tmp, s);
+    tv.nleaves = read_int(d_total, s);
+    tv.leaf_ids = pool.get<int>("leaf_ids", tv.nleaves);
+    hipLaunchKernelGGL(k_leaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.leaf_ids);
+    tv.leaf_of = pool.get<int>("leaf_of", n);
+    hipLaunchKernelGGL(k_leaf_keys, dim3(blocks(tv.nleaves)), dim3(kT), 0, s, tv.nleaves, tv.leaf_ids,
+                       tv.seg_first, tv.seg_count, perm1, sb.khi[0], sb.klo[0], sb.val[0], tv.leaf_of);
+    int bits = 1;
+    while ((1LL << bits) <= n) ++bits;
+    const int r2 = radix_sort(sb, n, 1, (bits + 8) / 9, s);
+    tv.x = pool.get<double>("bx", n);
+    tv.y = pool.get<double>("by", n);
+    tv.z = pool.get<double>("bz", n);
+    tv.m = pool.get<double>("bm", n);
+    tv.orig = pool.get<int>("orig", n);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, n, sb.val[r2], d_pos, d_mass, tv.x, tv.y, tv.z,
+                       tv.m, tv.orig);
+
+    // ---- geometry
+    tv.cfp = pool.get<double>("cfp", 3 * (size_t)nb);
+    tv.cdd = pool.get<double>("cdd", 6 * (size_t)nb);
+    tv.half = pool.get<double>("half", nb);
+    tv.invh = pool.get<double>("invh", nb);
+    tv.ic = pool.get<int>("ic", 3 * (size_t)nb);
+    hipLaunchKernelGGL(k_geometry, dim3(blocks(nb)), dim3(kT), 0, s, nb, d_root, tv.seg_first, tv.level, khi, klo,
+                       tv.cfp, tv.cdd, tv.half, tv.invh, tv.ic);
+
+    // ---- colleagues, V, M2L source table
+    tv.coll = pool.get<int>("coll", kCollSlots * (size_t)nb);
+    tv.ncoll = pool.get<int>("ncoll", nb);
+    tv.vl = pool.get<int>("vl", kVSlots * (size_t)nb);
+    tv.nv = pool.get<int>("nv", nb);
+    tv.src_at = pool.get<int>("src_at", kNumOffsets * (size_t)nb);
+    int* overflow = pool.get<int>("overflow", 1);
+    hipLaunchKernelGGL(k_fill, dim3(blocks((long long)kNumOffsets * nb)), dim3(kT), 0, s, tv.src_at,
+                       kNumOffsets * nb, -1);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, overflow, 1, 0);
+    for (int lv = 0; lv <= tv.max_level; ++lv) {
+        const int off = meta.level_off[lv], cnt = meta.level_off[lv + 1] - off;
+        hipLaunchKernelGGL(k_coll_v, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, root.tol, tv.parent,
+                           tv.child_first, tv.child_mask, tv.cfp, tv.half, tv.ic, tv.coll, tv.ncoll, tv.vl,
+                           tv.nv, tv.src_at, overflow);
+    }
+    if (read_int(overflow, s) != 0) throw std::runtime_error("interaction-list slot overflow");
+
+    // ---- U and W (count, scan, fill), X transpose
+    const int nl = tv.nleaves;
+    int* ucnt = pool.get<int>("ucnt", nl + 1);
+    tv.u_off = pool.get<int>("u_off", nl + 1);
+    hipLaunchKernelGGL(k_ulist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
+                       tv.child_first, tv.nchild, tv.cfp, tv.half, 0, ucnt, nullptr, nullptr);
+    scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
+    tv.n_u = read_int(tv.u_off + nl, s);
+    tv.u_idx = pool.get<int>("u_idx", tv.n_u);
+    hipLaunchKernelGGL(k_ulist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
+                       tv.child_first, tv.nchild, tv.cfp, tv.half, 1, nullptr, tv.u_off, tv.u_idx);
+
+    int* wcnt = pool.get<int>("wcnt", nl + 1);
+    tv.w_off = pool.get<int>("w_off", nl + 1);
+    hipLaunchKernelGGL(k_wlist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
+                       tv.child_first, tv.nchild, tv.cfp, tv.half, tv.coll, tv.ncoll, 0, wcnt, nullptr, nullptr);
+    scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp, s);
+    tv.n_w = read_int(tv.w_off + nl, s);
+    tv.w_idx = pool.get<int>("w_idx", tv.n_w);
+    hipLaunchKernelGGL(k_wlist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
+                       tv.child_first, tv.nchild, tv.cfp, tv.half, tv.coll, tv.ncoll, 1, nullptr, tv.w_off,
+                       tv.w_idx);
+
+    int* xcnt = pool.get<int>("xcnt", nb + 1);
+    int* xcur = pool.get<int>("xcur", nb + 1);
+    tv.x_off = pool.get<int>("x_off", nb + 1);
+    hipLaunchKernelGGL(k_fill, dim3(blocks(nb)), dim3(kT), 0, s, xcnt, nb, 0);
+    hipLaunchKernelGGL(k_fill, dim3(blocks(nb)), dim3(kT), 0, s, xcur, nb, 0);
+    hipLaunchKernelGGL(k_xcount, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.w_off, tv.w_idx, xcnt);
+    scan_exclusive(xcnt, tv.x_off, nb, tv.x_off + nb, scan_tmp, s);
+    tv.n_x = read_int(tv.x_off + nb, s);
+    tv.x_idx = pool.get<int>("x_idx", tv.n_x);
+    hipLaunchKernelGGL(k_xfill, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, tv.w_off, tv.w_idx, tv.x_off,
+                       xcur, tv.x_idx);
+    hipLaunchKernelGGL(k_xsort, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.x_off, tv.refid, tv.x_idx);
+    AFMM_CHECK(hipGetLastError());
+}
+
+}  // namespace afmm

@@ -166,6 +166,27 @@ int orc_direct(int64_t n, const double* pos, const double* mass, double G, doubl
     return 0;
 }
 
+// DirectSolver3D's pair formula (DirectSolver3D.hpp:25-30) for a subset of
+// targets: acc[t] = G sum_{j != idx[t]} m_j d / (|d|^2 + eps^2)^{3/2}.
+int orc_direct_targets(int64_t n, const double* pos, const double* mass, double G, double eps, int64_t nt,
+                       const int64_t* idx, double* acc) {
+    const double eps2 = eps * eps;
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t i = idx[t];
+        double ax = 0, ay = 0, az = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                         dz = pos[3 * j + 2] - pos[3 * i + 2];
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double s = mass[j] / (r2 * std::sqrt(r2));
+            ax += s * dx, ay += s * dy, az += s * dz;
+        }
+        acc[3 * t] = G * ax, acc[3 * t + 1] = G * ay, acc[3 * t + 2] = G * az;
+    }
+    return 0;
+}
+
 void* orc_tree_new(int64_t n, const double* pos, const double* mass, int cap, double epsFloor) {
     Tree* t = new Tree();
     t->build(pos, mass, (std::size_t)n, cap, epsFloor);

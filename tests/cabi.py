@@ -189,6 +189,20 @@ def reference():
     return _cache["ref"]
 
 
+def direct_targets(pos, mass, idx, G=1.0, eps=0.0):
+    """FP64 direct sum on a subset of targets (oracle restatement)."""
+    lib = oracle().lib
+    lib.orc_direct_targets.argtypes = [C.c_int64, _dp, _dp, C.c_double, C.c_double, C.c_int64,
+                                       C.POINTER(C.c_int64), _dp]
+    pos = np.ascontiguousarray(pos, np.float64)
+    mass = np.ascontiguousarray(mass, np.float64)
+    idx = np.ascontiguousarray(idx, np.int64)
+    acc = np.empty((len(idx), 3))
+    lib.orc_direct_targets(len(mass), ptr(pos), ptr(mass), G, eps, len(idx),
+                           idx.ctypes.data_as(C.POINTER(C.c_int64)), ptr(acc))
+    return acc
+
+
 def operator_convergence(dir_first):
     lib = oracle().lib
     lib.orc_operator_convergence.argtypes = [C.c_int, _dp]
@@ -212,3 +226,28 @@ def uniform_cube(n, seed):
     """Deterministic cube [-1,1]^3 with masses in [0.5,1.5] (numpy stream)."""
     rng = np.random.default_rng(seed)
     return rng.uniform(-1, 1, (n, 3)), rng.uniform(0.5, 1.5, n)
+
+
+def product():
+    """The product binding (on-galaxy_amd/afmm.py) loaded by path."""
+    import importlib.util
+    if "product" not in _cache:
+        spec = importlib.util.spec_from_file_location(
+            "afmm_product", os.path.join(ROOT, "on-galaxy_amd", "afmm.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _cache["product"] = mod
+    return _cache["product"]
+
+
+def model_fmm(pos, mass, G, eps, p, cap):
+    """Host model of the HIP arithmetic (tests/model/libmodel.so)."""
+    if "model" not in _cache:
+        lib = C.CDLL(os.path.join(ROOT, "tests", "model", "libmodel.so"))
+        lib.model_fmm.argtypes = [C.c_int64, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]
+        _cache["model"] = lib
+    pos = np.ascontiguousarray(pos, np.float64)
+    mass = np.ascontiguousarray(mass, np.float64)
+    acc = np.empty_like(pos)
+    assert _cache["model"].model_fmm(len(mass), ptr(pos), ptr(mass), G, eps, p, cap, ptr(acc)) == 0
+    return acc

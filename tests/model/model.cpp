@@ -1,0 +1,157 @@
+// ============================================================================
+// model.cpp -- TEST INFRASTRUCTURE ONLY. A host-side model of the HIP
+// library's arithmetic: the oracle's tree, then the product's scaled real-basis
+// operators (on-galaxy_amd/csrc/afmm_math.h) and translation tables
+// (afmm_tables.cpp) in plain loops. It checks that the device formulation
+// (exact double-double centres, level-independent matrices, m >= 0 gradient
+// folding) reproduces the reference algorithm before any kernel runs.
+// Built by __graft_entry__.build() into tests/model/libmodel.so.
+// ============================================================================
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../oracle/oracle.hpp"
+#include "../../on-galaxy_amd/csrc/afmm_math.h"
+#include "../../on-galaxy_amd/csrc/afmm_tables.h"
+
+using namespace afmm;
+
+namespace {
+
+template <int P>
+void model_fmm(const oracle::Tree& t, double G, double eps, double* acc) {
+    constexpr int S = nterms(P);
+    const Tables& T = tables_for_order(P);
+    const int nb = (int)t.box.size();
+    const double croot[3] = {t.box[0].cx, t.box[0].cy, t.box[0].cz};
+    const double h0 = t.box[0].half;
+    std::vector<int> ic(3 * nb, 0), oct(nb, 0);
+    for (int b = 0; b < nb; ++b)
+        for (int o = 0; o < 8; ++o) {
+            const int c = t.box[b].child[o];
+            if (c < 0) continue;
+            oct[c] = o;
+            ic[3 * c] = 2 * ic[3 * b] + (o & 1);
+            ic[3 * c + 1] = 2 * ic[3 * b + 1] + ((o >> 1) & 1);
+            ic[3 * c + 2] = 2 * ic[3 * b + 2] + ((o >> 2) & 1);
+        }
+    std::vector<Centre> cen(3 * nb);
+    std::vector<double> invh(nb);
+    for (int b = 0; b < nb; ++b) {
+        for (int a = 0; a < 3; ++a) cen[3 * b + a] = exact_centre(croot[a], h0, t.box[b].level, ic[3 * b + a]);
+        invh[b] = std::ldexp(1.0 / h0, t.box[b].level);
+    }
+    std::vector<double> M((size_t)nb * S, 0.0), L((size_t)nb * S, 0.0);
+    auto dhat = [&](int b, double x, double y, double z, double* d) {
+        d[0] = offset_from(x, cen[3 * b]) * invh[b];
+        d[1] = offset_from(y, cen[3 * b + 1]) * invh[b];
+        d[2] = offset_from(z, cen[3 * b + 2]) * invh[b];
+    };
+    auto matvec = [&](const double* Tm, const double* in, double* out) {
+        for (int c = 0; c < S; ++c)
+            for (int r = 0; r < S; ++r) out[r] += Tm[(size_t)c * S + r] * in[c];
+    };
+    for (int b : t.leaves) {
+        double* mb = &M[(size_t)b * S];
+        for (int k = t.box[b].pfirst; k < t.box[b].pfirst + t.box[b].pcount; ++k) {
+            double d[3];
+            dhat(b, t.px[k], t.py[k], t.pz[k], d);
+            p2m_point<P>(t.m[k], d[0], d[1], d[2], [&](int i, double v) { mb[i] += v; });
+        }
+    }
+    for (int b = nb - 1; b >= 0; --b)
+        for (int o = 0; o < 8; ++o) {
+            const int c = t.box[b].child[o];
+            if (c >= 0) matvec(&T.m2m[(size_t)o * S * S], &M[(size_t)c * S], &M[(size_t)b * S]);
+        }
+    for (int b = 0; b < nb; ++b) {
+        for (int c : t.V[b]) {
+            const int k = offset_index(ic[3 * b] - ic[3 * c], ic[3 * b + 1] - ic[3 * c + 1],
+                                       ic[3 * b + 2] - ic[3 * c + 2]);
+            matvec(&T.m2l[(size_t)k * S * S], &M[(size_t)c * S], &L[(size_t)b * S]);
+        }
+        double* lb = &L[(size_t)b * S];
+        for (int c : t.X[b])
+            for (int k = t.box[c].pfirst; k < t.box[c].pfirst + t.box[c].pcount; ++k) {
+                double d[3];
+                dhat(b, t.px[k], t.py[k], t.pz[k], d);
+                p2l_point<P>(t.m[k], d[0], d[1], d[2], [&](int i, double v) { lb[i] += v; });
+            }
+    }
+    for (int b = 1; b < nb; ++b)  // parents have smaller ids
+        matvec(&T.l2l[(size_t)oct[b] * S * S], &L[(size_t)t.box[b].parent * S], &L[(size_t)b * S]);
+    const double eps2 = eps * eps;
+    for (int b : t.leaves) {
+        const double* lb = &L[(size_t)b * S];
+        const double s2 = invh[b] * invh[b];
+        for (int i = t.box[b].pfirst; i < t.box[b].pfirst + t.box[b].pcount; ++i) {
+            double d[3], g[3], f[3];
+            dhat(b, t.px[i], t.py[i], t.pz[i], d);
+            local_grad<P>([&](int k) { return lb[k]; }, d[0], d[1], d[2], g[0], g[1], g[2]);
+            for (int a = 0; a < 3; ++a) f[a] = g[a] * s2;
+            for (int w : t.W[b]) {
+                const double* mw = &M[(size_t)w * S];
+                dhat(w, t.px[i], t.py[i], t.pz[i], d);
+                multipole_grad<P>([&](int k) { return mw[k]; }, d[0], d[1], d[2], g[0], g[1], g[2]);
+                const double sw = invh[w] * invh[w];
+                for (int a = 0; a < 3; ++a) f[a] += g[a] * sw;
+            }
+            double nr[3] = {0, 0, 0};
+            for (int u : t.U[b])
+                for (int j = t.box[u].pfirst; j < t.box[u].pfirst + t.box[u].pcount; ++j) {
+                    if (j == i) continue;
+                    const double dx = t.px[j] - t.px[i], dy = t.py[j] - t.py[i], dz = t.pz[j] - t.pz[i];
+                    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                    const double s = t.m[j] / (r2 * std::sqrt(r2));
+                    nr[0] += s * dx, nr[1] += s * dy, nr[2] += s * dz;
+                }
+            const int o = t.orig[i];
+            for (int a = 0; a < 3; ++a) acc[3 * o + a] = G * (f[a] + nr[a]);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int model_fmm(int64_t n, const double* pos, const double* mass, double G, double eps, int p, int cap,
+              double* acc) {
+    for (int64_t i = 0; i < 3 * n; ++i) acc[i] = 0.0;
+    if (n == 0) return 0;
+    oracle::Tree t;
+    t.build(pos, mass, (size_t)n, cap, eps);
+    t.buildLists();
+    switch (p) {
+#define AFMM_CASE(P) \
+    case P: model_fmm<P>(t, G, eps, acc); return 0;
+        AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
+        AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+#undef AFMM_CASE
+    }
+    return -2;
+}
+
+// Per-point operators against the oracle's complex ones.
+int model_p2m_point(int p, double q, const double* d, double* out) {
+    const int S = nterms(p);
+    for (int i = 0; i < S; ++i) out[i] = 0;
+    auto add = [&](int i, double v) { out[i] += v; };
+    switch (p) {
+        case 4: p2m_point<4>(q, d[0], d[1], d[2], add); return 0;
+        case 8: p2m_point<8>(q, d[0], d[1], d[2], add); return 0;
+        case 12: p2m_point<12>(q, d[0], d[1], d[2], add); return 0;
+    }
+    return -2;
+}
+
+int model_tables(int p, double* m2m, double* l2l, double* m2l) {
+    const Tables& T = tables_for_order(p);
+    for (size_t i = 0; i < T.m2m.size(); ++i) m2m[i] = T.m2m[i];
+    for (size_t i = 0; i < T.l2l.size(); ++i) l2l[i] = T.l2l[i];
+    for (size_t i = 0; i < T.m2l.size(); ++i) m2l[i] = T.m2l[i];
+    return 0;
+}
+
+}  // extern "C"
