@@ -1,0 +1,179 @@
+"""Initial conditions for the benchmark and parity workloads (numpy, seeded).
+
+plummer()   the reference's Plummer recipe (InitialConditions3D.hpp:100-152):
+            inverse-CDF radii with u >= 1e-12, Aarseth-Henon-Wielen speed
+            rejection bounded by 0.1, COM position/momentum removed. numpy's
+            generator replaces libstdc++'s mt19937 stream (the oracle keeps
+            the libstdc++ stream for bit-level fixtures).
+hernquist() isotropic Hernquist sphere truncated at r_t (positions by inverse
+            CDF; velocities from the isotropic Jeans equation).
+merger()    the two-galaxy recipe: per galaxy mass 1 in equal-mass bodies,
+            80% exponential disk (scale length 1, rho ~ sech^2(z/z0), z0 = 0.1,
+            truncated at R <= 10, |z| <= 10 z0) and 20% Hernquist bulge
+            (a = 0.2, truncated at 20a); disks inclined +30 and -60 degrees
+            about x; centres 8 apart on a parabolic point-mass orbit with
+            pericentre 1. Disk velocities (optional, needs a force solver):
+            circular speed from the solver's own radial force, binned in R,
+            plus Gaussian dispersions sigma_R = 3.36 Q G Sigma / kappa (Q=1.2),
+            sigma_phi = sigma_R kappa / (2 Omega), sigma_z^2 = pi G Sigma z0.
+            The reference's nearest recipe (gpu/GalaxyIC.hpp:19-125) uses a
+            Plummer bulge and Gaussian thickness instead.
+"""
+import numpy as np
+
+
+def _isotropic(rng, n):
+    ct = 2.0 * rng.random(n) - 1.0
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    ph = 2.0 * np.pi * rng.random(n)
+    return np.stack([st * np.cos(ph), st * np.sin(ph), ct], axis=1)
+
+
+def plummer(n, a=1.0, M=1.0, G=1.0, seed=7):
+    rng = np.random.default_rng(seed)
+    u = np.maximum(rng.random(n), 1e-12)
+    r = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+    q = np.empty(n)
+    todo = np.arange(n)
+    while todo.size:
+        qq = rng.random(todo.size)
+        g = 0.1 * rng.random(todo.size)
+        ok = g <= qq * qq * (1.0 - qq * qq) ** 3.5
+        q[todo[ok]] = qq[ok]
+        todo = todo[~ok]
+    vesc = np.sqrt(2.0 * G * M) * (r * r + a * a) ** -0.25
+    pos = r[:, None] * _isotropic(rng, n)
+    vel = (q * vesc)[:, None] * _isotropic(rng, n)
+    mass = np.full(n, M / n)
+    pos -= (mass[:, None] * pos).sum(0) / mass.sum()
+    vel -= (mass[:, None] * vel).sum(0) / mass.sum()
+    return pos, vel, mass
+
+
+def _hernquist_sigma(r, a, M, rt, G=1.0, extra_mass=None):
+    """Isotropic Jeans dispersion of a truncated Hernquist sphere, optionally in
+    an extra enclosed-mass profile extra_mass(r) (spherical approximation)."""
+    grid = np.geomspace(max(1e-6, r.min() * 0.5), rt, 2048)
+    rho = a / (grid * (grid + a) ** 3)
+    Menc = M * (grid / (grid + a)) ** 2 / (rt / (rt + a)) ** 2
+    if extra_mass is not None:
+        Menc = Menc + extra_mass(grid)
+    integrand = rho * G * Menc / grid ** 2
+    # integral from grid[i] to rt
+    seg = 0.5 * (integrand[1:] + integrand[:-1]) * np.diff(grid)
+    tail = np.concatenate([np.cumsum(seg[::-1])[::-1], [0.0]])
+    sig2 = np.maximum(tail / rho, 0.0)
+    return np.sqrt(np.interp(r, grid, sig2))
+
+
+def hernquist(n, a=0.5, M=1.0, rt=None, G=1.0, seed=7, velocities=True):
+    rng = np.random.default_rng(seed)
+    rt = 20.0 * a if rt is None else rt
+    umax = (rt / (rt + a)) ** 2
+    s = np.sqrt(rng.random(n) * umax)
+    r = a * s / (1.0 - s)
+    pos = r[:, None] * _isotropic(rng, n)
+    mass = np.full(n, M / n)
+    vel = np.zeros_like(pos)
+    if velocities:
+        vel = _hernquist_sigma(r, a, M, rt, G)[:, None] * rng.standard_normal((n, 3))
+    pos -= (mass[:, None] * pos).sum(0) / mass.sum()
+    vel -= (mass[:, None] * vel).sum(0) / mass.sum()
+    return pos, vel, mass
+
+
+def _rot_x(v, ang):
+    c, s = np.cos(ang), np.sin(ang)
+    out = v.copy()
+    out[:, 1] = c * v[:, 1] - s * v[:, 2]
+    out[:, 2] = s * v[:, 1] + c * v[:, 2]
+    return out
+
+
+def galaxy(n, seed, disk_frac=0.8, Rd=1.0, z0=0.1, a_bulge=0.2, Rmax=10.0):
+    """Positions and masses of one galaxy (mass 1) in its own frame, plus the
+    disk mask; disk in the z = 0 plane."""
+    rng = np.random.default_rng(seed)
+    nd = int(round(disk_frac * n))
+    nbul = n - nd
+    R = np.empty(nd)
+    todo = np.arange(nd)
+    while todo.size:
+        rr = -Rd * np.log(rng.random(todo.size) * rng.random(todo.size) + 1e-300)
+        ok = rr <= Rmax
+        R[todo[ok]] = rr[ok]
+        todo = todo[~ok]
+    ph = 2.0 * np.pi * rng.random(nd)
+    u = rng.random(nd)
+    z = np.clip(z0 * np.arctanh(np.clip(2.0 * u - 1.0, -1 + 1e-15, 1 - 1e-15)), -10 * z0, 10 * z0)
+    pdisk = np.stack([R * np.cos(ph), R * np.sin(ph), z], axis=1)
+    rt = 20.0 * a_bulge
+    s = np.sqrt(rng.random(nbul) * (rt / (rt + a_bulge)) ** 2)
+    rb = a_bulge * s / (1.0 - s)
+    pbul = rb[:, None] * _isotropic(rng, nbul)
+    pos = np.concatenate([pdisk, pbul])
+    mass = np.full(n, 1.0 / n)
+    return pos, mass, np.arange(n) < nd
+
+
+def _disk_velocities(pos, acc, is_disk, seed, G=1.0, Rd=1.0, z0=0.1, Md=0.8, Q=1.2, a_bulge=0.2):
+    rng = np.random.default_rng(seed + 12345)
+    vel = np.zeros_like(pos)
+    R = np.hypot(pos[:, 0], pos[:, 1])
+    d = is_disk & (R > 1e-6)
+    aR = (acc[d, 0] * pos[d, 0] + acc[d, 1] * pos[d, 1]) / R[d]
+    vc2_body = np.maximum(-aR * R[d], 0.0)
+    edges = np.quantile(R[d], np.linspace(0, 1, 65))
+    mid = 0.5 * (edges[1:] + edges[:-1])
+    idx = np.clip(np.searchsorted(edges, R[d]) - 1, 0, 63)
+    vc2_bin = np.bincount(idx, vc2_body, 64) / np.maximum(np.bincount(idx, None, 64), 1)
+    vc2 = np.interp(R[d], mid, vc2_bin)
+    om2_bin = vc2_bin / mid ** 2
+    dom2 = np.gradient(om2_bin, mid)
+    kap2_bin = np.maximum(mid * dom2 + 4.0 * om2_bin, 1e-12)
+    kap = np.sqrt(np.interp(R[d], mid, kap2_bin))
+    vc = np.sqrt(vc2)
+    om = vc / R[d]
+    Sigma = Md / (2.0 * np.pi * Rd ** 2) * np.exp(-R[d] / Rd)
+    sR = 3.36 * Q * G * Sigma / kap
+    sphi = sR * kap / (2.0 * np.maximum(om, 1e-12))
+    sz = np.sqrt(np.pi * G * Sigma * z0)
+    cphi, sphi_ = pos[d, 0] / R[d], pos[d, 1] / R[d]
+    vR = sR * rng.standard_normal(d.sum())
+    vphi = vc + sphi * rng.standard_normal(d.sum())
+    vel[d, 0] = vR * cphi - vphi * sphi_
+    vel[d, 1] = vR * sphi_ + vphi * cphi
+    vel[d, 2] = sz * rng.standard_normal(d.sum())
+    # bulge: isotropic Jeans dispersion in bulge + (spherically averaged) disk mass
+    b = ~is_disk
+    rb = np.linalg.norm(pos[b], axis=1)
+    rs = np.sort(np.linalg.norm(pos[is_disk], axis=1))
+    disk_enc = lambda r: Md * np.searchsorted(rs, r) / max(len(rs), 1)  # noqa: E731
+    sig = _hernquist_sigma(rb, a_bulge, 1.0 - Md, 20.0 * a_bulge, G, disk_enc)
+    vel[b] = sig[:, None] * rng.standard_normal((b.sum(), 3))
+    return vel
+
+
+def merger(n_per_galaxy, seed=42, sep=8.0, r_peri=1.0, incl=(30.0, -60.0), G=1.0, solver=None):
+    """Two-galaxy merger ICs. With `solver` (an object with
+    computeAccelerations(pos, mass)), disk and bulge velocities are set from
+    its forces; without it velocities are zero (enough for force timing)."""
+    parts = []
+    for k in range(2):
+        pos, mass, is_disk = galaxy(n_per_galaxy, seed + 7 * k)
+        vel = np.zeros_like(pos)
+        if solver is not None:
+            vel = _disk_velocities(pos, solver.computeAccelerations(pos, mass), is_disk, seed + 7 * k, G)
+        ang = np.radians(incl[k])
+        parts.append((_rot_x(pos, ang), _rot_x(vel, ang), mass, np.full(n_per_galaxy, k, np.int8)))
+    Mt = 2.0
+    vrel = np.sqrt(2.0 * G * Mt / sep)
+    vt = np.sqrt(2.0 * G * Mt * r_peri) / sep
+    vr = -np.sqrt(max(vrel * vrel - vt * vt, 0.0))
+    rrel = np.array([sep, 0.0, 0.0])
+    vvec = np.array([vr, vt, 0.0])
+    pos = np.concatenate([parts[0][0] - 0.5 * rrel, parts[1][0] + 0.5 * rrel])
+    vel = np.concatenate([parts[0][1] - 0.5 * vvec, parts[1][1] + 0.5 * vvec])
+    mass = np.concatenate([parts[0][2], parts[1][2]])
+    species = np.concatenate([parts[0][3], parts[1][3]])
+    return pos, vel, mass, species
