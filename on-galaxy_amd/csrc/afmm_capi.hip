@@ -29,14 +29,14 @@ struct afmm_ctx {
     Pool pool;
     std::string err;
     // tables
-    double *d_m2m = nullptr, *d_l2l = nullptr, *d_m2l = nullptr;
+    double *d_m2m = nullptr, *d_l2l = nullptr, *d_m2l_sw = nullptr;
     // last evaluation
     bool have_tree = false;
     TreeView tv{};
     TreeMeta meta;
     RootInfo root{};
     afmm_stats stats{};
-    hipEvent_t ev[7] = {};
+    hipEvent_t ev[8] = {};
 };
 
 namespace {
@@ -87,7 +87,7 @@ int eval_device(afmm_ctx* c, int64_t n, const double* pos, const double* mass, d
     build_tree(c->pool, s, pos, mass, (int)n, c->prm.leaf_cap, c->prm.eps, tv, meta, root);
     if (root.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
     AFMM_CHECK(hipEventRecord(c->ev[1], s));
-    FmmTables tab{c->d_m2m, c->d_l2l, c->d_m2l};
+    FmmTables tab{c->d_m2m, c->d_l2l, c->d_m2l_sw};
     run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.G, c->prm.eps, acc, c->ev + 2);
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
@@ -124,10 +124,10 @@ int eval_device(afmm_ctx* c, int64_t n, const double* pos, const double* mass, d
     st.p2p_pairs = pairs - n;
     st.ms_tree = elapsed(c->ev[0], c->ev[1]);
     st.ms_upward = elapsed(c->ev[2], c->ev[3]);
-    st.ms_m2l = elapsed(c->ev[3], c->ev[4]);
+    st.ms_lists = elapsed(c->ev[3], c->ev[7]);  // M2L ordering / source table
+    st.ms_m2l = elapsed(c->ev[7], c->ev[4]);
     st.ms_downward = elapsed(c->ev[4], c->ev[5]);
     st.ms_leaf = elapsed(c->ev[5], c->ev[6]);
-    st.ms_lists = 0.0;
     st.ms_total = elapsed(c->ev[0], c->ev[6]);
     return AFMM_OK;
 }
@@ -169,10 +169,11 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
         const Tables& T = tables_for_order(prm->p);
         c->d_m2m = c->pool.get<double>("T_m2m", T.m2m.size());
         c->d_l2l = c->pool.get<double>("T_l2l", T.l2l.size());
-        c->d_m2l = c->pool.get<double>("T_m2l", T.m2l.size());
+        const std::vector<double> sw = mfma_a_layout(T.m2l, T.S, kNumOffsets);
+        c->d_m2l_sw = c->pool.get<double>("T_m2l_sw", sw.size());
         AFMM_CHECK(hipMemcpy(c->d_m2m, T.m2m.data(), sizeof(double) * T.m2m.size(), hipMemcpyHostToDevice));
         AFMM_CHECK(hipMemcpy(c->d_l2l, T.l2l.data(), sizeof(double) * T.l2l.size(), hipMemcpyHostToDevice));
-        AFMM_CHECK(hipMemcpy(c->d_m2l, T.m2l.data(), sizeof(double) * T.m2l.size(), hipMemcpyHostToDevice));
+        AFMM_CHECK(hipMemcpy(c->d_m2l_sw, sw.data(), sizeof(double) * sw.size(), hipMemcpyHostToDevice));
         return AFMM_OK;
     });
     if (r) {

@@ -9,9 +9,9 @@
 //   boxes      B, breadth-first: level-major, Morton order inside a level, so
 //              the children of a box are contiguous (child_first, child mask).
 //              seg_first/seg_count index the sorted bodies.
-//   lists      colleague (27 slots/box), V (189 slots/box) and the dense M2L
-//              source table src_at[B][343]; U, W (per leaf) and X (per box)
-//              in CSR.
+//   lists      colleague (27 slots/box) and V (189 slots/box); U, W (per
+//              leaf) and X (per box) in CSR. M2L builds its own transposed
+//              source table srcT[343][targets in (level, octant) order].
 //   expansions Mhat[B][S], Lhat[B][S] (FP64, scaled real basis).
 // ============================================================================
 #pragma once
@@ -113,7 +113,6 @@ struct TreeView {
     // lists
     int *coll, *ncoll;   // [27B], [B]
     int *vl, *nv;        // [189B], [B]
-    int* src_at;         // [343B] source box per M2L offset, -1 if none
     int *u_off, *u_idx;  // per leaf (index into leaf_ids)
     int *w_off, *w_idx;  // per leaf
     int *x_off, *x_idx;  // per box
@@ -157,11 +156,12 @@ struct FmmTimes {
     float upward = 0, m2l = 0, downward = 0, leaf = 0;
 };
 struct FmmTables {
-    const double *m2m, *l2l, *m2l;  // device, column-major S x S each
+    const double *m2m, *l2l;  // device, column-major S x S each
+    const double* m2l_sw;     // device, M2L operators in the MFMA A-fragment layout
 };
 // Runs the FMM passes on a built tree and writes G*(far + near) to d_acc in
 // input order (AoS n x 3).
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
-             const FmmTables& tab, double G, double eps, double* d_acc, hipEvent_t* ev /* 5 or null */);
+             const FmmTables& tab, double G, double eps, double* d_acc, hipEvent_t* ev /* 6 or null */);
 
 }  // namespace afmm

@@ -148,6 +148,20 @@ Tables build(int p) {
 
 }  // namespace
 
+std::vector<double> mfma_a_layout(const std::vector<double>& cm, int S, int nmat) {
+    const int RB = mfma_row_blocks(S), KB = mfma_k_steps(S);
+    std::vector<double> out((size_t)nmat * RB * KB * 64, 0.0);
+    for (int t = 0; t < nmat; ++t)
+        for (int rb = 0; rb < RB; ++rb)
+            for (int kb = 0; kb < KB; ++kb)
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int row = 16 * rb + lane % 16, col = 4 * kb + lane / 16;
+                    if (row < S && col < S)
+                        out[(((size_t)t * RB + rb) * KB + kb) * 64 + lane] = cm[(size_t)t * S * S + (size_t)col * S + row];
+                }
+    return out;
+}
+
 const Tables& tables_for_order(int p) {
     static std::mutex mu;
     static std::map<int, Tables> cache;

@@ -26,4 +26,12 @@ struct Tables {
 // Builds (and caches per p for the life of the process) the tables of order p.
 const Tables& tables_for_order(int p);
 
+// Re-lays S x S column-major matrices for v_mfma_f64_16x16x4_f64 A operands:
+// [matrix][row block rb][k step kb][lane], value T(row = 16 rb + lane%16,
+// col = 4 kb + lane/16), zero outside S x S. One fragment is then one
+// contiguous 512-byte load of 64 lanes.
+std::vector<double> mfma_a_layout(const std::vector<double>& colmajor, int S, int nmat);
+inline int mfma_row_blocks(int S) { return (S + 15) / 16; }
+inline int mfma_k_steps(int S) { return (S + 3) / 4; }
+
 }  // namespace afmm

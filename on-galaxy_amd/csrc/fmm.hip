@@ -104,28 +104,117 @@ __global__ void __launch_bounds__(64) k_m2m(TreeView t, int off, int cnt, const 
     }
 }
 
-// ---- M2L: one wave per target box, V list through the offset tables.
-template <int P>
-__global__ void __launch_bounds__(64) k_m2l(TreeView t, const double* __restrict__ T,
-                                           const double* __restrict__ M, double* __restrict__ L) {
-    constexpr int S = nterms(P);
-    constexpr int K = (S + 63) / 64;
-    const int b = blockIdx.x;
-    const int lane = threadIdx.x;
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    const int nv = t.nv[b];
-    for (int e = 0; e < nv; ++e) {
-        const int c = t.vl[kVSlots * (size_t)b + e];
-        const int o = offset_index(t.ic[3 * (size_t)b] - t.ic[3 * (size_t)c], t.ic[3 * (size_t)b + 1] - t.ic[3 * (size_t)c + 1],
-                                   t.ic[3 * (size_t)b + 2] - t.ic[3 * (size_t)c + 2]);
-        wave_matvec_acc<S>(T + (size_t)o * S * S, M + (size_t)c * S, acc, lane);
+// ---- M2L on FP64 matrix cores ------------------------------------------------
+// L_t = sum_{c in V(t)} T[offset(t,c)] M_c with level-independent S x S
+// operators. One wave owns 16 targets (the 16 columns of
+// v_mfma_f64_16x16x4_f64) and loops over the 316 separated offsets; for each
+// offset present in its tile it streams the pre-swizzled operator (A, one
+// contiguous 512-byte fragment per MFMA) against the 16 gathered source
+// multipoles (B, zero where a target has no source at that offset), keeping
+// the 16 x S output tile in accumulators for the whole loop. Targets are
+// ordered by (level, octant) so that a tile shares most of its offset pattern.
+// Operand layouts (CDNA 16x16x4 f64): A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D[i][j] in lane j + 16(i%4), register i/4 (rows interleave by 4,
+// unlike the f32 16x16x4 form; see ck's mfma_type for f64 16x16x4:
+// group_size 1, num_groups_per_blk 4).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(kT) k_m2l_keys(int nb, const int* __restrict__ nv, const int* __restrict__ level,
+                                                const int* __restrict__ octant, uint64_t* khi, uint32_t* klo, int* val) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    khi[b] = nv[b] > 0 ? uint64_t(level[b] * 8 + octant[b]) : 511u;
+    klo[b] = 0;
+    val[b] = b;
+}
+
+__global__ void __launch_bounds__(kT) k_m2l_rank(int nb, const int* __restrict__ order, const int* __restrict__ nv,
+                                                int* __restrict__ rank, int* __restrict__ ntgt) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= nb) return;
+    const int b = order[i];
+    rank[b] = i;
+    const bool has = nv[b] > 0;
+    const bool next_has = (i + 1 < nb) && nv[order[i + 1]] > 0;
+    if (has && !next_has) *ntgt = i + 1;
+    if (i == 0 && !has) *ntgt = 0;
+}
+
+__global__ void __launch_bounds__(kT) k_fill_int(int* a, size_t n, int v) {
+    const size_t i = blockIdx.x * (size_t)kT + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// srcT[offset * np + rank(t)] = source box of target t at that offset.
+__global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __restrict__ rank, const int* __restrict__ vl,
+                                               const int* __restrict__ nv, const int* __restrict__ ic,
+                                               int* __restrict__ srcT) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    const int r = rank[b];
+    for (int e = 0; e < nv[b]; ++e) {
+        const int c = vl[kVSlots * (size_t)b + e];
+        const int o = offset_index(ic[3 * (size_t)b] - ic[3 * (size_t)c], ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1],
+                                   ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2]);
+        srcT[(size_t)o * np + r] = c;
     }
+}
+
+template <int P>
+__global__ void __launch_bounds__(64) k_m2l_mfma(int nb, int np, const int* __restrict__ ntgt_p,
+                                                const int* __restrict__ order, const int* __restrict__ srcT,
+                                                const double* __restrict__ Asw, const double* __restrict__ M,
+                                                double* __restrict__ L) {
+    constexpr int S = nterms(P);
+    constexpr int RB = (S + 15) / 16;
+    constexpr int KB = (S + 3) / 4;
+    constexpr int FRAG = RB * KB * 64;
+    const int lane = threadIdx.x;
+    const int j = lane & 15, kq = lane >> 4;
+    const int r0 = blockIdx.x * 16;
+    const int ntgt = *ntgt_p;
+    const int rank = r0 + j;
+    dbl4 acc[RB];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int r = lane + 64 * k;
-        if (r < S) L[(size_t)b * S + r] = acc[k];
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = dbl4{0.0, 0.0, 0.0, 0.0};
+    if (r0 < ntgt) {
+        const bool valid = rank < ntgt;
+        int nxt = valid ? srcT[(size_t)offset_index(-3, -3, -3) * np + rank] : -1;
+        for (int o = 0; o < kNumOffsets; ++o) {
+            const int dx = o / 49 - 3, dy = (o / 7) % 7 - 3, dz = o % 7 - 3;
+            const int src = nxt;
+            if (o + 1 < kNumOffsets) {
+                const int o1 = o + 1;
+                const int ex = o1 / 49 - 3, ey = (o1 / 7) % 7 - 3, ez = o1 % 7 - 3;
+                const bool near1 = ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1 && ez >= -1 && ez <= 1;
+                nxt = (valid && !near1) ? srcT[(size_t)o1 * np + rank] : -1;
+            }
+            if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+            if (__ballot(src >= 0) == 0ull) continue;
+            const double* __restrict__ ms = M + (size_t)(src >= 0 ? src : 0) * S;
+            double bv[KB];
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                const int k = 4 * kb + kq;
+                bv[kb] = (src >= 0 && k < S) ? ms[k] : 0.0;
+            }
+            const double* __restrict__ A = Asw + (size_t)o * FRAG + lane;
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(rb * KB + kb) * 64], bv[kb], acc[rb], 0, 0, 0);
+        }
+    }
+    if (rank < nb) {
+        const int t = order[rank];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rb + kq + 4 * r;
+                if (row < S) L[(size_t)t * S + row] = acc[rb][r];
+            }
     }
 }
 
@@ -327,7 +416,32 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         hipLaunchKernelGGL(k_m2m<P>, dim3(cnt), dim3(64), 0, s, t, off, cnt, tab.m2m, M);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[1], s));
-    hipLaunchKernelGGL(k_m2l<P>, dim3(nb), dim3(64), 0, s, t, tab.m2l, M, L);
+    {
+        // target order, tile count and transposed source table
+        const int np = (nb + 15) / 16 * 16;
+        uint64_t* khi = pool.get<uint64_t>("m2l_khi", 2 * (size_t)nb);
+        uint32_t* klo = pool.get<uint32_t>("m2l_klo", 2 * (size_t)nb);
+        int* val = pool.get<int>("m2l_val", 2 * (size_t)nb);
+        SortBufs sb;
+        sb.khi[0] = khi, sb.khi[1] = khi + nb;
+        sb.klo[0] = klo, sb.klo[1] = klo + nb;
+        sb.val[0] = val, sb.val[1] = val + nb;
+        sb.hist = pool.get<int>("m2l_hist", 512 * (size_t)sort_blocks(nb));
+        sb.scan_tmp = pool.get<int>("m2l_scan", scan_tmp_ints(512 * sort_blocks(nb) + nb) + 16);
+        int* rank = pool.get<int>("m2l_rank", nb);
+        int* ntgt = pool.get<int>("m2l_ntgt", 1);
+        int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
+        hipLaunchKernelGGL(k_m2l_keys, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.nv, t.level, t.octant, sb.khi[0],
+                           sb.klo[0], sb.val[0]);
+        const int r = radix_sort(sb, nb, 1, 1, s);
+        hipLaunchKernelGGL(k_m2l_rank, dim3(blocks(nb)), dim3(kT), 0, s, nb, sb.val[r], t.nv, rank, ntgt);
+        hipLaunchKernelGGL(k_fill_int, dim3(blocks((long long)kNumOffsets * np)), dim3(kT), 0, s, srcT,
+                           (size_t)kNumOffsets * np, -1);
+        hipLaunchKernelGGL(k_m2l_src, dim3(blocks(nb)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv, t.ic, srcT);
+        if (ev) AFMM_CHECK(hipEventRecord(ev[5], s));
+        hipLaunchKernelGGL(k_m2l_mfma<P>, dim3(np / 16), dim3(64), 0, s, nb, np, ntgt, sb.val[r], srcT, tab.m2l_sw,
+                           M, L);
+    }
     if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
     if (t.n_x > 0) hipLaunchKernelGGL(k_p2l<P>, dim3(nb), dim3(64), 0, s, t, L);
     for (int l = 1; l <= t.max_level; ++l) {

@@ -360,8 +360,7 @@ __device__ __forceinline__ bool adjacent(const double* cfp, const double* half, 
 __global__ void __launch_bounds__(kT) k_coll_v(int off, int cnt, double tol, const int* parent,
                                               const int* child_first, const unsigned char* mask,
                                               const double* cfp, const double* half, const int* ic,
-                                              int* coll, int* ncoll, int* vl, int* nv, int* src_at,
-                                              int* overflow) {
+                                              int* coll, int* ncoll, int* vl, int* nv, int* overflow) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= cnt) return;
     const int b = off + i;
@@ -391,7 +390,6 @@ __global__ void __launch_bounds__(kT) k_coll_v(int off, int cnt, double tol, con
                     const int dy = ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1];
                     const int dz = ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2];
                     if (dx < -3 || dx > 3 || dy < -3 || dy > 3 || dz < -3 || dz > 3) atomicOr(overflow, 2);
-                    else src_at[kNumOffsets * (size_t)b + offset_index(dx, dy, dz)] = c;
                 } else {
                     atomicOr(overflow, 1);
                 }
@@ -650,16 +648,13 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     tv.ncoll = pool.get<int>("ncoll", nb);
     tv.vl = pool.get<int>("vl", kVSlots * (size_t)nb);
     tv.nv = pool.get<int>("nv", nb);
-    tv.src_at = pool.get<int>("src_at", kNumOffsets * (size_t)nb);
     int* overflow = pool.get<int>("overflow", 1);
-    hipLaunchKernelGGL(k_fill, dim3(blocks((long long)kNumOffsets * nb)), dim3(kT), 0, s, tv.src_at,
-                       kNumOffsets * nb, -1);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, overflow, 1, 0);
     for (int lv = 0; lv <= tv.max_level; ++lv) {
         const int off = meta.level_off[lv], cnt = meta.level_off[lv + 1] - off;
         hipLaunchKernelGGL(k_coll_v, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, root.tol, tv.parent,
                            tv.child_first, tv.child_mask, tv.cfp, tv.half, tv.ic, tv.coll, tv.ncoll, tv.vl,
-                           tv.nv, tv.src_at, overflow);
+                           tv.nv, overflow);
     }
     if (read_int(overflow, s) != 0) throw std::runtime_error("interaction-list slot overflow");
 
