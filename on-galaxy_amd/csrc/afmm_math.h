@@ -232,4 +232,15 @@ AFMM_HD constexpr int offset_index(int dx, int dy, int dz) {
 }
 constexpr int kNumOffsets = 343;
 
+// M2L operator tiling for v_mfma_f64_16x16x4_f64: RB row blocks of 16 and the
+// k steps of 4 split into NKC chunks of KC steps, one chunk of all row blocks
+// (RB*KC*64 doubles, RB*KC <= 60) small enough to double-buffer in under
+// 64 KB of LDS.
+AFMM_HD constexpr int m2l_rb(int S) { return (S + 15) / 16; }
+AFMM_HD constexpr int m2l_kb(int S) { return (S + 3) / 4; }
+AFMM_HD constexpr int m2l_kcmax(int S) { return 60 / m2l_rb(S) > 0 ? 60 / m2l_rb(S) : 1; }
+AFMM_HD constexpr int m2l_nkc(int S) { return (m2l_kb(S) + m2l_kcmax(S) - 1) / m2l_kcmax(S); }
+AFMM_HD constexpr int m2l_kc(int S) { return (m2l_kb(S) + m2l_nkc(S) - 1) / m2l_nkc(S); }
+AFMM_HD constexpr int m2l_chunk(int S) { return m2l_rb(S) * m2l_kc(S) * 64; }
+
 }  // namespace afmm

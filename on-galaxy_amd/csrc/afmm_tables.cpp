@@ -149,16 +149,18 @@ Tables build(int p) {
 }  // namespace
 
 std::vector<double> mfma_a_layout(const std::vector<double>& cm, int S, int nmat) {
-    const int RB = mfma_row_blocks(S), KB = mfma_k_steps(S);
-    std::vector<double> out((size_t)nmat * RB * KB * 64, 0.0);
+    const int RB = m2l_rb(S), NKC = m2l_nkc(S), KC = m2l_kc(S), CH = m2l_chunk(S);
+    std::vector<double> out((size_t)nmat * NKC * CH, 0.0);
     for (int t = 0; t < nmat; ++t)
-        for (int rb = 0; rb < RB; ++rb)
-            for (int kb = 0; kb < KB; ++kb)
-                for (int lane = 0; lane < 64; ++lane) {
-                    const int row = 16 * rb + lane % 16, col = 4 * kb + lane / 16;
-                    if (row < S && col < S)
-                        out[(((size_t)t * RB + rb) * KB + kb) * 64 + lane] = cm[(size_t)t * S * S + (size_t)col * S + row];
-                }
+        for (int c = 0; c < NKC; ++c)
+            for (int rb = 0; rb < RB; ++rb)
+                for (int kk = 0; kk < KC; ++kk)
+                    for (int lane = 0; lane < 64; ++lane) {
+                        const int row = 16 * rb + lane % 16, col = 4 * (c * KC + kk) + lane / 16;
+                        if (row < S && col < S)
+                            out[((size_t)t * NKC + c) * CH + ((size_t)rb * KC + kk) * 64 + lane] =
+                                cm[(size_t)t * S * S + (size_t)col * S + row];
+                    }
     return out;
 }
 

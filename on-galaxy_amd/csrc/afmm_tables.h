@@ -27,11 +27,10 @@ struct Tables {
 const Tables& tables_for_order(int p);
 
 // Re-lays S x S column-major matrices for v_mfma_f64_16x16x4_f64 A operands:
-// [matrix][row block rb][k step kb][lane], value T(row = 16 rb + lane%16,
-// col = 4 kb + lane/16), zero outside S x S. One fragment is then one
-// contiguous 512-byte load of 64 lanes.
+// [matrix][k chunk c][row block rb][k step kk within chunk][lane], value
+// T(row = 16 rb + lane%16, col = 4 (c*KC + kk) + lane/16), zero outside
+// S x S (chunking of afmm_math.h m2l_*). A chunk is contiguous and one
+// fragment is one 512-byte run of 64 lanes.
 std::vector<double> mfma_a_layout(const std::vector<double>& colmajor, int S, int nmat);
-inline int mfma_row_blocks(int S) { return (S + 15) / 16; }
-inline int mfma_k_steps(int S) { return (S + 3) / 4; }
 
 }  // namespace afmm

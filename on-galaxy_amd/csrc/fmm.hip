@@ -160,60 +160,157 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
     }
 }
 
+constexpr int kM2LWaves = 8;
 template <int P>
-__global__ void __launch_bounds__(64) k_m2l_mfma(int nb, int np, const int* __restrict__ ntgt_p,
-                                                const int* __restrict__ order, const int* __restrict__ srcT,
-                                                const double* __restrict__ Asw, const double* __restrict__ M,
-                                                double* __restrict__ L) {
-    constexpr int S = nterms(P);
-    constexpr int RB = (S + 15) / 16;
-    constexpr int KB = (S + 3) / 4;
-    constexpr int FRAG = RB * KB * 64;
-    const int lane = threadIdx.x;
-    const int j = lane & 15, kq = lane >> 4;
-    const int r0 = blockIdx.x * 16;
+constexpr int m2l_cb() { return P <= 8 ? 2 : 1; }  // 16-target column blocks per wave
+template <int P>
+constexpr int m2l_tile() { return kM2LWaves * 16 * m2l_cb<P>(); }
+
+// present[wg][o] = 1 when some target of workgroup tile wg has a source at
+// offset o (one wave per (tile, offset); the tile's ranks are contiguous in srcT).
+__global__ void __launch_bounds__(64) k_m2l_plan(int tile, int np, const int* __restrict__ ntgt_p,
+                                                const int* __restrict__ srcT, unsigned char* __restrict__ present) {
+    const int wg = blockIdx.x / kNumOffsets, o = blockIdx.x % kNumOffsets;
+    const int dx = o / 49 - 3, dy = (o / 7) % 7 - 3, dz = o % 7 - 3;
+    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
     const int ntgt = *ntgt_p;
-    const int rank = r0 + j;
-    dbl4 acc[RB];
+    bool any = false;
+    if (!near)
+        for (int k = threadIdx.x; k < tile; k += 64) {
+            const int r = wg * tile + k;
+            if (r < ntgt && srcT[(size_t)o * np + r] >= 0) any = true;
+        }
+    const bool f = __ballot(any) != 0ull;
+    if (threadIdx.x == 0) present[(size_t)wg * kNumOffsets + o] = f ? 1 : 0;
+}
+
+template <int P>
+__global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, const int* __restrict__ ntgt_p,
+                                                           const int* __restrict__ order,
+                                                           const int* __restrict__ srcT,
+                                                           const unsigned char* __restrict__ present_g,
+                                                           const double* __restrict__ Asw,
+                                                           const double* __restrict__ M, double* __restrict__ L) {
+    constexpr int S = nterms(P);
+    constexpr int RB = m2l_rb(S), NKC = m2l_nkc(S), KC = m2l_kc(S), CH = m2l_chunk(S);
+    constexpr int CB = m2l_cb<P>();
+    constexpr int TILE = m2l_tile<P>();
+    constexpr int NT = kM2LWaves * 64;
+    constexpr int PER = (CH + NT - 1) / NT;
+    __shared__ double abuf[2][CH];
+    __shared__ unsigned char present[kNumOffsets + 1];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int j = lane & 15, kq = lane >> 4;
+    const int base = blockIdx.x * TILE;
+    const int ntgt = *ntgt_p;
+    int rank[CB];
+    bool valid[CB];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) acc[rb] = dbl4{0.0, 0.0, 0.0, 0.0};
-    if (r0 < ntgt) {
-        const bool valid = rank < ntgt;
-        int nxt = valid ? srcT[(size_t)offset_index(-3, -3, -3) * np + rank] : -1;
-        for (int o = 0; o < kNumOffsets; ++o) {
-            const int dx = o / 49 - 3, dy = (o / 7) % 7 - 3, dz = o % 7 - 3;
-            const int src = nxt;
-            if (o + 1 < kNumOffsets) {
-                const int o1 = o + 1;
-                const int ex = o1 / 49 - 3, ey = (o1 / 7) % 7 - 3, ez = o1 % 7 - 3;
-                const bool near1 = ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1 && ez >= -1 && ez <= 1;
-                nxt = (valid && !near1) ? srcT[(size_t)o1 * np + rank] : -1;
+    for (int cb = 0; cb < CB; ++cb) {
+        rank[cb] = base + (wave * CB + cb) * 16 + j;
+        valid[cb] = rank[cb] < ntgt;
+    }
+    dbl4 acc[CB][RB];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = dbl4{0.0, 0.0, 0.0, 0.0};
+    if (base < ntgt) {
+        for (int t = tid; t < kNumOffsets; t += NT) present[t] = present_g[(size_t)blockIdx.x * kNumOffsets + t];
+        if (tid == 0) present[kNumOffsets] = 1;  // sentinel
+        __syncthreads();
+        auto next_present = [&](int o) {
+            ++o;
+            while (!present[o]) ++o;
+            return o;  // kNumOffsets when none left
+        };
+        int o = next_present(-1);
+        int snext[CB];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) snext[cb] = -1;
+        if (o < kNumOffsets) {
+            const double* src = Asw + (size_t)o * NKC * CH;
+            for (int e = tid; e < CH; e += NT) abuf[0][e] = src[e];
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) snext[cb] = valid[cb] ? srcT[(size_t)o * np + rank[cb]] : -1;
+        }
+        __syncthreads();
+        int buf = 0, c = 0;
+        int scur[CB];
+        bool wave_has = false;
+        while (o < kNumOffsets) {
+            if (c == 0) {
+                bool any = false;
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb) {
+                    scur[cb] = snext[cb];
+                    any |= scur[cb] >= 0;
+                }
+                wave_has = __ballot(any) != 0ull;
+                const int o2 = next_present(o);
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb)
+                    snext[cb] = (o2 < kNumOffsets && valid[cb]) ? srcT[(size_t)o2 * np + rank[cb]] : -1;
             }
-            if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
-            if (__ballot(src >= 0) == 0ull) continue;
-            const double* __restrict__ ms = M + (size_t)(src >= 0 ? src : 0) * S;
-            double bv[KB];
-#pragma unroll
-            for (int kb = 0; kb < KB; ++kb) {
-                const int k = 4 * kb + kq;
-                bv[kb] = (src >= 0 && k < S) ? ms[k] : 0.0;
+            int on = o, cn = c + 1;
+            if (cn == NKC) {
+                cn = 0;
+                on = next_present(o);
             }
-            const double* __restrict__ A = Asw + (size_t)o * FRAG + lane;
+            double pre[PER];
+            if (on < kNumOffsets) {
+                const double* src = Asw + ((size_t)on * NKC + cn) * CH;
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb)
+                for (int i = 0; i < PER; ++i) {
+                    const int e = tid + NT * i;
+                    pre[i] = (e < CH) ? src[e] : 0.0;
+                }
+            }
+            if (wave_has) {
+                double bv[CB][KC];
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb)
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(rb * KB + kb) * 64], bv[kb], acc[rb], 0, 0, 0);
+                for (int cb = 0; cb < CB; ++cb) {
+                    const double* ms = M + (size_t)(scur[cb] >= 0 ? scur[cb] : 0) * S;
+#pragma unroll
+                    for (int kk = 0; kk < KC; ++kk) {
+                        const int k = 4 * (c * KC + kk) + kq;
+                        bv[cb][kk] = (scur[cb] >= 0 && k < S) ? ms[k] : 0.0;
+                    }
+                }
+                const double* ab = abuf[buf] + lane;
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) {
+                        const double a = ab[(rb * KC + kk) * 64];
+#pragma unroll
+                        for (int cb = 0; cb < CB; ++cb)
+                            acc[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[cb][kk], acc[cb][rb], 0, 0, 0);
+                    }
+            }
+            if (on < kNumOffsets) {
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int e = tid + NT * i;
+                    if (e < CH) abuf[buf ^ 1][e] = pre[i];
+                }
+            }
+            __syncthreads();
+            buf ^= 1;
+            o = on;
+            c = cn;
         }
     }
-    if (rank < nb) {
-        const int t = order[rank];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+        if (rank[cb] >= nb) continue;
+        const int t = order[rank[cb]];
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * rb + kq + 4 * r;
-                if (row < S) L[(size_t)t * S + row] = acc[rb][r];
+                if (row < S) L[(size_t)t * S + row] = acc[cb][rb][r];
             }
     }
 }
@@ -418,7 +515,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (ev) AFMM_CHECK(hipEventRecord(ev[1], s));
     {
         // target order, tile count and transposed source table
-        const int np = (nb + 15) / 16 * 16;
+        const int np = (nb + m2l_tile<P>() - 1) / m2l_tile<P>() * m2l_tile<P>();
         uint64_t* khi = pool.get<uint64_t>("m2l_khi", 2 * (size_t)nb);
         uint32_t* klo = pool.get<uint32_t>("m2l_klo", 2 * (size_t)nb);
         int* val = pool.get<int>("m2l_val", 2 * (size_t)nb);
@@ -438,9 +535,13 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         hipLaunchKernelGGL(k_fill_int, dim3(blocks((long long)kNumOffsets * np)), dim3(kT), 0, s, srcT,
                            (size_t)kNumOffsets * np, -1);
         hipLaunchKernelGGL(k_m2l_src, dim3(blocks(nb)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv, t.ic, srcT);
+        constexpr int TILE = m2l_tile<P>();
+        const int nwg = (nb + TILE - 1) / TILE;
+        unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)nwg * kNumOffsets);
+        hipLaunchKernelGGL(k_m2l_plan, dim3(nwg * kNumOffsets), dim3(64), 0, s, TILE, np, ntgt, srcT, present);
         if (ev) AFMM_CHECK(hipEventRecord(ev[5], s));
-        hipLaunchKernelGGL(k_m2l_mfma<P>, dim3(np / 16), dim3(64), 0, s, nb, np, ntgt, sb.val[r], srcT, tab.m2l_sw,
-                           M, L);
+        hipLaunchKernelGGL(k_m2l_mfma<P>, dim3(nwg), dim3(kM2LWaves * 64), 0, s, nb, np, ntgt, sb.val[r], srcT,
+                           present, tab.m2l_sw, M, L);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
     if (t.n_x > 0) hipLaunchKernelGGL(k_p2l<P>, dim3(nb), dim3(64), 0, s, t, L);
