@@ -29,7 +29,7 @@ struct afmm_ctx {
     Pool pool;
     std::string err;
     // tables
-    double *d_m2m = nullptr, *d_l2l = nullptr, *d_m2l_sw = nullptr;
+    double *d_m2m_sw = nullptr, *d_l2l_sw = nullptr, *d_m2l_sw = nullptr;
     // last evaluation
     bool have_tree = false;
     TreeView tv{};
@@ -87,7 +87,7 @@ int eval_device(afmm_ctx* c, int64_t n, const double* pos, const double* mass, d
     build_tree(c->pool, s, pos, mass, (int)n, c->prm.leaf_cap, c->prm.eps, tv, meta, root);
     if (root.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
     AFMM_CHECK(hipEventRecord(c->ev[1], s));
-    FmmTables tab{c->d_m2m, c->d_l2l, c->d_m2l_sw};
+    FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw};
     run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.G, c->prm.eps, acc, c->ev + 2);
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
@@ -167,13 +167,15 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
         AFMM_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto& e : c->ev) AFMM_CHECK(hipEventCreate(&e));
         const Tables& T = tables_for_order(prm->p);
-        c->d_m2m = c->pool.get<double>("T_m2m", T.m2m.size());
-        c->d_l2l = c->pool.get<double>("T_l2l", T.l2l.size());
-        const std::vector<double> sw = mfma_a_layout(T.m2l, T.S, kNumOffsets);
-        c->d_m2l_sw = c->pool.get<double>("T_m2l_sw", sw.size());
-        AFMM_CHECK(hipMemcpy(c->d_m2m, T.m2m.data(), sizeof(double) * T.m2m.size(), hipMemcpyHostToDevice));
-        AFMM_CHECK(hipMemcpy(c->d_l2l, T.l2l.data(), sizeof(double) * T.l2l.size(), hipMemcpyHostToDevice));
-        AFMM_CHECK(hipMemcpy(c->d_m2l_sw, sw.data(), sizeof(double) * sw.size(), hipMemcpyHostToDevice));
+        const std::vector<double> m2m = mfma_a_layout(T.m2m, T.S, 8);
+        const std::vector<double> l2l = mfma_a_layout(T.l2l, T.S, 8);
+        const std::vector<double> m2l = mfma_a_layout(T.m2l, T.S, kNumOffsets);
+        c->d_m2m_sw = c->pool.get<double>("T_m2m_sw", m2m.size());
+        c->d_l2l_sw = c->pool.get<double>("T_l2l_sw", l2l.size());
+        c->d_m2l_sw = c->pool.get<double>("T_m2l_sw", m2l.size());
+        AFMM_CHECK(hipMemcpy(c->d_m2m_sw, m2m.data(), sizeof(double) * m2m.size(), hipMemcpyHostToDevice));
+        AFMM_CHECK(hipMemcpy(c->d_l2l_sw, l2l.data(), sizeof(double) * l2l.size(), hipMemcpyHostToDevice));
+        AFMM_CHECK(hipMemcpy(c->d_m2l_sw, m2l.data(), sizeof(double) * m2l.size(), hipMemcpyHostToDevice));
         return AFMM_OK;
     });
     if (r) {

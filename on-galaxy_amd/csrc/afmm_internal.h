@@ -110,6 +110,8 @@ struct TreeView {
     double* invh;        // [B]
     int* ic;             // [3B] integer coordinates at the box's level
     int* leaf_ids;       // [nleaves] BFS ids of the leaves
+    int* lo_order;       // [B] boxes sorted by (level, octant); level l is [level_off[l], level_off[l+1])
+    int* nonleaf_ids;    // [B - nleaves] non-leaf boxes in BFS order
     // lists
     int *coll, *ncoll;   // [27B], [B]
     int *vl, *nv;        // [189B], [B]
@@ -123,6 +125,7 @@ struct TreeView {
 // ids [level_off[l], level_off[l+1]).
 struct TreeMeta {
     std::vector<int> level_off;
+    std::vector<int> nonleaf_off;  // non-leaf boxes of level l: nonleaf_ids[nonleaf_off[l] .. nonleaf_off[l+1])
 };
 
 // ---- scan_sort.hip -----------------------------------------------------------
@@ -156,8 +159,8 @@ struct FmmTimes {
     float upward = 0, m2l = 0, downward = 0, leaf = 0;
 };
 struct FmmTables {
-    const double *m2m, *l2l;  // device, column-major S x S each
-    const double* m2l_sw;     // device, M2L operators in the MFMA A-fragment layout
+    const double *m2m_sw, *l2l_sw;  // device, 8 octant operators in the MFMA A-fragment layout
+    const double* m2l_sw;           // device, 343 offset operators in the same layout
 };
 // Runs the FMM passes on a built tree and writes G*(far + near) to d_acc in
 // input order (AoS n x 3).

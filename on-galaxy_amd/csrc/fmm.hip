@@ -19,21 +19,6 @@ namespace {
 constexpr int kT = 256;
 inline int blocks(long long n, int t = kT) { return (int)std::max<long long>(1, (n + t - 1) / t); }
 
-// out[r] (+)= sum_c T[c*S + r] * in[c] for one wave; in[] is broadcast.
-template <int S>
-__device__ __forceinline__ void wave_matvec_acc(const double* __restrict__ T, const double* __restrict__ in,
-                                                double (&acc)[(S + 63) / 64], int lane) {
-    for (int c = 0; c < S; ++c) {
-        const double v = in[c];
-        const double* col = T + (size_t)c * S;
-#pragma unroll
-        for (int k = 0; k < (S + 63) / 64; ++k) {
-            const int r = lane + 64 * k;
-            if (r < S) acc[k] += col[r] * v;
-        }
-    }
-}
-
 // Rows of per-body coefficient vectors staged in LDS (kept within 64 KB).
 template <int S>
 constexpr int lds_rows() { return 64 * S * 8 <= 65536 ? 64 : 32; }
@@ -80,30 +65,6 @@ __global__ void __launch_bounds__(64) k_p2m(TreeView t, double* __restrict__ M) 
     }
 }
 
-// ---- M2M: one wave per parent of level l, 8 child-octant matrices.
-template <int P>
-__global__ void __launch_bounds__(64) k_m2m(TreeView t, int off, int cnt, const double* __restrict__ T,
-                                           double* __restrict__ M) {
-    constexpr int S = nterms(P);
-    constexpr int K = (S + 63) / 64;
-    const int b = off + blockIdx.x;
-    const int nch = t.nchild[b];
-    if (nch == 0) return;
-    const int lane = threadIdx.x;
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    for (int r = 0; r < nch; ++r) {
-        const int c = t.child_first[b] + r;
-        wave_matvec_acc<S>(T + (size_t)t.octant[c] * S * S, M + (size_t)c * S, acc, lane);
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int r = lane + 64 * k;
-        if (r < S) M[(size_t)b * S + r] = acc[k];
-    }
-}
-
 // ---- M2L on FP64 matrix cores ------------------------------------------------
 // L_t = sum_{c in V(t)} T[offset(t,c)] M_c with level-independent S x S
 // operators. One wave owns 16 targets (the 16 columns of
@@ -119,25 +80,9 @@ __global__ void __launch_bounds__(64) k_m2m(TreeView t, int off, int cnt, const 
 // group_size 1, num_groups_per_blk 4).
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(kT) k_m2l_keys(int nb, const int* __restrict__ nv, const int* __restrict__ level,
-                                                const int* __restrict__ octant, uint64_t* khi, uint32_t* klo, int* val) {
-    const int b = blockIdx.x * kT + threadIdx.x;
-    if (b >= nb) return;
-    khi[b] = nv[b] > 0 ? uint64_t(level[b] * 8 + octant[b]) : 511u;
-    klo[b] = 0;
-    val[b] = b;
-}
-
-__global__ void __launch_bounds__(kT) k_m2l_rank(int nb, const int* __restrict__ order, const int* __restrict__ nv,
-                                                int* __restrict__ rank, int* __restrict__ ntgt) {
+__global__ void __launch_bounds__(kT) k_rank_of(int nb, const int* __restrict__ order, int* __restrict__ rank) {
     const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= nb) return;
-    const int b = order[i];
-    rank[b] = i;
-    const bool has = nv[b] > 0;
-    const bool next_has = (i + 1 < nb) && nv[order[i + 1]] > 0;
-    if (has && !next_has) *ntgt = i + 1;
-    if (i == 0 && !has) *ntgt = 0;
+    if (i < nb) rank[order[i]] = i;
 }
 
 __global__ void __launch_bounds__(kT) k_fill_int(int* a, size_t n, int v) {
@@ -168,12 +113,11 @@ constexpr int m2l_tile() { return kM2LWaves * 16 * m2l_cb<P>(); }
 
 // present[wg][o] = 1 when some target of workgroup tile wg has a source at
 // offset o (one wave per (tile, offset); the tile's ranks are contiguous in srcT).
-__global__ void __launch_bounds__(64) k_m2l_plan(int tile, int np, const int* __restrict__ ntgt_p,
-                                                const int* __restrict__ srcT, unsigned char* __restrict__ present) {
+__global__ void __launch_bounds__(64) k_m2l_plan(int tile, int np, int ntgt, const int* __restrict__ srcT,
+                                                unsigned char* __restrict__ present) {
     const int wg = blockIdx.x / kNumOffsets, o = blockIdx.x % kNumOffsets;
     const int dx = o / 49 - 3, dy = (o / 7) % 7 - 3, dz = o % 7 - 3;
     const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
-    const int ntgt = *ntgt_p;
     bool any = false;
     if (!near)
         for (int k = threadIdx.x; k < tile; k += 64) {
@@ -185,7 +129,7 @@ __global__ void __launch_bounds__(64) k_m2l_plan(int tile, int np, const int* __
 }
 
 template <int P>
-__global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, const int* __restrict__ ntgt_p,
+__global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int ntgt,
                                                            const int* __restrict__ order,
                                                            const int* __restrict__ srcT,
                                                            const unsigned char* __restrict__ present_g,
@@ -202,7 +146,6 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, con
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int j = lane & 15, kq = lane >> 4;
     const int base = blockIdx.x * TILE;
-    const int ntgt = *ntgt_p;
     int rank[CB];
     bool valid[CB];
 #pragma unroll
@@ -315,6 +258,78 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, con
     }
 }
 
+// ---- M2M / L2L on FP64 matrix cores ----------------------------------------------
+// out[t] (+)= sum_o T[o] in[src(o, t)] over the 8 octant operators, one wave
+// per 16 targets of one level.
+//   MODE 0 (M2M, FmmExpansions3D.hpp:242-265): targets are the level's
+//     non-leaf boxes, src(o, t) = child of t in octant o; overwrites Mhat[t].
+//   MODE 1 (L2L, :307-330): targets are the level's boxes in octant order (so a
+//     tile mostly shares one operator), src(o, t) = parent if octant(t) == o;
+//     adds into Lhat[t].
+template <int P, int MODE>
+__global__ void __launch_bounds__(64) k_level_mfma(int n, const int* __restrict__ tgt,
+                                                  const int* __restrict__ child_first,
+                                                  const unsigned char* __restrict__ mask,
+                                                  const int* __restrict__ parent, const int* __restrict__ octant,
+                                                  const double* __restrict__ Tsw, const double* __restrict__ in,
+                                                  double* __restrict__ out) {
+    constexpr int S = nterms(P);
+    constexpr int RB = m2l_rb(S), NKC = m2l_nkc(S), KC = m2l_kc(S), CH = m2l_chunk(S);
+    const int lane = threadIdx.x, j = lane & 15, kq = lane >> 4;
+    const int ti = blockIdx.x * 16 + j;
+    const bool valid = ti < n;
+    const int t = valid ? tgt[ti] : -1;
+    unsigned mk = 0;
+    int cf = 0, oct = -1, par = -1;
+    if (valid) {
+        if (MODE == 0) {
+            mk = mask[t];
+            cf = child_first[t];
+        } else {
+            oct = octant[t];
+            par = parent[t];
+        }
+    }
+    dbl4 acc[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int o = 0; o < 8; ++o) {
+        int src = -1;
+        if (MODE == 0) {
+            if ((mk >> o) & 1u) src = cf + __popc(mk & ((1u << o) - 1u));
+        } else {
+            if (oct == o) src = par;
+        }
+        if (__ballot(src >= 0) == 0ull) continue;
+        const double* __restrict__ ms = in + (size_t)(src >= 0 ? src : 0) * S;
+        for (int c = 0; c < NKC; ++c) {
+            double bv[KC];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) {
+                const int k = 4 * (c * KC + kk) + kq;
+                bv[kk] = (src >= 0 && k < S) ? ms[k] : 0.0;
+            }
+            const double* __restrict__ A = Tsw + ((size_t)o * NKC + c) * CH + lane;
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(rb * KC + kk) * 64], bv[kk], acc[rb], 0, 0, 0);
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * rb + kq + 4 * r;
+            if (row < S) {
+                if (MODE == 0) out[(size_t)t * S + row] = acc[rb][r];
+                else out[(size_t)t * S + row] += acc[rb][r];
+            }
+        }
+}
+
 // ---- P2L: one wave per box with an X list; rows of q*conj(S) in LDS.
 template <int P>
 __global__ void __launch_bounds__(64) k_p2l(TreeView t, double* __restrict__ L) {
@@ -359,28 +374,6 @@ __global__ void __launch_bounds__(64) k_p2l(TreeView t, double* __restrict__ L) 
     for (int k = 0; k < K; ++k) {
         const int r = lane + 64 * k;
         if (r < S) L[(size_t)b * S + r] += acc[k];
-    }
-}
-
-// ---- L2L: one wave per box of level l, parent -> child octant matrix.
-template <int P>
-__global__ void __launch_bounds__(64) k_l2l(TreeView t, int off, const double* __restrict__ T,
-                                           double* __restrict__ L) {
-    constexpr int S = nterms(P);
-    constexpr int K = (S + 63) / 64;
-    const int b = off + blockIdx.x;
-    const int lane = threadIdx.x;
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int r = lane + 64 * k;
-        acc[k] = (r < S) ? L[(size_t)b * S + r] : 0.0;
-    }
-    wave_matvec_acc<S>(T + (size_t)t.octant[b] * S * S, L + (size_t)t.parent[b] * S, acc, lane);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int r = lane + 64 * k;
-        if (r < S) L[(size_t)b * S + r] = acc[k];
     }
 }
 
@@ -442,48 +435,132 @@ __global__ void __launch_bounds__(kT) k_far_w(TreeView t, const int* __restrict_
     far[3 * (size_t)i + 2] = fz;
 }
 
-// ---- P2P over U + final combine: one 64-thread block per target leaf,
-// 64-body source tiles staged in LDS; the self pair is skipped by index.
+// ---- P2P over U + final combine (AdaptiveFMM3D.hpp:96-106) ------------------
+// One wave per target leaf. The source bodies of U(b) (contiguous ranges in
+// sorted order) are streamed as one concatenated sequence in full 64-body LDS
+// tiles. The wave is split into G = 64/T lane groups (T = smallest power of two
+// >= the leaf's body count, at least 8); group g takes the tile's sources
+// g, g+G, ... so short leaves still keep all lanes busy. Partial sums are
+// reduced across groups in a fixed order (deterministic). The kernel is
+// (|d|^2 + eps^2)^{-3/2} from v_rsq_f64 refined by two Newton steps; the self
+// pair is skipped by index, as in the reference.
+constexpr int kP2PMaxU = 256;
+
+__device__ __forceinline__ double rsqrt_f64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * (1.5 - hx * y * y);
+    y = y * (1.5 - hx * y * y);
+    return y;
+}
+
 __global__ void __launch_bounds__(64) k_p2p(TreeView t, double G, double eps2, const double* __restrict__ far,
                                            double* __restrict__ acc) {
     __shared__ double sx[64], sy[64], sz[64], sm[64];
+    __shared__ int sj[64];
+    __shared__ int ufirst[kP2PMaxU], ucount[kP2PMaxU];
+    __shared__ double red[3][64];
     const int li = blockIdx.x;
     const int b = t.leaf_ids[li];
-    const int f = t.seg_first[b], cnt = t.seg_count[b];
+    const int f = t.seg_first[b], nt = t.seg_count[b];
     const int lane = threadIdx.x;
-    const int ua = t.u_off[li], ue = t.u_off[li + 1];
-    for (int tb = 0; tb < cnt; tb += 64) {
-        const int i = f + tb + lane;
-        const bool act = tb + lane < cnt;
+    const int ua = t.u_off[li], nu = t.u_off[li + 1] - ua;
+    const bool cached = nu <= kP2PMaxU;
+    if (cached)
+        for (int e = lane; e < nu; e += 64) {
+            const int c = t.u_idx[ua + e];
+            ufirst[e] = t.seg_first[c];
+            ucount[e] = t.seg_count[c];
+        }
+    __syncthreads();
+    auto range = [&](int e, int& first, int& cnt) {
+        if (cached) {
+            first = ufirst[e];
+            cnt = ucount[e];
+        } else {
+            const int c = t.u_idx[ua + e];
+            first = t.seg_first[c];
+            cnt = t.seg_count[c];
+        }
+    };
+    int T = 64;
+    while (T > 8 && T / 2 >= nt) T >>= 1;
+    const int Gs = 64 / T;
+    const int g = lane / T, tl = lane % T;
+    for (int tb = 0; tb < nt; tb += T) {
+        const int i = f + tb + tl;
+        const bool act = tb + tl < nt;
         const double xi = act ? t.x[i] : 0.0, yi = act ? t.y[i] : 0.0, zi = act ? t.z[i] : 0.0;
         double ax = 0.0, ay = 0.0, az = 0.0;
-        for (int e = ua; e < ue; ++e) {
-            const int c = t.u_idx[e];
-            const int cf = t.seg_first[c], cc = t.seg_count[c];
-            for (int sbase = 0; sbase < cc; sbase += 64) {
-                const int j = cf + sbase + lane;
-                if (sbase + lane < cc) {
-                    sx[lane] = t.x[j];
-                    sy[lane] = t.y[j];
-                    sz[lane] = t.z[j];
-                    sm[lane] = t.m[j];
+        int e = 0, off = 0;  // stream position: range e, offset off
+        while (e < nu) {
+            // lane takes stream position (e, off) + lane
+            int ee = e, oo = off + lane, fst = 0, cnt = 0;
+            bool ok = false;
+            while (ee < nu) {
+                range(ee, fst, cnt);
+                if (oo < cnt) {
+                    ok = true;
+                    break;
                 }
-                __syncthreads();
-                const int ns = min(64, cc - sbase);
-                for (int k = 0; k < ns; ++k) {
-                    const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
-                    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-                    const double s = sm[k] / (r2 * sqrt(r2));
-                    if (cf + sbase + k != i) {
-                        ax += s * dx;
-                        ay += s * dy;
-                        az += s * dz;
-                    }
-                }
-                __syncthreads();
+                oo -= cnt;
+                ++ee;
             }
+            if (ok) {
+                const int j = fst + oo;
+                sx[lane] = t.x[j];
+                sy[lane] = t.y[j];
+                sz[lane] = t.z[j];
+                sm[lane] = t.m[j];
+                sj[lane] = j;
+            } else {
+                sj[lane] = -1;
+            }
+            // advance the stream by 64 bodies (wave-uniform)
+            {
+                int rem = 64;
+                while (e < nu) {
+                    int f2, c2;
+                    range(e, f2, c2);
+                    if (off + rem < c2) {
+                        off += rem;
+                        rem = 0;
+                        break;
+                    }
+                    rem -= c2 - off;
+                    off = 0;
+                    ++e;
+                }
+            }
+            __syncthreads();
+            for (int k = g; k < 64; k += Gs) {
+                const int j = sj[k];
+                if (j < 0) break;
+                const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+                const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                const double rin = rsqrt_f64(r2);
+                const double s = (j != i) ? sm[k] * (rin * rin * rin) : 0.0;
+                ax += s * dx;
+                ay += s * dy;
+                az += s * dz;
+            }
+            __syncthreads();
         }
-        if (act) {
+        if (Gs > 1) {
+            red[0][lane] = ax;
+            red[1][lane] = ay;
+            red[2][lane] = az;
+            __syncthreads();
+            if (g == 0) {
+                for (int q = 1; q < Gs; ++q) {
+                    ax += red[0][q * T + tl];
+                    ay += red[1][q * T + tl];
+                    az += red[2][q * T + tl];
+                }
+            }
+            __syncthreads();
+        }
+        if (act && g == 0) {
             const int o = t.orig[i];
             acc[3 * (size_t)o] = G * (far[3 * (size_t)i] + ax);
             acc[3 * (size_t)o + 1] = G * (far[3 * (size_t)i + 1] + ay);
@@ -509,45 +586,35 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (ev) AFMM_CHECK(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL(k_p2m<P>, dim3(t.nleaves), dim3(64), 0, s, t, M);
     for (int l = t.max_level - 1; l >= 0; --l) {
-        const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
-        hipLaunchKernelGGL(k_m2m<P>, dim3(cnt), dim3(64), 0, s, t, off, cnt, tab.m2m, M);
+        const int off = meta.nonleaf_off[l], cnt = meta.nonleaf_off[l + 1] - off;
+        if (cnt > 0)
+            hipLaunchKernelGGL((k_level_mfma<P, 0>), dim3((cnt + 15) / 16), dim3(64), 0, s, cnt, t.nonleaf_ids + off,
+                               t.child_first, t.child_mask, t.parent, t.octant, tab.m2m_sw, M, M);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[1], s));
     {
-        // target order, tile count and transposed source table
-        const int np = (nb + m2l_tile<P>() - 1) / m2l_tile<P>() * m2l_tile<P>();
-        uint64_t* khi = pool.get<uint64_t>("m2l_khi", 2 * (size_t)nb);
-        uint32_t* klo = pool.get<uint32_t>("m2l_klo", 2 * (size_t)nb);
-        int* val = pool.get<int>("m2l_val", 2 * (size_t)nb);
-        SortBufs sb;
-        sb.khi[0] = khi, sb.khi[1] = khi + nb;
-        sb.klo[0] = klo, sb.klo[1] = klo + nb;
-        sb.val[0] = val, sb.val[1] = val + nb;
-        sb.hist = pool.get<int>("m2l_hist", 512 * (size_t)sort_blocks(nb));
-        sb.scan_tmp = pool.get<int>("m2l_scan", scan_tmp_ints(512 * sort_blocks(nb) + nb) + 16);
+        // M2L targets in (level, octant) order; transposed source table; tile plan
+        constexpr int TILE = m2l_tile<P>();
+        const int np = (nb + TILE - 1) / TILE * TILE;
+        const int nwg = np / TILE;
         int* rank = pool.get<int>("m2l_rank", nb);
-        int* ntgt = pool.get<int>("m2l_ntgt", 1);
         int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
-        hipLaunchKernelGGL(k_m2l_keys, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.nv, t.level, t.octant, sb.khi[0],
-                           sb.klo[0], sb.val[0]);
-        const int r = radix_sort(sb, nb, 1, 1, s);
-        hipLaunchKernelGGL(k_m2l_rank, dim3(blocks(nb)), dim3(kT), 0, s, nb, sb.val[r], t.nv, rank, ntgt);
+        unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)nwg * kNumOffsets);
+        hipLaunchKernelGGL(k_rank_of, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.lo_order, rank);
         hipLaunchKernelGGL(k_fill_int, dim3(blocks((long long)kNumOffsets * np)), dim3(kT), 0, s, srcT,
                            (size_t)kNumOffsets * np, -1);
         hipLaunchKernelGGL(k_m2l_src, dim3(blocks(nb)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv, t.ic, srcT);
-        constexpr int TILE = m2l_tile<P>();
-        const int nwg = (nb + TILE - 1) / TILE;
-        unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)nwg * kNumOffsets);
-        hipLaunchKernelGGL(k_m2l_plan, dim3(nwg * kNumOffsets), dim3(64), 0, s, TILE, np, ntgt, srcT, present);
+        hipLaunchKernelGGL(k_m2l_plan, dim3(nwg * kNumOffsets), dim3(64), 0, s, TILE, np, nb, srcT, present);
         if (ev) AFMM_CHECK(hipEventRecord(ev[5], s));
-        hipLaunchKernelGGL(k_m2l_mfma<P>, dim3(nwg), dim3(kM2LWaves * 64), 0, s, nb, np, ntgt, sb.val[r], srcT,
-                           present, tab.m2l_sw, M, L);
+        hipLaunchKernelGGL(k_m2l_mfma<P>, dim3(nwg), dim3(kM2LWaves * 64), 0, s, nb, np, nb, t.lo_order, srcT, present,
+                           tab.m2l_sw, M, L);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
     if (t.n_x > 0) hipLaunchKernelGGL(k_p2l<P>, dim3(nb), dim3(64), 0, s, t, L);
     for (int l = 1; l <= t.max_level; ++l) {
         const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
-        hipLaunchKernelGGL(k_l2l<P>, dim3(cnt), dim3(64), 0, s, t, off, tab.l2l, L);
+        hipLaunchKernelGGL((k_level_mfma<P, 1>), dim3((cnt + 15) / 16), dim3(64), 0, s, cnt, t.lo_order + off,
+                           t.child_first, t.child_mask, t.parent, t.octant, tab.l2l_sw, L, L);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far);

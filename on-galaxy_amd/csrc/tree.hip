@@ -100,13 +100,12 @@ __global__ void __launch_bounds__(kT) k_bbox_partial(const double* pos, const do
 }
 
 // AdaptiveOctree::build (:71-81) from the partials; also the key depth.
-__global__ void k_bbox_final(const double* pv, const int* pi, const int* pbad, int nparts,
-                             double eps_floor, RootInfo* root) {
-    if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(kT) k_bbox_final(const double* pv, const int* pi, const int* pbad, int nparts,
+                                                  double eps_floor, RootInfo* root) {
     Ext e[6];
     for (int a = 0; a < 3; ++a) e[a] = Ext{1e300, -1}, e[3 + a] = Ext{-1e300, -1};
     int bad = 0;
-    for (int b = 0; b < nparts; ++b) {
+    for (int b = threadIdx.x; b < nparts; b += kT) {
         for (int k = 0; k < 6; ++k) {
             const Ext x{pv[k * nparts + b], pi[k * nparts + b]};
             if (k < 3) take_lo(e[k], x);
@@ -114,11 +113,31 @@ __global__ void k_bbox_final(const double* pv, const int* pi, const int* pbad, i
         }
         bad += pbad[b];
     }
+    __shared__ double sv[6][kT];
+    __shared__ int si[6][kT];
+    __shared__ int sb[kT];
+    for (int k = 0; k < 6; ++k) sv[k][threadIdx.x] = e[k].v, si[k][threadIdx.x] = e[k].i;
+    sb[threadIdx.x] = bad;
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const int o = threadIdx.x + off;
+            for (int k = 0; k < 6; ++k) {
+                Ext x{sv[k][threadIdx.x], si[k][threadIdx.x]}, y{sv[k][o], si[k][o]};
+                if (k < 3) take_lo(x, y);
+                else take_hi(x, y);
+                sv[k][threadIdx.x] = x.v, si[k][threadIdx.x] = x.i;
+            }
+            sb[threadIdx.x] += sb[o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     RootInfo r;
-    for (int a = 0; a < 3; ++a) r.c[a] = 0.5 * (e[a].v + e[3 + a].v);
-    double ext = e[3].v - e[0].v;
-    if (ext < e[4].v - e[1].v) ext = e[4].v - e[1].v;
-    if (ext < e[5].v - e[2].v) ext = e[5].v - e[2].v;
+    for (int a = 0; a < 3; ++a) r.c[a] = 0.5 * (sv[a][0] + sv[3 + a][0]);
+    double ext = sv[3][0] - sv[0][0];
+    if (ext < sv[4][0] - sv[1][0]) ext = sv[4][0] - sv[1][0];
+    if (ext < sv[5][0] - sv[2][0]) ext = sv[5][0] - sv[2][0];
     double half = 0.5 * ext;
     if (!(half > 0)) half = 1.0;
     half *= 1.0 + 1e-9;
@@ -132,7 +151,7 @@ __global__ void k_bbox_final(const double* pv, const int* pi, const int* pbad, i
         h = 0.5 * h;
     }
     r.key_levels = L;
-    r.nonfinite = bad;
+    r.nonfinite = sb[0];
     *root = r;
 }
 
@@ -275,6 +294,25 @@ __global__ void __launch_bounds__(kT) k_refid(int nb, const int* pre, const int*
     for (int r = 0; r < k; ++r) refid[child_first[b] + r] = first + r;
 }
 
+// ---- (level, octant) order and non-leaf lists ------------------------------------
+__global__ void __launch_bounds__(kT) k_lo_keys(int nb, const int* level, const int* octant, uint64_t* khi,
+                                               uint32_t* klo, int* val) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    khi[b] = uint64_t(level[b] * 8 + octant[b]);
+    klo[b] = 0;
+    val[b] = b;
+}
+__global__ void __launch_bounds__(kT) k_nonleaf_ids(int nb, const int* nchild, const int* leaf_scan, int* ids) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b < nb && nchild[b] > 0) ids[b - leaf_scan[b]] = b;
+}
+// leaf_scan at each level boundary (levels given by host offsets)
+__global__ void k_gather_at(const int* a, const int* idx, int n, int total, int* out) {
+    const int i = threadIdx.x;
+    if (i < n) out[i] = idx[i] < total ? a[idx[i]] : -1;
+}
+
 // ---- leaves and the in-leaf input order ------------------------------------------
 __global__ void __launch_bounds__(kT) k_leaf_flags(int nb, const int* nchild, int* flag) {
     const int b = blockIdx.x * kT + threadIdx.x;
@@ -357,48 +395,61 @@ __device__ __forceinline__ bool adjacent(const double* cfp, const double* half, 
 
 // Colleagues and V of every box of one level (AdaptiveOctree.hpp:143-161):
 // children of the parent's colleagues, colleague order then octant order.
-__global__ void __launch_bounds__(kT) k_coll_v(int off, int cnt, double tol, const int* parent,
-                                              const int* child_first, const unsigned char* mask,
-                                              const double* cfp, const double* half, const int* ic,
-                                              int* coll, int* ncoll, int* vl, int* nv, int* overflow) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= cnt) return;
-    const int b = off + i;
+// One wave per box: lane k takes candidate k = 8*slot + octant (<= 216), and
+// ballots keep the emitted order (= ascending id = Morton order in a level).
+__global__ void __launch_bounds__(64) k_coll_v(int off, double tol, const int* __restrict__ parent,
+                                              const int* __restrict__ child_first,
+                                              const unsigned char* __restrict__ mask, const double* __restrict__ cfp,
+                                              const double* __restrict__ half, const int* __restrict__ ic,
+                                              int* __restrict__ coll, int* __restrict__ ncoll, int* __restrict__ vl,
+                                              int* __restrict__ nv, int* __restrict__ overflow) {
+    const int b = off + blockIdx.x;
+    const int lane = threadIdx.x;
     if (b == 0) {
-        coll[0] = 0;
-        ncoll[0] = 1;
-        nv[0] = 0;
+        if (lane == 0) {
+            coll[0] = 0;
+            ncoll[0] = 1;
+            nv[0] = 0;
+        }
         return;
     }
     const int par = parent[b];
+    const int npc = ncoll[par];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int kc = 0, kv = 0;
-    for (int s = 0; s < ncoll[par]; ++s) {
-        const int pc = coll[kCollSlots * (size_t)par + s];
-        const unsigned mk = mask[pc];
-        int r = 0;
-        for (int o = 0; o < 8; ++o) {
-            if (!((mk >> o) & 1)) continue;
-            const int c = child_first[pc] + r++;
-            if (adjacent(cfp, half, tol, b, c)) {
-                if (kc < kCollSlots) coll[kCollSlots * (size_t)b + kc] = c;
-                else atomicOr(overflow, 1);
-                ++kc;
-            } else {
-                if (kv < kVSlots) {
-                    vl[kVSlots * (size_t)b + kv] = c;
-                    const int dx = ic[3 * (size_t)b] - ic[3 * (size_t)c];
-                    const int dy = ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1];
-                    const int dz = ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2];
-                    if (dx < -3 || dx > 3 || dy < -3 || dy > 3 || dz < -3 || dz > 3) atomicOr(overflow, 2);
-                } else {
-                    atomicOr(overflow, 1);
-                }
-                ++kv;
+    for (int base = 0; base < 8 * npc; base += 64) {
+        const int k = base + lane;
+        const int slot = k >> 3, o = k & 7;
+        int c = -1;
+        bool adj = false;
+        if (slot < npc) {
+            const int pc = coll[kCollSlots * (size_t)par + slot];
+            const unsigned mk = mask[pc];
+            if ((mk >> o) & 1u) {
+                c = child_first[pc] + __popc(mk & ((1u << o) - 1u));
+                adj = adjacent(cfp, half, tol, b, c);
             }
         }
+        const uint64_t bc = __ballot(c >= 0 && adj), bv = __ballot(c >= 0 && !adj);
+        if (c >= 0 && adj) {
+            const int pos = kc + __popcll(bc & lt);
+            if (pos < kCollSlots) coll[kCollSlots * (size_t)b + pos] = c;
+            else atomicOr(overflow, 1);
+        } else if (c >= 0) {
+            const int pos = kv + __popcll(bv & lt);
+            if (pos < kVSlots) vl[kVSlots * (size_t)b + pos] = c;
+            else atomicOr(overflow, 1);
+            const int dx = ic[3 * (size_t)b] - ic[3 * (size_t)c], dy = ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1],
+                      dz = ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2];
+            if (dx < -3 || dx > 3 || dy < -3 || dy > 3 || dz < -3 || dz > 3) atomicOr(overflow, 2);
+        }
+        kc += __popcll(bc);
+        kv += __popcll(bv);
     }
-    ncoll[b] = kc < kCollSlots ? kc : kCollSlots;
-    nv[b] = kv < kVSlots ? kv : kVSlots;
+    if (lane == 0) {
+        ncoll[b] = kc < kCollSlots ? kc : kCollSlots;
+        nv[b] = kv < kVSlots ? kv : kVSlots;
+    }
 }
 
 // Next box in DFS order after finishing `node`'s subtree, or -1 on reaching `stop`.
@@ -412,61 +463,74 @@ __device__ __forceinline__ int dfs_next(int node, int stop, const int* parent, c
     return -1;
 }
 
-// U(b): non-empty leaves adjacent to b, DFS from the root (:164, :176-180).
-// mode 0 counts, mode 1 writes.
-__global__ void __launch_bounds__(kT) k_ulist(int nleaves, const int* leaf_ids, double tol, const int* parent,
-                                             const int* child_first, const int* nchild, const double* cfp,
-                                             const double* half, int mode, int* cnt, const int* off, int* out) {
+// U and W of each leaf b in one pass (AdaptiveOctree.hpp:163-169, :176-189).
+// U = every non-empty leaf adjacent to b. A coarser or same-level one is a
+// leaf colleague of b's ancestor at its own level (it contains a box adjacent
+// to b); a finer one lies under a non-leaf colleague of b. The descent into
+// each non-leaf colleague (colleague order, octant DFS) is the reference's
+// descendW: a separated box goes to W and is not entered, an adjacent leaf is
+// a finer U entry, an adjacent non-leaf is entered. U is then put in the
+// reference's DFS-from-root order, which for disjoint leaves is ascending
+// first body. mode 0 counts, mode 1 writes.
+__global__ void __launch_bounds__(kT) k_uwlist(int nleaves, const int* __restrict__ leaf_ids, double tol,
+                                              const int* __restrict__ parent, const int* __restrict__ child_first,
+                                              const int* __restrict__ nchild, const int* __restrict__ seg_first,
+                                              const int* __restrict__ level, const double* __restrict__ cfp,
+                                              const double* __restrict__ half, const int* __restrict__ coll,
+                                              const int* __restrict__ ncoll, int mode, int* __restrict__ ucnt,
+                                              int* __restrict__ wcnt, const int* __restrict__ uoff,
+                                              const int* __restrict__ woff, int* __restrict__ uidx,
+                                              int* __restrict__ widx) {
     const int li = blockIdx.x * kT + threadIdx.x;
     if (li >= nleaves) return;
     const int b = leaf_ids[li];
-    int k = 0;
-    int* dst = mode ? out + off[li] : nullptr;
-    int node = 0;
-    while (node >= 0) {
-        if (adjacent(cfp, half, tol, node, b)) {
-            if (nchild[node] == 0) {
-                if (mode) dst[k] = node;
-                ++k;
-                node = dfs_next(node, 0, parent, child_first, nchild);
-            } else {
-                node = child_first[node];
+    int ku = 0, kw = 0;
+    int* ud = mode ? uidx + uoff[li] : nullptr;
+    int* wd = mode ? widx + woff[li] : nullptr;
+    // coarser and same-level adjacent leaves
+    for (int a = b;; a = parent[a]) {
+        for (int sl = 0; sl < ncoll[a]; ++sl) {
+            const int c = coll[kCollSlots * (size_t)a + sl];
+            if (nchild[c] == 0 && adjacent(cfp, half, tol, c, b)) {
+                if (mode) ud[ku] = c;
+                ++ku;
             }
-        } else {
-            node = dfs_next(node, 0, parent, child_first, nchild);
         }
+        if (a == 0) break;
     }
-    if (!mode) cnt[li] = k;
-}
-
-// W(b): for each non-leaf colleague c != b, DFS through adjacent non-leaf
-// boxes of c's subtree, emitting separated boxes (:167-169, :181-189).
-__global__ void __launch_bounds__(kT) k_wlist(int nleaves, const int* leaf_ids, double tol, const int* parent,
-                                             const int* child_first, const int* nchild, const double* cfp,
-                                             const double* half, const int* coll, const int* ncoll, int mode,
-                                             int* cnt, const int* off, int* out) {
-    const int li = blockIdx.x * kT + threadIdx.x;
-    if (li >= nleaves) return;
-    const int b = leaf_ids[li];
-    int k = 0;
-    int* dst = mode ? out + off[li] : nullptr;
-    for (int s = 0; s < ncoll[b]; ++s) {
-        const int c = coll[kCollSlots * (size_t)b + s];
+    // finer adjacent leaves (U) and W, descending non-leaf colleagues of b
+    for (int sl = 0; sl < ncoll[b]; ++sl) {
+        const int c = coll[kCollSlots * (size_t)b + sl];
         if (c == b || nchild[c] == 0) continue;
         int node = child_first[c];
         while (node >= 0) {
             if (!adjacent(cfp, half, tol, node, b)) {
-                if (mode) dst[k] = node;
-                ++k;
+                if (mode) wd[kw] = node;
+                ++kw;
                 node = dfs_next(node, c, parent, child_first, nchild);
             } else if (nchild[node] > 0) {
                 node = child_first[node];
             } else {
+                if (mode) ud[ku] = node;
+                ++ku;
                 node = dfs_next(node, c, parent, child_first, nchild);
             }
         }
     }
-    if (!mode) cnt[li] = k;
+    if (!mode) {
+        ucnt[li] = ku;
+        wcnt[li] = kw;
+        return;
+    }
+    for (int i = 1; i < ku; ++i) {  // ascending first body = DFS order
+        const int v = ud[i], key = seg_first[v];
+        int j2 = i - 1;
+        while (j2 >= 0 && seg_first[ud[j2]] > key) {
+            ud[j2 + 1] = ud[j2];
+            --j2;
+        }
+        ud[j2 + 1] = v;
+    }
 }
 
 // X = transpose of W (:172-173).
@@ -519,7 +583,7 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     RootInfo* d_root = pool.get<RootInfo>("root", 1);
     const int nbb = std::min(kBBoxBlocks, blocks(n));
     hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kT), 0, s, d_pos, d_mass, n, pv, pi, pbad);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, pv, pi, pbad, nbb, eps_floor, d_root);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kT), 0, s, pv, pi, pbad, nbb, eps_floor, d_root);
     AFMM_CHECK(hipMemcpyAsync(&root, d_root, sizeof(RootInfo), hipMemcpyDeviceToHost, s));
     AFMM_CHECK(hipStreamSynchronize(s));
     if (root.nonfinite) return;
@@ -536,7 +600,7 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     }
     sb.hist = pool.get<int>("sort_hist", 512 * (size_t)nsb);
     const int scan_cap = std::max(n, 512 * nsb) + 1;
-    int* scan_tmp = pool.get<int>("scan_tmp", scan_tmp_ints(scan_cap) + 16);
+    int* scan_tmp = pool.get<int>("scan_tmp", 2 * scan_tmp_ints(scan_cap) + 16);
     sb.scan_tmp = scan_tmp;
     hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kT), 0, s, d_pos, n, d_root, sb.khi[0], sb.klo[0], sb.val[0]);
     const int L = root.key_levels;
@@ -643,6 +707,36 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     hipLaunchKernelGGL(k_geometry, dim3(blocks(nb)), dim3(kT), 0, s, nb, d_root, tv.seg_first, tv.level, khi, klo,
                        tv.cfp, tv.cdd, tv.half, tv.invh, tv.ic);
 
+    // ---- (level, octant) order; non-leaf boxes per level
+    {
+        SortBufs lb;
+        for (int k = 0; k < 2; ++k) {
+            lb.khi[k] = pool.get<uint64_t>(k ? "lo_khi1" : "lo_khi0", nb);
+            lb.klo[k] = pool.get<uint32_t>(k ? "lo_klo1" : "lo_klo0", nb);
+            lb.val[k] = pool.get<int>(k ? "lo_val1" : "lo_val0", nb);
+        }
+        lb.hist = pool.get<int>("lo_hist", 512 * (size_t)sort_blocks(nb));
+        lb.scan_tmp = pool.get<int>("lo_scan", scan_tmp_ints(512 * sort_blocks(nb) + nb) + 16);
+        hipLaunchKernelGGL(k_lo_keys, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.level, tv.octant, lb.khi[0],
+                           lb.klo[0], lb.val[0]);
+        const int r = radix_sort(lb, nb, 1, 1, s);
+        tv.lo_order = pool.get<int>("lo_order", nb);
+        AFMM_CHECK(hipMemcpyAsync(tv.lo_order, lb.val[r], sizeof(int) * nb, hipMemcpyDeviceToDevice, s));
+        tv.nonleaf_ids = pool.get<int>("nonleaf_ids", nb);
+        hipLaunchKernelGGL(k_nonleaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.nonleaf_ids);
+        const int nlev = (int)meta.level_off.size();
+        std::vector<int> bounds(meta.level_off), got(nlev);
+        int* d_idx = pool.get<int>("lvl_idx", nlev);
+        int* d_out = pool.get<int>("lvl_out", nlev);
+        AFMM_CHECK(hipMemcpyAsync(d_idx, bounds.data(), sizeof(int) * nlev, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_idx, nlev, nb, d_out);
+        AFMM_CHECK(hipMemcpyAsync(got.data(), d_out, sizeof(int) * nlev, hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
+        meta.nonleaf_off.resize(nlev);
+        for (int l = 0; l < nlev; ++l)
+            meta.nonleaf_off[l] = bounds[l] - (bounds[l] < nb ? got[l] : tv.nleaves);
+    }
+
     // ---- colleagues, V, M2L source table
     tv.coll = pool.get<int>("coll", kCollSlots * (size_t)nb);
     tv.ncoll = pool.get<int>("ncoll", nb);
@@ -652,34 +746,35 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, overflow, 1, 0);
     for (int lv = 0; lv <= tv.max_level; ++lv) {
         const int off = meta.level_off[lv], cnt = meta.level_off[lv + 1] - off;
-        hipLaunchKernelGGL(k_coll_v, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, root.tol, tv.parent,
-                           tv.child_first, tv.child_mask, tv.cfp, tv.half, tv.ic, tv.coll, tv.ncoll, tv.vl,
-                           tv.nv, overflow);
+        hipLaunchKernelGGL(k_coll_v, dim3(cnt), dim3(64), 0, s, off, root.tol, tv.parent, tv.child_first,
+                           tv.child_mask, tv.cfp, tv.half, tv.ic, tv.coll, tv.ncoll, tv.vl, tv.nv, overflow);
     }
     if (read_int(overflow, s) != 0) throw std::runtime_error("interaction-list slot overflow");
 
     // ---- U and W (count, scan, fill), X transpose
     const int nl = tv.nleaves;
     int* ucnt = pool.get<int>("ucnt", nl + 1);
-    tv.u_off = pool.get<int>("u_off", nl + 1);
-    hipLaunchKernelGGL(k_ulist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
-                       tv.child_first, tv.nchild, tv.cfp, tv.half, 0, ucnt, nullptr, nullptr);
-    scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
-    tv.n_u = read_int(tv.u_off + nl, s);
-    tv.u_idx = pool.get<int>("u_idx", tv.n_u);
-    hipLaunchKernelGGL(k_ulist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
-                       tv.child_first, tv.nchild, tv.cfp, tv.half, 1, nullptr, tv.u_off, tv.u_idx);
-
     int* wcnt = pool.get<int>("wcnt", nl + 1);
+    tv.u_off = pool.get<int>("u_off", nl + 1);
     tv.w_off = pool.get<int>("w_off", nl + 1);
-    hipLaunchKernelGGL(k_wlist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
-                       tv.child_first, tv.nchild, tv.cfp, tv.half, tv.coll, tv.ncoll, 0, wcnt, nullptr, nullptr);
-    scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp, s);
-    tv.n_w = read_int(tv.w_off + nl, s);
+    hipLaunchKernelGGL(k_uwlist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
+                       tv.child_first, tv.nchild, tv.seg_first, tv.level, tv.cfp, tv.half, tv.coll, tv.ncoll, 0,
+                       ucnt, wcnt, nullptr, nullptr, nullptr, nullptr);
+    scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
+    scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp + scan_tmp_ints(scan_cap), s);
+    {
+        int tot[2];
+        AFMM_CHECK(hipMemcpyAsync(&tot[0], tv.u_off + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipMemcpyAsync(&tot[1], tv.w_off + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
+        tv.n_u = tot[0];
+        tv.n_w = tot[1];
+    }
+    tv.u_idx = pool.get<int>("u_idx", tv.n_u);
     tv.w_idx = pool.get<int>("w_idx", tv.n_w);
-    hipLaunchKernelGGL(k_wlist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
-                       tv.child_first, tv.nchild, tv.cfp, tv.half, tv.coll, tv.ncoll, 1, nullptr, tv.w_off,
-                       tv.w_idx);
+    hipLaunchKernelGGL(k_uwlist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
+                       tv.child_first, tv.nchild, tv.seg_first, tv.level, tv.cfp, tv.half, tv.coll, tv.ncoll, 1,
+                       nullptr, nullptr, tv.u_off, tv.w_off, tv.u_idx, tv.w_idx);
 
     int* xcnt = pool.get<int>("xcnt", nb + 1);
     int* xcur = pool.get<int>("xcur", nb + 1);
