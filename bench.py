@@ -13,6 +13,14 @@ HBM when timing starts. Prints one JSON line. Extra fields:
                 of the same bodies, scaled linearly in N to the full workload
   phases_ms     per-phase device time (hipEvents) averaged over timed steps
   host_io_s_per_eval  afmm_eval with host arrays in and out (PCIe-inclusive)
+  mixed         the same evaluation in mixed precision (split-FP32 MFMA M2L,
+                FP32 P2P with FP64 accumulation): s/eval, speedup, phases
+  accuracy      relative RMS and max relErr (floor 1e-6 max|a|) of both forms
+                against FP64 direct summation at random targets (GPU)
+  batched       256 systems in one batched evaluation vs 256 sequential solves
+                (aggregate body-interactions/s), leaf capacity autotuned
+  serving       a Poisson job stream through the continuous-batching server vs
+                one job at a time: p50/p99 time-to-result, bodies*steps/s
 """
 import argparse
 import json
@@ -30,13 +38,13 @@ PEAK_FP64_TFLOPS = 78.6
 def load_pkg():
     import importlib.util
     out = {}
-    for name in ("afmm", "ics"):
+    for name in ("afmm", "ics", "tune"):
         spec = importlib.util.spec_from_file_location(f"on_galaxy_amd_{name}",
                                                       os.path.join(ROOT, "on-galaxy_amd", f"{name}.py"))
         mod = importlib.util.module_from_spec(spec)
         spec.loader.exec_module(mod)
         out[name] = mod
-    return out["afmm"], out["ics"]
+    return out["afmm"], out["ics"], out["tune"]
 
 
 def cpu_baseline(pos, mass, args):
@@ -59,6 +67,137 @@ def cpu_baseline(pos, mass, args):
                       f"value = that x N/{args.cpu_sample}"}
 
 
+def batched_leg(afmm, ics, tune, torch, args):
+    """256 independent systems (N log-uniform over 2k-200k; even: Plummer a=1,
+    odd: Hernquist a=0.5 truncated at 20a; total mass 1 each), p=8,
+    eps=0.01, mixed precision, leaf capacity from the autotuner: one batched
+    evaluation (one launch per FMM phase for all systems) against 256
+    sequential plain (FP64, leaf capacity 64) solves of the same systems."""
+    rng = np.random.default_rng(args.seed + 3)
+    ns = np.exp(rng.uniform(np.log(2000), np.log(200000), args.batch_systems)).astype(np.int64)
+    parts = []
+    for q, nq in enumerate(ns):
+        if q % 2 == 0:
+            pq, _, mq = ics.plummer(int(nq), seed=1000 + q)
+        else:
+            pq, _, mq = ics.hernquist(int(nq), a=0.5, seed=1000 + q, velocities=False)
+        parts.append((pq, mq))
+    off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    dev = torch.device("cuda:0")
+    d_pos = torch.from_numpy(np.ascontiguousarray(np.concatenate([p for p, _ in parts]))).to(dev)
+    d_mass = torch.from_numpy(np.ascontiguousarray(np.concatenate([m for _, m in parts]))).to(dev)
+    d_acc = torch.empty_like(d_pos)
+    P, E = d_pos.data_ptr(), d_mass.data_ptr()
+    A = d_acc.data_ptr()
+    cap, tune_ms = tune.tune_leaf_cap(afmm, off, P, E, A, 8, afmm.PRECISION_MIXED, eps=0.01)
+
+    def timed_batch(precision, leaf_cap, reps=3):
+        s = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=leaf_cap, precision=precision)
+        s.compute_batch_device(off, P, E, A)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.compute_batch_device(off, P, E, A)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, s.stats()
+
+    t_mix, st = timed_batch(afmm.PRECISION_MIXED, cap)
+    t_b64, _ = timed_batch(afmm.PRECISION_FP64, 64)
+    # 256 sequential plain solves (FP64, cap 64), device-resident slices
+    s = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=64)
+    s.compute_device(P, E, A, int(ns[0]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in range(len(ns)):
+        o = int(off[q])
+        s.compute_device(P + 24 * o, E + 8 * o, A + 24 * o, int(ns[q]))
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    inter = float(np.sum(ns.astype(np.float64) * (ns - 1)))
+    return {"systems": int(len(ns)), "bodies": int(off[-1]), "p": 8, "eps": 0.01,
+            "leaf_cap": int(cap), "autotune_ms_total_by_cap": {str(k): round(v, 3) for k, v in tune_ms.items()},
+            "batched_mixed_s_per_step": t_mix, "batched_fp64_cap64_s_per_step": t_b64,
+            "sequential_plain_s_per_step": t_seq,
+            "body_interactions_per_s": inter / t_mix,
+            "sequential_body_interactions_per_s": inter / t_seq,
+            "speedup_vs_sequential": t_seq / t_mix,
+            "actual_p2p_pairs": int(st["p2p_pairs"]), "v_pairs": int(st["n_v"]),
+            "metric_note": "body-interactions/s = sum_i N_i (N_i - 1) / t_step (direct-equivalent)"}
+
+
+def serving_leg(afmm, ics, args):
+    """A seeded Poisson stream (mean 20 jobs/s) of Plummer or two-galaxy jobs,
+    N log-uniform over 10k-2M, p in {4, 8, 12}, 1-50 leapfrog steps of 0.01,
+    through the continuous-batching server, and the same jobs one at a time
+    (each on its own resident leapfrog, in arrival order)."""
+    import threading
+    rng = np.random.default_rng(args.seed + 4)
+    arrivals = np.cumsum(rng.exponential(1.0 / 20.0, args.serve_jobs))
+    jobs = []
+    ic_solver = afmm.AdaptiveFMM3D(eps=0.005, p=4, threshold=64)
+    for k in range(args.serve_jobs):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        p = int(rng.choice([4, 8, 12]))
+        steps = int(rng.integers(1, 51))
+        if rng.random() < 0.5:
+            pos, vel, m = ics.plummer(n, seed=2000 + k)
+            eps = 0.01
+        else:
+            pos, vel, m, _ = ics.merger(max(n // 2, 1), seed=2000 + k, solver=ic_solver)
+            eps = 0.005
+        jobs.append(dict(pos=pos, vel=vel, mass=m, eps=eps, p=p, steps=steps, t=float(arrivals[k])))
+    ic_solver.close()
+    work = float(sum(len(j["mass"]) * j["steps"] for j in jobs))
+
+    def summarize(lat, makespan):
+        lat = np.asarray(lat)
+        return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                "mean_s": float(lat.mean()), "makespan_s": makespan, "bodies_steps_per_s": work / makespan}
+
+    sv = afmm.Server(precision=afmm.PRECISION_FP64)
+    for p in (4, 8, 12):
+        sv.warm(p)
+    ids = [None] * len(jobs)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for k, j in enumerate(jobs):
+            dt = t0 + j["t"] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[k] = sv.submit(j["pos"], j["vel"], j["mass"], 1.0, j["eps"], 0.01, j["p"], j["steps"])
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    sv.drain()
+    infos = [sv.info(i) for i in ids]
+    lat = [i["t_done"] - i["t_submit"] for i in infos]
+    span = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
+    batched = summarize(lat, span)
+    batched["max_jobs_in_one_batch_evals"] = int(max(i["batches"] for i in infos))
+    sv.close()
+    # plain one-job-at-a-time path
+    lat2 = []
+    t0 = time.perf_counter()
+    for j in jobs:
+        dt = t0 + j["t"] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        s = afmm.AdaptiveFMM3D(eps=j["eps"], p=j["p"], threshold=64)
+        st = afmm.LeapfrogState(s, j["pos"], j["vel"], j["mass"])
+        st.step(0.01, j["steps"])
+        st.download()
+        st.close()
+        s.close()
+        lat2.append(time.perf_counter() - (t0 + j["t"]))
+    span2 = time.perf_counter() - t0
+    return {"jobs": len(jobs), "arrival_rate_per_s": 20.0, "total_bodies_steps": work,
+            "n_range": [int(min(len(j["mass"]) for j in jobs)), int(max(len(j["mass"]) for j in jobs))],
+            "orders": sorted({j["p"] for j in jobs}), "precision": "fp64",
+            "server": batched, "one_at_a_time": summarize(lat2, span2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,53 +209,94 @@ def main():
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--targets", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--serve-jobs", type=int, default=16)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (scale for HBM counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(args.copy_calibration << 17, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": a.numel() * 8, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards as described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
     import torch
-    afmm, ics = load_pkg()
+    afmm, ics, tune = load_pkg()
     pos, _, mass, _ = ics.merger(args.n_per_galaxy, seed=args.seed)
     n = len(mass)
     dev = torch.device("cuda:0")
     d_pos = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
     d_mass = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
     d_acc = torch.empty_like(d_pos)
-    solver = afmm.AdaptiveFMM3D(G=1.0, eps=args.eps, p=args.p, threshold=args.cap)
+    def timed(precision):
+        """W warmup + K timed evaluations; returns (s/eval, averaged phase ms, last stats, solver)."""
+        solver = afmm.AdaptiveFMM3D(G=1.0, eps=args.eps, p=args.p, threshold=args.cap, precision=precision)
 
-    def step():
-        solver.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
+        def step():
+            solver.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
 
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    phases = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        phases.append(solver.stats())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    s_per_eval = dt / args.steps
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        phases = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            phases.append(solver.stats())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        keys = ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_leaf", "ms_total")
+        avg = {k[3:]: float(np.mean([ph[k] for ph in phases])) for k in keys}
+        return dt / args.steps, avg, phases[-1], solver
 
-    st = phases[-1]
-    keys = ("ms_tree", "ms_upward", "ms_m2l", "ms_downward", "ms_leaf", "ms_total")
-    avg = {k[3:]: float(np.mean([ph[k] for ph in phases])) for k in keys}
+    s_per_eval, avg, st, solver = timed(afmm.PRECISION_FP64)
+    acc64 = d_acc.cpu().numpy()
     S = (args.p + 1) ** 2
     m2l_flops = 2.0 * S * S * st["n_v"]
     achieved = m2l_flops / (avg["m2l"] * 1e-3) / 1e12
-    roofline = {"kernel": "M2L (k_m2l)", "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+    roofline = {"kernel": "M2L (k_m2l_mfma, FP64 MFMA)", "bound": "mfma", "achieved": achieved,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": None,
                 "algorithmic_per_launch": f"2*S^2*|V| = 2*{S}^2*{st['n_v']} = {m2l_flops:.4e} flop"}
 
     # PCIe-inclusive drop-in semantics (host arrays in and out), reported aside
     t1 = time.perf_counter()
     solver.computeAccelerations(pos, mass)
     host_io = time.perf_counter() - t1
+
+    # mixed precision on the same bodies
+    mix_s, mix_avg, _, _ = timed(afmm.PRECISION_MIXED)
+    accmx = d_acc.cpu().numpy()
+    mix_achieved = m2l_flops / (mix_avg["m2l"] * 1e-3) / 1e12
+
+    # accuracy: FP64 direct summation (on the GPU) at random targets
+    idx = np.random.default_rng(args.seed + 1).choice(n, args.targets, replace=False)
+    ref = solver.direct(pos, mass, idx)
+
+    def errs(ref, a):
+        floor = 1e-30 + 1e-6 * np.linalg.norm(ref, axis=1).max()
+        d = np.linalg.norm(a - ref, axis=1)
+        return {"rms": float(np.sqrt((d ** 2).sum() / (np.linalg.norm(ref, axis=1) ** 2).sum())),
+                "max_rel": float((d / np.maximum(np.linalg.norm(ref, axis=1), floor)).max())}
+
+    accuracy = {"targets": args.targets, "fp64_vs_direct": errs(ref, acc64[idx]),
+                "mixed_vs_direct": errs(ref, accmx[idx]), "mixed_vs_fp64": errs(acc64, accmx)}
+    mixed = {"s_per_eval": mix_s, "speedup_vs_fp64": s_per_eval / mix_s, "phases_ms": mix_avg,
+             "m2l_tflops_algorithmic": mix_achieved,
+             "m2l_form": "split FP32 MFMA (hi/lo operands, 3 products)", "p2p_form": "FP32, FP64 accumulation"}
 
     out = {"metric": METRIC, "value": s_per_eval, "unit": "s/eval", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": s_per_eval * 1e3, "higher_is_better": False,
@@ -128,10 +308,18 @@ def main():
                       "precision": "fp64", "parallelism": "single GPU"},
            "roofline": roofline,
            "phases_ms": avg,
+           "mixed": mixed,
+           "accuracy": accuracy,
            "tree": {"boxes": st["n_boxes"], "leaves": st["n_leaves"], "max_level": st["max_level"],
                     "max_leaf_occ": st["max_leaf_occ"], "v_pairs": st["n_v"], "p2p_pairs": st["p2p_pairs"],
                     "u_entries": st["n_u"], "w_entries": st["n_w"]},
            "host_io_s_per_eval": host_io}
+    if not args.no_batch:
+        out["batched"] = batched_leg(afmm, ics, tune, torch, args)
+    if not args.no_serve:
+        out["serving"] = serving_leg(afmm, ics, args)
+        out["latency_p50_s"] = out["serving"]["server"]["p50_s"]
+        out["latency_p99_s"] = out["serving"]["server"]["p99_s"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, mass, args)
     else:

@@ -34,6 +34,12 @@ typedef enum {
 } afmm_status;
 
 #define AFMM_MAX_ORDER 16
+/* FP64: every phase in FP64 (the reference CPU path's arithmetic).
+ * MIXED: M2L as split-FP32 MFMA products (each operand carried as an FP32
+ * hi/lo pair, three products per block) and P2P in FP32 with FP64
+ * accumulation per target; tree, P2M/M2M/L2L/P2L, L2P/M2P and the pair
+ * differences stay FP64. The CUDA twin's reduced precision is its `real`
+ * template (gpu/AdaptiveFMM3DGPU.cu:219-334). */
 #define AFMM_PRECISION_FP64 0
 #define AFMM_PRECISION_MIXED 1
 
@@ -46,7 +52,7 @@ typedef struct {
     double eps;
     int32_t p;          /* expansion order, 1..AFMM_MAX_ORDER */
     int32_t leaf_cap;   /* split a box while it holds > leaf_cap bodies (64) */
-    int32_t precision;  /* AFMM_PRECISION_FP64 */
+    int32_t precision;  /* AFMM_PRECISION_FP64 or AFMM_PRECISION_MIXED */
     int32_t device;     /* HIP device ordinal */
 } afmm_params;
 
@@ -108,6 +114,136 @@ int afmm_eval(afmm_ctx* ctx, int64_t n, const double* pos_xyz, const double* mas
  * the context's stream; no host round-trip of bodies or results. */
 int afmm_eval_device(afmm_ctx* ctx, int64_t n, const double* d_pos_xyz, const double* d_mass,
                      double* d_acc_xyz);
+
+/* Batched evaluation of nsys independent systems with one launch per FMM
+ * phase over all of them (a forest of octrees, one root per system). System q
+ * owns bodies [sys_off[q], sys_off[q+1]) of the concatenated arrays
+ * (sys_off[0] = 0, strictly increasing: every system has a body) and uses
+ * G[q], eps[q] (either array may be NULL: the context's values). Expansion
+ * order, leaf capacity and precision are the context's. Each system's tree,
+ * accelerations and potentials are those of afmm_eval on that system alone,
+ * bit for bit. Host arrays (afmm_eval_batch) or device arrays
+ * (afmm_eval_batch_device; sys_off, G and eps stay host arrays); phi may be
+ * NULL. There is no reference counterpart: the reference evaluates one
+ * system per call (AdaptiveFMM3D.hpp:29-110). */
+int afmm_eval_batch(afmm_ctx* ctx, int32_t nsys, const int64_t* sys_off, const double* G, const double* eps,
+                    const double* pos_xyz, const double* mass, double* acc_xyz, double* phi);
+int afmm_eval_batch_device(afmm_ctx* ctx, int32_t nsys, const int64_t* sys_off, const double* G,
+                           const double* eps, const double* d_pos_xyz, const double* d_mass, double* d_acc_xyz,
+                           double* d_phi);
+
+/* ---- device-resident time integration ----------------------------------- */
+/* Leapfrog state of nsys independent systems kept in HBM between steps
+ * (bodies, velocities, accelerations), integrated with the reference's
+ * velocity Verlet (VelocityVerlet3D, Integrator3D.hpp:19-40):
+ *   x += v*dt + a*(0.5*dt*dt);  a' = forces(x);  v += (a + a')*(0.5*dt);  a = a'
+ * in the reference's operation order per component, with one batched force
+ * evaluation (afmm_eval_batch semantics: sys_off, per-system G and eps) per
+ * step for all systems. The reference round-trips host arrays through its
+ * solver every step (AdaptiveFMM3DGPU.cu:252-307). */
+typedef struct afmm_state afmm_state;
+
+/* Uploads the systems and computes their initial accelerations
+ * (VelocityVerlet3D::initialize). The state uses ctx (order, leaf capacity,
+ * precision, stream) for its evaluations; destroy it before ctx. */
+int afmm_state_create(afmm_ctx* ctx, int32_t nsys, const int64_t* sys_off, const double* G, const double* eps,
+                      const double* pos_xyz, const double* vel_xyz, const double* mass, afmm_state** out);
+/* nsteps velocity-Verlet steps of every system with time step dt. */
+int afmm_state_step(afmm_state* st, double dt, int32_t nsteps);
+/* Copies the current positions, velocities and accelerations to host arrays
+ * (input order; any pointer may be NULL). */
+int afmm_state_download(afmm_state* st, double* pos_xyz, double* vel_xyz, double* acc_xyz);
+/* Diagnostics3D per system (Diagnostics3D.hpp:20-38) into out[9*nsys]:
+ * {kinetic, potential, total, momentum xyz, angular momentum xyz}. The
+ * potential energy is 0.5 sum_i m_i phi_i with the FMM potential of
+ * afmm_eval_potential (one evaluation), not the reference's O(N^2) pair sum;
+ * the reductions are FP64 in a fixed order. */
+int afmm_state_diagnostics(afmm_state* st, double* out);
+void afmm_state_destroy(afmm_state* st);
+
+/* ---- request server ------------------------------------------------------- */
+/* Continuous batching of independent jobs on one GPU. A job is a system
+ * (n bodies, G, eps, order p) and either one force evaluation (steps = 0) or
+ * `steps` velocity-Verlet steps of size dt (afmm_state semantics). Jobs are
+ * admitted in arrival order while the sum of their footprint estimates
+ * (afmm_job_bytes) fits hbm_budget; admitted jobs of the same order are
+ * stepped together, one batched evaluation per step (afmm_eval_batch), and
+ * jobs that arrive meanwhile join at the next step. Each order runs on its
+ * own worker thread and stream, so batches of different orders overlap. A
+ * job's result is bit-identical to running it alone. The reference has no
+ * counterpart (its drivers step one system per call, gpu/galaxy_sim.cu:230-237). */
+typedef struct afmm_server afmm_server;
+typedef struct {
+    int64_t hbm_budget;  /* bytes of device memory jobs may claim (<= 0: 240 GiB) */
+    int32_t leaf_cap;    /* leaf capacity of every job (64) */
+    int32_t precision;   /* AFMM_PRECISION_FP64 or AFMM_PRECISION_MIXED */
+    int32_t device;      /* HIP device ordinal */
+    int32_t max_batch;   /* most jobs in one batched step of an order (<= 0: no limit) */
+} afmm_server_params;
+typedef struct {
+    int64_t n;
+    const double* pos_xyz;  /* copied at submission */
+    const double* vel_xyz;  /* may be NULL when steps == 0 */
+    const double* mass;
+    double G, eps, dt;
+    int32_t p;              /* expansion order 1..12 */
+    int32_t steps;          /* 0: force evaluation; k > 0: k leapfrog steps */
+} afmm_job_desc;
+#define AFMM_JOB_QUEUED 0
+#define AFMM_JOB_RUNNING 1
+#define AFMM_JOB_DONE 2
+#define AFMM_JOB_FAILED 3
+typedef struct {
+    int32_t state;       /* AFMM_JOB_* */
+    int32_t error;       /* afmm_status of a failed job */
+    int64_t hbm_bytes;   /* admission estimate */
+    double t_submit, t_admit, t_done;  /* seconds since afmm_server_create */
+    int32_t steps_done;  /* leapfrog steps completed */
+    int32_t batches;     /* batched evaluations the job took part in */
+} afmm_job_info;
+
+/* Footprint estimate used for admission: resident arrays, the job's share of
+ * the batch, tree and sort buffers, its expansions and lists. */
+int64_t afmm_job_bytes(int64_t n, int32_t p);
+int afmm_server_create(const afmm_server_params* prm, afmm_server** out);
+/* Prepares the worker of order p (context, translation tables) ahead of jobs. */
+int afmm_server_warm(afmm_server* sv, int32_t p);
+int afmm_server_submit(afmm_server* sv, const afmm_job_desc* job, int64_t* id);
+/* Blocks until the job is done or failed; returns its afmm_status. */
+int afmm_server_wait(afmm_server* sv, int64_t id);
+/* Blocks until every submitted job is done or failed. */
+int afmm_server_drain(afmm_server* sv);
+int afmm_server_info(afmm_server* sv, int64_t id, afmm_job_info* info);
+/* Final positions, velocities and accelerations of a done job (any pointer
+ * may be NULL); the job is then forgotten. */
+int afmm_server_result(afmm_server* sv, int64_t id, double* pos_xyz, double* vel_xyz, double* acc_xyz);
+/* Sum of the footprint estimates of the admitted, unfinished jobs. */
+int64_t afmm_server_hbm_in_use(afmm_server* sv);
+/* Finishes running jobs, fails queued ones, joins the workers. */
+void afmm_server_destroy(afmm_server* sv);
+
+/* FP64 direct summation over all n bodies onto the targets tgt[0..ntgt)
+ * (indices into the bodies), with the context's G and eps: the accuracy
+ * yardstick DirectSolver3D (DirectSolver3D.hpp:15-34) restricted to a target
+ * subsample, and the softened potential of Kernel3D.hpp:39-42:
+ *   acc_xyz[k] = G sum_{j != tgt[k]} m_j d / (|d|^2 + eps^2)^{3/2}
+ *   phi[k]     = -G sum_{j != tgt[k]} m_j / sqrt(|d|^2 + eps^2)   (phi may be NULL)
+ * Host arrays; the sum order is fixed, so results are reproducible. */
+int afmm_direct(afmm_ctx* ctx, int64_t n, const double* pos_xyz, const double* mass, int64_t ntgt,
+                const int64_t* tgt, double* acc_xyz, double* phi);
+
+/* afmm_eval / afmm_eval_device plus the FMM potential per body (input
+ * order) when phi is non-null: phi[i] = -G (far-field expansions at x_i +
+ * sum_{j in U, j != i} m_j / sqrt(|d|^2 + eps^2)), the approximation of
+ * -G sum_{j != i} m_j / sqrt(|d|^2 + eps^2) (Kernel3D.hpp:39-42) whose
+ * gradient the accelerations are; softening acts in the near field only, as
+ * for the accelerations. FP64 in both precision modes except the near-field
+ * sum, which follows the context's precision. 0.5 sum_i m_i phi_i is the
+ * potential energy of Diagnostics3D (Diagnostics3D.hpp:20-38). */
+int afmm_eval_potential(afmm_ctx* ctx, int64_t n, const double* pos_xyz, const double* mass, double* acc_xyz,
+                        double* phi);
+int afmm_eval_device_potential(afmm_ctx* ctx, int64_t n, const double* d_pos_xyz, const double* d_mass,
+                               double* d_acc_xyz, double* d_phi);
 
 /* Block until the context's stream is idle. */
 int afmm_synchronize(afmm_ctx* ctx);

@@ -25,8 +25,10 @@ namespace galaxy3d {
 
 class AdaptiveFMM3D_HIP final : public ForceSolver3D {
 public:
-    AdaptiveFMM3D_HIP(KernelParams3D params, int p, int threshold = 64, int device = 0) : params_(params) {
-        afmm_params prm{params.G, params.eps, p, threshold, AFMM_PRECISION_FP64, device};
+    AdaptiveFMM3D_HIP(KernelParams3D params, int p, int threshold = 64, int device = 0,
+                      int precision = AFMM_PRECISION_FP64)
+        : params_(params), precision_(precision) {
+        afmm_params prm{params.G, params.eps, p, threshold, precision, device};
         afmm_ctx* c = nullptr;
         const int rc = afmm_create(&prm, &c);
         if (rc) throw std::runtime_error(std::string("afmm_create: ") + afmm_strerror(rc));
@@ -51,7 +53,10 @@ public:
         return acc;
     }
 
-    const char* name() const override { return "AdaptiveFMM3D(CGR, MI355X HIP FP64)"; }
+    const char* name() const override {
+        return precision_ == AFMM_PRECISION_MIXED ? "AdaptiveFMM3D(CGR, MI355X HIP mixed)"
+                                                  : "AdaptiveFMM3D(CGR, MI355X HIP FP64)";
+    }
 
     afmm_ctx* context() const { return ctx_.get(); }
 
@@ -63,6 +68,7 @@ private:
         void operator()(afmm_ctx* c) const { afmm_destroy(c); }
     };
     KernelParams3D params_;
+    int precision_;
     std::unique_ptr<afmm_ctx, Deleter> ctx_;
 };
 

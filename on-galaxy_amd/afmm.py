@@ -58,6 +58,27 @@ class Tree(C.Structure):
                 ("x_idx", _ip), ("tol", C.c_double)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("leaf_cap", C.c_int32), ("precision", C.c_int32),
+                ("device", C.c_int32), ("max_batch", C.c_int32)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("n", C.c_int64), ("pos_xyz", _dp), ("vel_xyz", _dp), ("mass", _dp), ("G", C.c_double),
+                ("eps", C.c_double), ("dt", C.c_double), ("p", C.c_int32), ("steps", C.c_int32)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("error", C.c_int32), ("hbm_bytes", C.c_int64), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("steps_done", C.c_int32),
+                ("batches", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+
 _lib = None
 
 
@@ -76,7 +97,32 @@ def library():
         L.afmm_last_error.restype = C.c_char_p
         L.afmm_eval.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp]
         L.afmm_eval_device.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.afmm_eval_potential.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp]
+        L.afmm_eval_device_potential.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p]
         L.afmm_synchronize.argtypes = [C.c_void_p]
+        L.afmm_eval_batch.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp, _dp, _dp]
+        L.afmm_eval_batch_device.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), _dp, _dp, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]
+        L.afmm_state_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp, _dp,
+                                        C.POINTER(C.c_void_p)]
+        L.afmm_state_step.argtypes = [C.c_void_p, C.c_double, C.c_int32]
+        L.afmm_state_download.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.afmm_state_diagnostics.argtypes = [C.c_void_p, _dp]
+        L.afmm_state_destroy.argtypes = [C.c_void_p]
+        L.afmm_job_bytes.argtypes = [C.c_int64, C.c_int32]
+        L.afmm_job_bytes.restype = C.c_int64
+        L.afmm_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(C.c_void_p)]
+        L.afmm_server_warm.argtypes = [C.c_void_p, C.c_int32]
+        L.afmm_server_submit.argtypes = [C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_int64)]
+        L.afmm_server_wait.argtypes = [C.c_void_p, C.c_int64]
+        L.afmm_server_drain.argtypes = [C.c_void_p]
+        L.afmm_server_info.argtypes = [C.c_void_p, C.c_int64, C.POINTER(JobInfo)]
+        L.afmm_server_result.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp]
+        L.afmm_server_hbm_in_use.argtypes = [C.c_void_p]
+        L.afmm_server_hbm_in_use.restype = C.c_int64
+        L.afmm_server_destroy.argtypes = [C.c_void_p]
+        L.afmm_direct.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64, C.POINTER(C.c_int64), _dp, _dp]
         L.afmm_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.afmm_tree_export.argtypes = [C.c_void_p, C.POINTER(Tree)]
         _lib = L
@@ -117,7 +163,8 @@ class AdaptiveFMM3D:
                             self.lib.afmm_strerror(rc).decode())
 
     def name(self):
-        return "AdaptiveFMM3D(MI355X HIP, FP64)"
+        prec = "mixed" if self.params.precision == PRECISION_MIXED else "FP64"
+        return f"AdaptiveFMM3D(MI355X HIP, {prec})"
 
     def computeAccelerations(self, pos, mass):
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
@@ -128,10 +175,68 @@ class AdaptiveFMM3D:
         self._check(self.lib.afmm_eval(self.ctx, len(mass), _p(pos), _p(mass), _p(acc)))
         return acc
 
-    def compute_device(self, pos_ptr, mass_ptr, acc_ptr, n):
+    def computeAccelerationsAndPotential(self, pos, mass):
+        """(acc (n,3), phi (n,)): phi_i = -G sum_{j != i} m_j / sqrt(|d|^2 + eps^2)
+        through the same expansions; 0.5 * mass @ phi is the potential energy."""
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, np.float64)
+        if len(mass) != len(pos):
+            raise ValueError("pos and mass lengths differ")
+        acc = np.zeros_like(pos)
+        phi = np.zeros(len(mass))
+        self._check(self.lib.afmm_eval_potential(self.ctx, len(mass), _p(pos), _p(mass), _p(acc), _p(phi)))
+        return acc, phi
+
+    def compute_device(self, pos_ptr, mass_ptr, acc_ptr, n, phi_ptr=None):
         """Device-resident evaluation on caller-owned device pointers."""
-        self._check(self.lib.afmm_eval_device(self.ctx, n, C.c_void_p(pos_ptr), C.c_void_p(mass_ptr),
-                                               C.c_void_p(acc_ptr)))
+        if phi_ptr is None:
+            self._check(self.lib.afmm_eval_device(self.ctx, n, C.c_void_p(pos_ptr), C.c_void_p(mass_ptr),
+                                                   C.c_void_p(acc_ptr)))
+        else:
+            self._check(self.lib.afmm_eval_device_potential(self.ctx, n, C.c_void_p(pos_ptr),
+                                                             C.c_void_p(mass_ptr), C.c_void_p(acc_ptr),
+                                                             C.c_void_p(phi_ptr)))
+
+    def computeBatch(self, pos, mass, sys_off, G=None, eps=None, potential=False):
+        """Batched evaluation of independent systems (one launch per phase):
+        system q = rows sys_off[q]:sys_off[q+1] of pos/mass; G, eps per system
+        (None: the solver's). Returns acc (and phi when potential=True)."""
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, np.float64)
+        off = np.ascontiguousarray(sys_off, np.int64)
+        nsys = len(off) - 1
+        g = None if G is None else np.ascontiguousarray(np.broadcast_to(G, (nsys,)), np.float64)
+        e = None if eps is None else np.ascontiguousarray(np.broadcast_to(eps, (nsys,)), np.float64)
+        acc = np.zeros_like(pos)
+        phi = np.zeros(len(mass)) if potential else None
+        self._check(self.lib.afmm_eval_batch(self.ctx, nsys, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             None if g is None else _p(g), None if e is None else _p(e),
+                                             _p(pos), _p(mass), _p(acc), None if phi is None else _p(phi)))
+        return (acc, phi) if potential else acc
+
+    def compute_batch_device(self, sys_off, pos_ptr, mass_ptr, acc_ptr, G=None, eps=None, phi_ptr=None):
+        """Batched evaluation on device pointers (sys_off, G, eps on the host)."""
+        off = np.ascontiguousarray(sys_off, np.int64)
+        nsys = len(off) - 1
+        g = None if G is None else np.ascontiguousarray(np.broadcast_to(G, (nsys,)), np.float64)
+        e = None if eps is None else np.ascontiguousarray(np.broadcast_to(eps, (nsys,)), np.float64)
+        self._check(self.lib.afmm_eval_batch_device(
+            self.ctx, nsys, off.ctypes.data_as(C.POINTER(C.c_int64)), None if g is None else _p(g),
+            None if e is None else _p(e), C.c_void_p(pos_ptr), C.c_void_p(mass_ptr), C.c_void_p(acc_ptr),
+            C.c_void_p(phi_ptr) if phi_ptr else None))
+
+    def direct(self, pos, mass, targets, potential=False):
+        """FP64 direct sum (DirectSolver3D semantics, Kernel3D.hpp:31-42) of all
+        bodies onto the bodies `targets`: (ntgt,3) accelerations, plus the
+        softened potentials when potential=True."""
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, np.float64)
+        tgt = np.ascontiguousarray(targets, np.int64)
+        acc = np.zeros((len(tgt), 3))
+        phi = np.zeros(len(tgt))
+        self._check(self.lib.afmm_direct(self.ctx, len(mass), _p(pos), _p(mass), len(tgt),
+                                         tgt.ctypes.data_as(C.POINTER(C.c_int64)), _p(acc), _p(phi)))
+        return (acc, phi) if potential else acc
 
     def synchronize(self):
         self._check(self.lib.afmm_synchronize(self.ctx))
@@ -163,3 +268,130 @@ class AdaptiveFMM3D:
             a[k + "_idx"] = a[k + "_idx"][:cnt]
         a["tol"] = t.tol
         return a
+
+
+class LeapfrogState:
+    """Device-resident velocity Verlet of one or more systems
+    (VelocityVerlet3D, Integrator3D.hpp:19-40; afmm_state_* in afmm.h).
+
+        st = LeapfrogState(solver, pos, vel, mass)          # initialize
+        st.step(dt, nsteps); pos, vel, acc = st.download()
+        diag = st.diagnostics()   # per system: kinetic, potential, total, momentum, angular
+    """
+
+    def __init__(self, solver, pos, vel, mass, sys_off=None, G=None, eps=None):
+        self.solver = solver
+        lib = solver.lib
+        self.lib = lib
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, np.float64)
+        off = np.ascontiguousarray([0, len(mass)] if sys_off is None else sys_off, np.int64)
+        self.nsys = len(off) - 1
+        self.n = len(mass)
+        g = None if G is None else np.ascontiguousarray(np.broadcast_to(G, (self.nsys,)), np.float64)
+        e = None if eps is None else np.ascontiguousarray(np.broadcast_to(eps, (self.nsys,)), np.float64)
+        h = C.c_void_p()
+        solver._check(lib.afmm_state_create(solver.ctx, self.nsys, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            None if g is None else _p(g), None if e is None else _p(e),
+                                            _p(pos), _p(vel), _p(mass), C.byref(h)))
+        self.h = h
+
+    def step(self, dt, nsteps=1):
+        self.solver._check(self.lib.afmm_state_step(self.h, dt, nsteps))
+
+    def download(self):
+        pos = np.empty((self.n, 3))
+        vel = np.empty((self.n, 3))
+        acc = np.empty((self.n, 3))
+        self.solver._check(self.lib.afmm_state_download(self.h, _p(pos), _p(vel), _p(acc)))
+        return pos, vel, acc
+
+    def diagnostics(self):
+        out = np.empty(9 * self.nsys)
+        self.solver._check(self.lib.afmm_state_diagnostics(self.h, _p(out)))
+        out = out.reshape(self.nsys, 9)
+        return [{"kinetic": r[0], "potential": r[1], "total": r[2], "momentum": r[3:6], "angular": r[6:9]}
+                for r in out]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.afmm_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server with continuous batching (afmm_server_* in afmm.h).
+
+        sv = Server(hbm_budget=64 << 30)
+        jid = sv.submit(pos, vel, mass, G=1, eps=0.01, dt=0.01, p=8, steps=10)
+        sv.wait(jid); pos, vel, acc = sv.result(jid)
+    """
+
+    def __init__(self, hbm_budget=0, leaf_cap=64, precision=PRECISION_FP64, device=0, max_batch=0):
+        self.lib = library()
+        self.prm = ServerParams(hbm_budget, leaf_cap, precision, device, max_batch)
+        h = C.c_void_p()
+        rc = self.lib.afmm_server_create(C.byref(self.prm), C.byref(h))
+        if rc:
+            raise AfmmError(rc, self.lib.afmm_strerror(rc).decode())
+        self.h = h
+        self.sizes = {}
+
+    def _check(self, rc):
+        if rc:
+            raise AfmmError(rc, self.lib.afmm_strerror(rc).decode())
+
+    def warm(self, p):
+        self._check(self.lib.afmm_server_warm(self.h, p))
+
+    def submit(self, pos, vel, mass, G=1.0, eps=0.01, dt=0.01, p=8, steps=0):
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, np.float64)
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+        d = JobDesc(len(mass), _p(pos), None if vel is None else _p(vel), _p(mass), G, eps, dt, p, steps)
+        jid = C.c_int64()
+        self._check(self.lib.afmm_server_submit(self.h, C.byref(d), C.byref(jid)))
+        self.sizes[jid.value] = len(mass)
+        return jid.value
+
+    def wait(self, jid):
+        self._check(self.lib.afmm_server_wait(self.h, jid))
+
+    def drain(self):
+        self._check(self.lib.afmm_server_drain(self.h))
+
+    def info(self, jid):
+        i = JobInfo()
+        self._check(self.lib.afmm_server_info(self.h, jid, C.byref(i)))
+        return i.as_dict()
+
+    def result(self, jid):
+        n = self.sizes.pop(jid)
+        pos, vel, acc = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3))
+        self._check(self.lib.afmm_server_result(self.h, jid, _p(pos), _p(vel), _p(acc)))
+        return pos, vel, acc
+
+    def hbm_in_use(self):
+        return self.lib.afmm_server_hbm_in_use(self.h)
+
+    @staticmethod
+    def job_bytes(n, p):
+        return library().afmm_job_bytes(n, p)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.afmm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

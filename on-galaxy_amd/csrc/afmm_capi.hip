@@ -17,55 +17,21 @@
 #include <vector>
 
 #include "../../include/afmm.h"
+#include "afmm_ctx.h"
 #include "afmm_internal.h"
 #include "afmm_math.h"
 #include "afmm_tables.h"
 
 using namespace afmm;
 
-struct afmm_ctx {
-    afmm_params prm{};
-    hipStream_t stream = nullptr;
-    Pool pool;
-    std::string err;
-    // tables
-    double *d_m2m_sw = nullptr, *d_l2l_sw = nullptr, *d_m2l_sw = nullptr;
-    // last evaluation
-    bool have_tree = false;
-    TreeView tv{};
-    TreeMeta meta;
-    RootInfo root{};
-    afmm_stats stats{};
-    hipEvent_t ev[8] = {};
-};
-
 namespace {
-
-int fail(afmm_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
-    return code;
-}
-
-template <class F>
-int guarded(afmm_ctx* c, F&& f) {
-    try {
-        return f();
-    } catch (const HipError& e) {
-        const bool oom = e.code == hipErrorOutOfMemory || e.code == hipErrorMemoryAllocation;
-        return fail(c, oom ? AFMM_ERR_NO_MEMORY : AFMM_ERR_DEVICE, e.what());
-    } catch (const std::bad_alloc&) {
-        return fail(c, AFMM_ERR_NO_MEMORY, "host allocation failed");
-    } catch (const std::exception& e) {
-        return fail(c, AFMM_ERR_DEVICE, e.what());
-    }
-}
 
 int check_params(const afmm_params* p) {
     if (!p) return AFMM_ERR_ARG;
     if (!(std::isfinite(p->G)) || !(std::isfinite(p->eps)) || p->eps < 0) return AFMM_ERR_ARG;
     if (p->p < 1 || p->p > kMaxOrder) return AFMM_ERR_ORDER;
     if (p->leaf_cap < 1) return AFMM_ERR_LEAF_CAP;
-    if (p->precision != AFMM_PRECISION_FP64) return AFMM_ERR_PRECISION;
+    if (p->precision != AFMM_PRECISION_FP64 && p->precision != AFMM_PRECISION_MIXED) return AFMM_ERR_PRECISION;
     return AFMM_OK;
 }
 
@@ -75,24 +41,39 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// The evaluation on device arrays (pos AoS n x 3, mass n) into acc (AoS, input order).
-int eval_device(afmm_ctx* c, int64_t n, const double* pos, const double* mass, double* acc) {
+}  // namespace
+
+// The evaluation of nsys systems (concatenated device arrays: pos AoS n x 3,
+// mass n; system q = bodies [sys_off[q], sys_off[q+1])) into acc (AoS, input
+// order) and optionally phi. G and eps per system (host arrays).
+int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G, const double* eps, const double* pos,
+                const double* mass, double* acc, double* phi) {
     c->have_tree = false;
+    const int64_t n = sys_off[nsys];
     if (n == 0) return AFMM_OK;
     hipStream_t s = c->stream;
+    // per-system constants (uploaded before the build's first synchronisation)
+    std::vector<double> h_ge(2 * (size_t)nsys);
+    for (int q = 0; q < nsys; ++q) h_ge[q] = G[q], h_ge[nsys + q] = eps[q] * eps[q];
+    double* d_ge = c->pool.get<double>("sys_G_eps2", 2 * (size_t)nsys);
     AFMM_CHECK(hipEventRecord(c->ev[0], s));
+    AFMM_CHECK(hipMemcpyAsync(d_ge, h_ge.data(), sizeof(double) * 2 * nsys, hipMemcpyHostToDevice, s));
     TreeView tv{};
     TreeMeta meta;
-    RootInfo root{};
-    build_tree(c->pool, s, pos, mass, (int)n, c->prm.leaf_cap, c->prm.eps, tv, meta, root);
-    if (root.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
+    std::vector<RootInfo> roots;
+    build_forest(c->pool, s, pos, mass, nsys, sys_off, eps, c->prm.leaf_cap, tv, meta, roots);
+    for (const RootInfo& r : roots)
+        if (r.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
+    tv.sys_G = d_ge;
+    tv.sys_eps2 = d_ge + nsys;
     AFMM_CHECK(hipEventRecord(c->ev[1], s));
-    FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw};
-    run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.G, c->prm.eps, acc, c->ev + 2);
+    FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw, c->d_m2l_hl};
+    run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision == AFMM_PRECISION_MIXED, acc, phi, c->ev + 2);
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
     c->meta = meta;
-    c->root = root;
+    c->root = roots[0];
+    c->nsys = nsys;
     c->have_tree = true;
     afmm_stats& st = c->stats;
     st = afmm_stats{};
@@ -103,25 +84,12 @@ int eval_device(afmm_ctx* c, int64_t n, const double* pos, const double* mass, d
     st.n_w = tv.n_w;
     st.n_x = tv.n_x;
     st.max_level = tv.max_level;
-    // small host reductions for the stats
-    std::vector<int> nc(tv.nb), nv(tv.nb), cnt(tv.nleaves), lid(tv.nleaves), scount(tv.nb), uoff(tv.nleaves + 1),
-        uidx(tv.n_u);
-    AFMM_CHECK(hipMemcpy(nc.data(), tv.ncoll, sizeof(int) * tv.nb, hipMemcpyDeviceToHost));
-    AFMM_CHECK(hipMemcpy(nv.data(), tv.nv, sizeof(int) * tv.nb, hipMemcpyDeviceToHost));
-    AFMM_CHECK(hipMemcpy(lid.data(), tv.leaf_ids, sizeof(int) * tv.nleaves, hipMemcpyDeviceToHost));
-    AFMM_CHECK(hipMemcpy(scount.data(), tv.seg_count, sizeof(int) * tv.nb, hipMemcpyDeviceToHost));
-    AFMM_CHECK(hipMemcpy(uoff.data(), tv.u_off, sizeof(int) * (tv.nleaves + 1), hipMemcpyDeviceToHost));
-    if (tv.n_u) AFMM_CHECK(hipMemcpy(uidx.data(), tv.u_idx, sizeof(int) * tv.n_u, hipMemcpyDeviceToHost));
-    for (int b = 0; b < tv.nb; ++b) st.n_colleague += nc[b], st.n_v += nv[b];
-    int64_t pairs = 0;
-    for (int li = 0; li < tv.nleaves; ++li) {
-        const int nt = scount[lid[li]];
-        st.max_leaf_occ = std::max(st.max_leaf_occ, nt);
-        int64_t srcs = 0;
-        for (int e = uoff[li]; e < uoff[li + 1]; ++e) srcs += scount[uidx[e]];
-        pairs += (int64_t)nt * srcs;
-    }
-    st.p2p_pairs = pairs - n;
+    unsigned long long tot[4];
+    AFMM_CHECK(hipMemcpy(tot, tv.list_totals, sizeof(tot), hipMemcpyDeviceToHost));
+    st.n_colleague = (int64_t)tot[0];
+    st.n_v = (int64_t)tot[1];
+    st.max_leaf_occ = (int32_t)tot[2];
+    st.p2p_pairs = (int64_t)tot[3] - n;
     st.ms_tree = elapsed(c->ev[0], c->ev[1]);
     st.ms_upward = elapsed(c->ev[2], c->ev[3]);
     st.ms_lists = elapsed(c->ev[3], c->ev[7]);  // M2L ordering / source table
@@ -129,6 +97,29 @@ int eval_device(afmm_ctx* c, int64_t n, const double* pos, const double* mass, d
     st.ms_downward = elapsed(c->ev[4], c->ev[5]);
     st.ms_leaf = elapsed(c->ev[5], c->ev[6]);
     st.ms_total = elapsed(c->ev[0], c->ev[6]);
+    return AFMM_OK;
+}
+
+namespace {
+
+// Validates a batch description; fills 32-bit offsets and per-system G, eps.
+int batch_args(afmm_ctx* c, int32_t nsys, const int64_t* sys_off, const double* G, const double* eps,
+               std::vector<int>& off, std::vector<double>& g, std::vector<double>& e) {
+    if (nsys < 1 || !sys_off || sys_off[0] != 0) return fail(c, AFMM_ERR_ARG, "bad system offsets");
+    if (sys_off[nsys] >= (1LL << 30)) return fail(c, AFMM_ERR_TOO_LARGE, "n beyond 2^30");
+    off.resize(nsys + 1);
+    g.resize(nsys);
+    e.resize(nsys);
+    for (int q = 0; q <= nsys; ++q) {
+        if (q > 0 && sys_off[q] <= sys_off[q - 1]) return fail(c, AFMM_ERR_ARG, "every system needs a body");
+        off[q] = (int)sys_off[q];
+    }
+    for (int q = 0; q < nsys; ++q) {
+        g[q] = G ? G[q] : c->prm.G;
+        e[q] = eps ? eps[q] : c->prm.eps;
+        if (!std::isfinite(g[q]) || !std::isfinite(e[q]) || e[q] < 0)
+            return fail(c, AFMM_ERR_ARG, "bad per-system G or eps");
+    }
     return AFMM_OK;
 }
 
@@ -176,6 +167,15 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
         AFMM_CHECK(hipMemcpy(c->d_m2m_sw, m2m.data(), sizeof(double) * m2m.size(), hipMemcpyHostToDevice));
         AFMM_CHECK(hipMemcpy(c->d_l2l_sw, l2l.data(), sizeof(double) * l2l.size(), hipMemcpyHostToDevice));
         AFMM_CHECK(hipMemcpy(c->d_m2l_sw, m2l.data(), sizeof(double) * m2l.size(), hipMemcpyHostToDevice));
+        if (prm->precision == AFMM_PRECISION_MIXED) {
+            std::vector<float2> hl(m2l.size());
+            for (size_t i = 0; i < m2l.size(); ++i) {
+                const float hi = (float)m2l[i];
+                hl[i] = float2{hi, (float)(m2l[i] - (double)hi)};
+            }
+            c->d_m2l_hl = c->pool.get<float2>("T_m2l_hl", hl.size());
+            AFMM_CHECK(hipMemcpy(c->d_m2l_hl, hl.data(), sizeof(float2) * hl.size(), hipMemcpyHostToDevice));
+        }
         return AFMM_OK;
     });
     if (r) {
@@ -204,16 +204,22 @@ int afmm_synchronize(afmm_ctx* c) {
     });
 }
 
-int afmm_eval_device(afmm_ctx* c, int64_t n, const double* pos, const double* mass, double* acc) {
+int afmm_eval_device_potential(afmm_ctx* c, int64_t n, const double* pos, const double* mass, double* acc,
+                               double* phi) {
     if (!c || n < 0 || (n > 0 && (!pos || !mass || !acc))) return AFMM_ERR_ARG;
     if (n >= (1LL << 30)) return fail(c, AFMM_ERR_TOO_LARGE, "n beyond 2^30");
     return guarded(c, [&]() -> int {
         AFMM_CHECK(hipSetDevice(c->prm.device));
-        return eval_device(c, n, pos, mass, acc);
+        const int off[2] = {0, (int)n};
+        return eval_forest(c, 1, off, &c->prm.G, &c->prm.eps, pos, mass, acc, phi);
     });
 }
 
-int afmm_eval(afmm_ctx* c, int64_t n, const double* pos, const double* mass, double* acc) {
+int afmm_eval_device(afmm_ctx* c, int64_t n, const double* pos, const double* mass, double* acc) {
+    return afmm_eval_device_potential(c, n, pos, mass, acc, nullptr);
+}
+
+int afmm_eval_potential(afmm_ctx* c, int64_t n, const double* pos, const double* mass, double* acc, double* phi) {
     if (!c || n < 0 || (n > 0 && (!pos || !mass || !acc))) return AFMM_ERR_ARG;
     if (n >= (1LL << 30)) return fail(c, AFMM_ERR_TOO_LARGE, "n beyond 2^30");
     if (n == 0) {
@@ -226,11 +232,85 @@ int afmm_eval(afmm_ctx* c, int64_t n, const double* pos, const double* mass, dou
         double* dpos = c->pool.get<double>("in_pos", 3 * (size_t)n);
         double* dmass = c->pool.get<double>("in_mass", n);
         double* dacc = c->pool.get<double>("out_acc", 3 * (size_t)n);
+        double* dphi = phi ? c->pool.get<double>("out_phi", n) : nullptr;
         AFMM_CHECK(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
         AFMM_CHECK(hipMemcpyAsync(dmass, mass, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        const int rc = eval_device(c, n, dpos, dmass, dacc);
+        const int off[2] = {0, (int)n};
+        const int rc = eval_forest(c, 1, off, &c->prm.G, &c->prm.eps, dpos, dmass, dacc, dphi);
         if (rc) return rc;
         AFMM_CHECK(hipMemcpyAsync(acc, dacc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        if (phi) AFMM_CHECK(hipMemcpyAsync(phi, dphi, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
+        return AFMM_OK;
+    });
+}
+
+int afmm_eval(afmm_ctx* c, int64_t n, const double* pos, const double* mass, double* acc) {
+    return afmm_eval_potential(c, n, pos, mass, acc, nullptr);
+}
+
+int afmm_eval_batch_device(afmm_ctx* c, int32_t nsys, const int64_t* sys_off, const double* G, const double* eps,
+                           const double* pos, const double* mass, double* acc, double* phi) {
+    if (!c || !pos || !mass || !acc) return AFMM_ERR_ARG;
+    std::vector<int> off;
+    std::vector<double> g, e;
+    if (int rc = batch_args(c, nsys, sys_off, G, eps, off, g, e)) return rc;
+    return guarded(c, [&]() -> int {
+        AFMM_CHECK(hipSetDevice(c->prm.device));
+        return eval_forest(c, nsys, off.data(), g.data(), e.data(), pos, mass, acc, phi);
+    });
+}
+
+int afmm_eval_batch(afmm_ctx* c, int32_t nsys, const int64_t* sys_off, const double* G, const double* eps,
+                    const double* pos, const double* mass, double* acc, double* phi) {
+    if (!c || !pos || !mass || !acc) return AFMM_ERR_ARG;
+    std::vector<int> off;
+    std::vector<double> g, e;
+    if (int rc = batch_args(c, nsys, sys_off, G, eps, off, g, e)) return rc;
+    const int64_t n = off[nsys];
+    return guarded(c, [&]() -> int {
+        AFMM_CHECK(hipSetDevice(c->prm.device));
+        hipStream_t s = c->stream;
+        double* dpos = c->pool.get<double>("in_pos", 3 * (size_t)n);
+        double* dmass = c->pool.get<double>("in_mass", n);
+        double* dacc = c->pool.get<double>("out_acc", 3 * (size_t)n);
+        double* dphi = phi ? c->pool.get<double>("out_phi", n) : nullptr;
+        AFMM_CHECK(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+        AFMM_CHECK(hipMemcpyAsync(dmass, mass, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        const int rc = eval_forest(c, nsys, off.data(), g.data(), e.data(), dpos, dmass, dacc, dphi);
+        if (rc) return rc;
+        AFMM_CHECK(hipMemcpyAsync(acc, dacc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        if (phi) AFMM_CHECK(hipMemcpyAsync(phi, dphi, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
+        return AFMM_OK;
+    });
+}
+
+int afmm_direct(afmm_ctx* c, int64_t n, const double* pos, const double* mass, int64_t ntgt, const int64_t* tgt,
+                double* acc, double* phi) {
+    if (!c || n < 0 || ntgt < 0) return AFMM_ERR_ARG;
+    if (ntgt == 0) return AFMM_OK;
+    if (!pos || !mass || !tgt || !acc) return AFMM_ERR_ARG;
+    if (n >= (1LL << 30) || ntgt >= (1LL << 30)) return fail(c, AFMM_ERR_TOO_LARGE, "n beyond 2^30");
+    std::vector<int> t32(ntgt);
+    for (int64_t k = 0; k < ntgt; ++k) {
+        if (tgt[k] < 0 || tgt[k] >= n) return fail(c, AFMM_ERR_ARG, "target index outside [0, n)");
+        t32[k] = (int)tgt[k];
+    }
+    return guarded(c, [&]() -> int {
+        AFMM_CHECK(hipSetDevice(c->prm.device));
+        hipStream_t s = c->stream;
+        double* dpos = c->pool.get<double>("dir_pos", 3 * (size_t)n);
+        double* dmass = c->pool.get<double>("dir_mass", n);
+        int* dtgt = c->pool.get<int>("dir_tgt", ntgt);
+        double* dacc = c->pool.get<double>("dir_acc", 3 * (size_t)ntgt);
+        double* dphi = c->pool.get<double>("dir_phi", ntgt);
+        AFMM_CHECK(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+        AFMM_CHECK(hipMemcpyAsync(dmass, mass, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        AFMM_CHECK(hipMemcpyAsync(dtgt, t32.data(), sizeof(int) * ntgt, hipMemcpyHostToDevice, s));
+        direct_targets(c->pool, s, dpos, dmass, (int)n, dtgt, (int)ntgt, c->prm.G, c->prm.eps, dacc, dphi);
+        AFMM_CHECK(hipMemcpyAsync(acc, dacc, sizeof(double) * 3 * ntgt, hipMemcpyDeviceToHost, s));
+        if (phi) AFMM_CHECK(hipMemcpyAsync(phi, dphi, sizeof(double) * ntgt, hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipStreamSynchronize(s));
         return AFMM_OK;
     });
@@ -247,6 +327,7 @@ int afmm_get_stats(const afmm_ctx* c, afmm_stats* out) {
 int afmm_tree_export(afmm_ctx* c, afmm_tree* out) {
     if (!c || !out) return AFMM_ERR_ARG;
     if (!c->have_tree) return fail(c, AFMM_ERR_STATE, "no tree: run afmm_eval first");
+    if (c->nsys != 1) return fail(c, AFMM_ERR_STATE, "tree export covers single-system evaluations");
     return guarded(c, [&]() -> int {
         const TreeView& t = c->tv;
         const int nb = t.nb, n = t.n, nl = t.nleaves;
@@ -347,6 +428,147 @@ int afmm_tree_export(afmm_ctx* c, afmm_tree* out) {
         out->tol = t.tol;
         return AFMM_OK;
     });
+}
+
+// ---- resident leapfrog --------------------------------------------------------
+struct afmm_state {
+    afmm_ctx* ctx = nullptr;
+    int nsys = 0;
+    std::vector<int> off;
+    std::vector<double> G, eps;
+    Pool mem;  // the state's own arrays (released with the state)
+    double *pos = nullptr, *vel = nullptr, *mass = nullptr, *acc = nullptr, *phi = nullptr;
+    std::vector<JobRef> jobs;
+    JobRef* d_jobs = nullptr;
+    int *d_boff = nullptr, *d_n = nullptr;
+    int64_t steps = 0;
+};
+
+namespace {
+// One batched evaluation of the state's systems, with drift/kick when `step`.
+int state_eval(afmm_state* st, bool step, double dt, bool with_phi) {
+    afmm_ctx* c = st->ctx;
+    hipStream_t s = c->stream;
+    const int n = st->off[st->nsys];
+    for (JobRef& j : st->jobs) {
+        j.dt = dt;
+        j.drift = step ? 1 : 0;
+        j.phi = with_phi ? j.phi : nullptr;
+    }
+    if (with_phi)
+        for (int q = 0; q < st->nsys; ++q) st->jobs[q].phi = st->phi + st->off[q];
+    AFMM_CHECK(hipMemcpyAsync(st->d_jobs, st->jobs.data(), sizeof(JobRef) * st->nsys, hipMemcpyHostToDevice, s));
+    double* bpos = c->pool.get<double>("lf_pos", 3 * (size_t)n);
+    double* bmass = c->pool.get<double>("lf_mass", n);
+    double* bacc = c->pool.get<double>("lf_acc", 3 * (size_t)n);
+    double* bphi = with_phi ? c->pool.get<double>("lf_phi", n) : nullptr;
+    leapfrog_gather(s, n, st->nsys, st->d_boff, st->d_jobs, bpos, bmass);
+    const int rc = eval_forest(c, st->nsys, st->off.data(), st->G.data(), st->eps.data(), bpos, bmass, bacc, bphi);
+    if (rc) return rc;
+    leapfrog_kick(s, n, st->nsys, st->d_boff, st->d_jobs, bacc, bphi);
+    for (JobRef& j : st->jobs) j.init = 1;
+    AFMM_CHECK(hipMemcpyAsync(st->d_jobs, st->jobs.data(), sizeof(JobRef) * st->nsys, hipMemcpyHostToDevice, s));
+    AFMM_CHECK(hipStreamSynchronize(s));
+    return AFMM_OK;
+}
+}  // namespace
+
+int afmm_state_create(afmm_ctx* c, int32_t nsys, const int64_t* sys_off, const double* G, const double* eps,
+                      const double* pos, const double* vel, const double* mass, afmm_state** out) {
+    if (!c || !out || !pos || !vel || !mass) return AFMM_ERR_ARG;
+    *out = nullptr;
+    std::vector<int> off;
+    std::vector<double> g, e;
+    if (int rc = batch_args(c, nsys, sys_off, G, eps, off, g, e)) return rc;
+    afmm_state* st = new (std::nothrow) afmm_state();
+    if (!st) return AFMM_ERR_NO_MEMORY;
+    st->ctx = c;
+    st->nsys = nsys;
+    st->off = off;
+    st->G = g;
+    st->eps = e;
+    const int n = off[nsys];
+    const int rc = guarded(c, [&]() -> int {
+        AFMM_CHECK(hipSetDevice(c->prm.device));
+        hipStream_t s = c->stream;
+        st->pos = st->mem.get<double>("pos", 3 * (size_t)n);
+        st->vel = st->mem.get<double>("vel", 3 * (size_t)n);
+        st->acc = st->mem.get<double>("acc", 3 * (size_t)n);
+        st->mass = st->mem.get<double>("mass", n);
+        st->phi = st->mem.get<double>("phi", n);
+        st->d_jobs = st->mem.get<JobRef>("jobs", nsys);
+        st->d_boff = st->mem.get<int>("boff", nsys + 1);
+        st->d_n = st->mem.get<int>("n", nsys);
+        AFMM_CHECK(hipMemcpyAsync(st->pos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+        AFMM_CHECK(hipMemcpyAsync(st->vel, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+        AFMM_CHECK(hipMemcpyAsync(st->mass, mass, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        std::vector<int> cnt(nsys);
+        for (int q = 0; q < nsys; ++q) cnt[q] = off[q + 1] - off[q];
+        AFMM_CHECK(hipMemcpyAsync(st->d_boff, off.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice, s));
+        AFMM_CHECK(hipMemcpyAsync(st->d_n, cnt.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
+        st->jobs.resize(nsys);
+        for (int q = 0; q < nsys; ++q) {
+            const size_t o = (size_t)off[q];
+            st->jobs[q] = JobRef{st->pos + 3 * o, st->vel + 3 * o, st->acc + 3 * o, nullptr, st->mass + o, 0.0, 0, 0};
+        }
+        return state_eval(st, false, 0.0, false);  // VelocityVerlet3D::initialize
+    });
+    if (rc) {
+        delete st;
+        return rc;
+    }
+    *out = st;
+    return AFMM_OK;
+}
+
+int afmm_state_step(afmm_state* st, double dt, int32_t nsteps) {
+    if (!st || nsteps < 0 || !std::isfinite(dt)) return AFMM_ERR_ARG;
+    afmm_ctx* c = st->ctx;
+    return guarded(c, [&]() -> int {
+        AFMM_CHECK(hipSetDevice(c->prm.device));
+        for (int k = 0; k < nsteps; ++k) {
+            if (int rc = state_eval(st, true, dt, false)) return rc;
+            ++st->steps;
+        }
+        return AFMM_OK;
+    });
+}
+
+int afmm_state_download(afmm_state* st, double* pos, double* vel, double* acc) {
+    if (!st) return AFMM_ERR_ARG;
+    afmm_ctx* c = st->ctx;
+    const size_t n = (size_t)st->off[st->nsys];
+    return guarded(c, [&]() -> int {
+        AFMM_CHECK(hipSetDevice(c->prm.device));
+        hipStream_t s = c->stream;
+        if (pos) AFMM_CHECK(hipMemcpyAsync(pos, st->pos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        if (vel) AFMM_CHECK(hipMemcpyAsync(vel, st->vel, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        if (acc) AFMM_CHECK(hipMemcpyAsync(acc, st->acc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
+        return AFMM_OK;
+    });
+}
+
+int afmm_state_diagnostics(afmm_state* st, double* out) {
+    if (!st || !out) return AFMM_ERR_ARG;
+    afmm_ctx* c = st->ctx;
+    return guarded(c, [&]() -> int {
+        AFMM_CHECK(hipSetDevice(c->prm.device));
+        if (int rc = state_eval(st, false, 0.0, true)) return rc;
+        hipStream_t s = c->stream;
+        double* d_out = c->pool.get<double>("diag_out", 9 * (size_t)st->nsys);
+        diagnostics(s, st->nsys, st->d_jobs, st->d_n, d_out);
+        AFMM_CHECK(hipMemcpyAsync(out, d_out, sizeof(double) * 9 * st->nsys, hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
+        return AFMM_OK;
+    });
+}
+
+void afmm_state_destroy(afmm_state* st) {
+    if (!st) return;
+    if (st->ctx && st->ctx->stream) (void)hipStreamSynchronize(st->ctx->stream);
+    delete st;
 }
 
 }  // extern "C"

@@ -1,0 +1,59 @@
+// ============================================================================
+// afmm_ctx.h -- the solver context behind the opaque afmm_ctx of afmm.h and
+// the evaluation entry shared by the C ABI, the resident integrator and the
+// request server.
+// ============================================================================
+#pragma once
+
+#include <string>
+
+#include "../../include/afmm.h"
+#include "afmm_internal.h"
+
+struct afmm_ctx {
+    afmm_params prm{};
+    hipStream_t stream = nullptr;
+    afmm::Pool pool;
+    std::string err;
+    // tables
+    double *d_m2m_sw = nullptr, *d_l2l_sw = nullptr, *d_m2l_sw = nullptr;
+    float2* d_m2l_hl = nullptr;  // split FP32 M2L operators (mixed precision only)
+    // last evaluation
+    bool have_tree = false;
+    afmm::TreeView tv{};
+    afmm::TreeMeta meta;
+    afmm::RootInfo root{};  // of system 0
+    int nsys = 1;
+    afmm_stats stats{};
+    hipEvent_t ev[8] = {};
+};
+
+namespace afmm {
+
+inline int fail(afmm_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F>
+inline int guarded(afmm_ctx* c, F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        const bool oom = e.code == hipErrorOutOfMemory || e.code == hipErrorMemoryAllocation;
+        return fail(c, oom ? AFMM_ERR_NO_MEMORY : AFMM_ERR_DEVICE, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(c, AFMM_ERR_NO_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(c, AFMM_ERR_DEVICE, e.what());
+    }
+}
+
+// Forest evaluation of nsys systems on device arrays (concatenated pos AoS
+// n x 3 and mass; system q = bodies [sys_off[q], sys_off[q+1]), host offsets)
+// into acc (and phi when non-null) with per-system G and eps (host arrays).
+// Synchronises the context's stream; records the statistics of the last tree.
+int eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G, const double* eps, const double* pos,
+                const double* mass, double* acc, double* phi);
+
+}  // namespace afmm

@@ -97,7 +97,12 @@ struct RootInfo {
 // Device pointers of the built tree (breadth-first box ids).
 struct TreeView {
     int n = 0, nb = 0, nleaves = 0, max_level = 0;
-    double tol = 0;
+    int nsys = 1;                // systems (roots = boxes 0..nsys-1)
+    double tol = 0;              // tolerance of system 0
+    const RootInfo* roots;       // [nsys]
+    int* bsys;                   // [B] system of each box
+    const double* sys_G;         // [nsys] gravitational constant per system
+    const double* sys_eps2;      // [nsys] softening^2 per system
     double *x, *y, *z, *m;
     int* orig;
     int* leaf_of;        // [n] BFS leaf id of each sorted body
@@ -119,6 +124,9 @@ struct TreeView {
     int *w_off, *w_idx;  // per leaf
     int *x_off, *x_idx;  // per box
     int64_t n_u = 0, n_w = 0, n_x = 0;
+    // device totals of the lists: {colleague entries, V entries, largest leaf,
+    // sum over leaves of n_leaf * (bodies in its U leaves)}
+    unsigned long long* list_totals;  // [4]
 };
 
 // Host-side shape of the built tree: boxes of level l are the breadth-first
@@ -142,17 +150,24 @@ struct SortBufs {
     int* scan_tmp;
 };
 int sort_blocks(int n);
-// Stable LSD radix sort of (khi, klo, val) on 9-bit digits of the 72-bit key
-// (khi << 9) | klo: digit g is bits [9g, 9g+9). Sorts digits
-// first..first+count-1 in that order; returns the buffer set holding the result.
-int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s);
+// Stable LSD radix sort of (khi, klo, val) on 9-bit digits: digit 0 is klo
+// bits [0, 9), digits 1..7 are khi bits [9(g-1), 9g), digits 8 and 9 are klo
+// bits [9, 18) and [18, 27). Sorts digits first..first+count-1 in that order,
+// starting from buffer set `start`; returns the buffer set holding the result.
+int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, int start = 0);
 
 // ---- tree.hip ----------------------------------------------------------------
 // Builds the adaptive octree and all interaction lists for n bodies given as
 // device AoS positions and masses. Reproduces AdaptiveOctree::build and
 // buildLists (AdaptiveOctree.hpp:65-189) exactly.
-void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int n, int cap,
-                double eps_floor, TreeView& tv, TreeMeta& meta, RootInfo& root);
+// A forest of adaptive octrees, one per system: system q owns the bodies
+// [h_sys_off[q], h_sys_off[q+1]) (at least one each), has its own root cube,
+// tolerance and leaf-edge floor h_eps_floor[q]; box q of level 0 is its root.
+// Each tree, its box numbering (refid) and its lists are those the reference
+// builds for that system alone. roots[q].nonfinite > 0 aborts the build.
+void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
+                  const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
+                  std::vector<RootInfo>& roots);
 
 // ---- fmm.hip -----------------------------------------------------------------
 struct FmmTimes {
@@ -161,10 +176,39 @@ struct FmmTimes {
 struct FmmTables {
     const double *m2m_sw, *l2l_sw;  // device, 8 octant operators in the MFMA A-fragment layout
     const double* m2l_sw;           // device, 343 offset operators in the same layout
+    const float2* m2l_hl;           // device, the same as FP32 (hi, lo) pairs, or null
 };
-// Runs the FMM passes on a built tree and writes G*(far + near) to d_acc in
-// input order (AoS n x 3).
+// Runs the FMM passes on a built tree (forest) and writes G*(far + near) to
+// d_acc in input order (AoS n x 3) and, when d_phi is non-null, the potential
+// -G*(far + near) per body, with G and eps of each body's system (tv.sys_G,
+// tv.sys_eps2). mixed: M2L as split-FP32 MFMA and P2P in FP32 with FP64
+// accumulation; everything else FP64.
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
-             const FmmTables& tab, double G, double eps, double* d_acc, hipEvent_t* ev /* 6 or null */);
+             const FmmTables& tab, bool mixed, double* d_acc, double* d_phi,
+             hipEvent_t* ev /* 6 or null */);
+
+// ---- integrate.hip -----------------------------------------------------------
+// One system of a leapfrog step: its resident arrays and step settings.
+struct JobRef {
+    double *pos, *vel, *acc;  // [3n] AoS, input order
+    double* phi;              // [n] or null
+    const double* mass;       // [n]
+    double dt;
+    int init;   // accelerations hold the forces of the current positions
+    int drift;  // 1: drift before and kick after the evaluation; 0: evaluate only
+};
+// Batch body k is body k - boff[q] of system q (boff[q] <= k < boff[q+1]).
+void leapfrog_gather(hipStream_t s, int n, int njobs, const int* d_boff, const JobRef* d_jobs, double* bpos,
+                     double* bmass);
+void leapfrog_kick(hipStream_t s, int n, int njobs, const int* d_boff, const JobRef* d_jobs, const double* bacc,
+                   const double* bphi);
+// d_out[9q..9q+9) = {KE, PE = 0.5 sum m phi, KE+PE, momentum xyz, angular momentum xyz}
+void diagnostics(hipStream_t s, int njobs, const JobRef* d_jobs, const int* d_n, double* d_out);
+
+// ---- direct.hip --------------------------------------------------------------
+// FP64 direct sum of the n bodies (pos AoS n x 3) onto the bodies d_tgt[0..ntgt):
+// acc (AoS ntgt x 3) and, if d_phi is non-null, the softened potential.
+void direct_targets(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int n, const int* d_tgt,
+                    int ntgt, double G, double eps, double* d_acc, double* d_phi);
 
 }  // namespace afmm

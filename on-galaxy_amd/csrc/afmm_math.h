@@ -201,6 +201,28 @@ AFMM_HD void multipole_grad(Coef M, double dx, double dy, double dz, double& gx,
     gz = zz;
 }
 
+// Potentials of the expansions at the scaled point d: sum_{n<=P} L_n^m R_n^m
+// (FmmExpansions3D.hpp:203-213) and sum_{n<=P} M_n^m S_n^m (:166-177), both
+// real; folded onto m >= 0 as sum_n (L_n^0 R_n^0 + 2 sum_{a>=1} Re L_n^a R_n^a).
+template <int P, class Coef>
+AFMM_HD double local_pot(Coef L, double dx, double dy, double dz) {
+    double s = 0.0;
+    regular_stream<P>(dx, dy, dz, [&](int n, int a, double re, double im) {
+        const double lr = L(rid(n, a)), li = a ? L(rid(n, a) + 1) : 0.0;
+        s += (a ? 2.0 : 1.0) * (re * lr - im * li);
+    });
+    return s;
+}
+template <int P, class Coef>
+AFMM_HD double multipole_pot(Coef M, double dx, double dy, double dz) {
+    double s = 0.0;
+    singular_stream<P>(dx, dy, dz, [&](int n, int a, double re, double im) {
+        const double mr = M(rid(n, a)), mi = a ? M(rid(n, a) + 1) : 0.0;
+        s += (a ? 2.0 : 1.0) * (re * mr - im * mi);
+    });
+    return s;
+}
+
 // ---- exact box geometry -----------------------------------------------------
 // A box at level l with integer coordinate i (0 <= i < 2^l) along an axis has
 // the exact centre c_root + h0 * (2i + 1 - 2^l) / 2^l. It is kept as an

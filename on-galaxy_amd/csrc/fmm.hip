@@ -1,5 +1,5 @@
 // ============================================================================
-// fmm.hip -- FP64 FMM passes on a built tree (AdaptiveFMM3D.hpp:44-108):
+// fmm.hip -- FMM passes on a built tree (AdaptiveFMM3D.hpp:44-108):
 //   P2M at leaves -> M2M bottom-up -> M2L (V) -> P2L (X) -> L2L top-down
 //   -> L2P + M2P (W) per body -> P2P (U) per leaf, acc = G*(far + near).
 // M2L and P2L only read multipoles, so all levels run in one launch each;
@@ -19,49 +19,188 @@ namespace {
 constexpr int kT = 256;
 inline int blocks(long long n, int t = kT) { return (int)std::max<long long>(1, (n + t - 1) / t); }
 
-// Rows of per-body coefficient vectors staged in LDS (kept within 64 KB).
-template <int S>
-constexpr int lds_rows() { return 64 * S * 8 <= 65536 ? 64 : 32; }
-
-// ---- P2M: one wave per leaf; rows of q*conj(R) in LDS, column sums per lane.
-template <int P>
-__global__ void __launch_bounds__(64) k_p2m(TreeView t, double* __restrict__ M) {
-    constexpr int S = nterms(P);
-    constexpr int K = (S + 63) / 64;
-    constexpr int R = lds_rows<S>();
-    __shared__ double rows[R * S];
-    const int b = t.leaf_ids[blockIdx.x];
-    const int lane = threadIdx.x;
-    const int f = t.seg_first[b], cnt = t.seg_count[b];
-    const double ch[6] = {t.cdd[6 * (size_t)b], t.cdd[6 * (size_t)b + 1], t.cdd[6 * (size_t)b + 2],
-                          t.cdd[6 * (size_t)b + 3], t.cdd[6 * (size_t)b + 4], t.cdd[6 * (size_t)b + 5]};
-    const double ih = t.invh[b];
-    double acc[K];
+// ---- wave helpers -------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    for (int base = 0; base < cnt; base += R) {
-        double* row = rows + lane * S;
-        const int i = f + base + lane;
-        if (lane < R && base + lane < cnt) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;  // every lane holds the total (fixed butterfly order)
+}
+
+// A concatenated stream of the bodies of a list of boxes (their sorted-body
+// ranges), read 64 bodies per step: lane l of step k gets body 64k + l.
+struct BodyStream {
+    const int* list;  // box ids
+    int n;            // list length
+    const int *seg_first, *seg_count;
+    int e = 0, off = 0;  // current position: list entry e, offset off
+    __device__ bool done() const { return e >= n; }
+    // body index of stream position (current + lane), or -1 past the end
+    __device__ int body_at(int lane) const {
+        int ee = e, oo = off + lane;
+        while (ee < n) {
+            const int c = list[ee];
+            const int cnt = seg_count[c];
+            if (oo < cnt) return seg_first[c] + oo;
+            oo -= cnt;
+            ++ee;
+        }
+        return -1;
+    }
+    __device__ void advance(int k) {
+        while (e < n) {
+            const int cnt = seg_count[list[e]];
+            if (off + k < cnt) {
+                off += k;
+                return;
+            }
+            k -= cnt - off;
+            off = 0;
+            ++e;
+        }
+    }
+};
+
+// ---- P2M and P2L: per-body coefficient accumulation -------------------------------
+// One wave per target box. Lanes take the source bodies 64 at a time from the
+// packed stream; each lane accumulates q conj(R(d)) (P2M, FmmExpansions3D.hpp:
+// 148-163) or q conj(S(d)) (P2L, AdaptiveFMM3D.hpp:117-128) of its bodies into
+// a register window of the S coefficients; the windows are then summed across
+// the wave in a fixed butterfly order. Orders above 8 take two windows (two
+// passes over the stream) to bound registers.
+//   KIND 0: targets = leaves, sources = the leaf's own bodies, Mhat = result.
+//   KIND 1: targets = boxes, sources = bodies of X(box) leaves, Lhat += result.
+template <int S>
+constexpr int coef_window() { return S <= 96 ? S : (S + 1) / 2; }
+
+template <int P, int KIND, int C0, int CW>
+__device__ __forceinline__ void accumulate_window(const TreeView& t, BodyStream st, const double* ch, double ih,
+                                                  int lane, double* __restrict__ out, size_t out_base) {
+    constexpr int S = nterms(P);
+    double acc[CW];
+#pragma unroll
+    for (int k = 0; k < CW; ++k) acc[k] = 0.0;
+    while (!st.done()) {
+        const int i = st.body_at(lane);
+        st.advance(64);
+        if (i >= 0) {
             const double dx = offset_from(t.x[i], Centre{ch[0], ch[1]}) * ih;
             const double dy = offset_from(t.y[i], Centre{ch[2], ch[3]}) * ih;
             const double dz = offset_from(t.z[i], Centre{ch[4], ch[5]}) * ih;
-            p2m_point<P>(t.m[i], dx, dy, dz, [&](int k, double v) { row[k] = v; });
+            auto add = [&](int k, double v) {
+                if (k >= C0 && k < C0 + CW) acc[k - C0] += v;
+            };
+            if (KIND == 0) p2m_point<P>(t.m[i], dx, dy, dz, add);
+            else p2l_point<P>(t.m[i], dx, dy, dz, add);
         }
-        __syncthreads();
-        const int rows_used = min(R, cnt - base);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int c = lane + 64 * k;
-            if (c < S)
-                for (int r = 0; r < rows_used; ++r) acc[k] += rows[r * S + c];
-        }
-        __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int c = lane + 64 * k;
-        if (c < S) M[(size_t)b * S + c] = acc[k];
+    for (int k = 0; k < CW; ++k) {
+        const double v = wave_sum(acc[k]);
+        if (lane == (k & 63) && C0 + k < S) {
+            if (KIND == 0) out[out_base + C0 + k] = v;
+            else out[out_base + C0 + k] += v;
+        }
+    }
+}
+
+template <int P, int KIND>
+__global__ void __launch_bounds__(64) k_accumulate(TreeView t, double* __restrict__ out) {
+    constexpr int S = nterms(P);
+    constexpr int CW = coef_window<S>();
+    const int lane = threadIdx.x;
+    int b;
+    BodyStream st;
+    st.seg_first = t.seg_first;
+    st.seg_count = t.seg_count;
+    if (KIND == 0) {
+        b = t.leaf_ids[blockIdx.x];
+        st.list = t.leaf_ids + blockIdx.x;
+        st.n = 1;
+    } else {
+        b = blockIdx.x;
+        const int xa = t.x_off[b];
+        st.list = t.x_idx + xa;
+        st.n = t.x_off[b + 1] - xa;
+        if (st.n == 0) return;
+    }
+    const double ch[6] = {t.cdd[6 * (size_t)b], t.cdd[6 * (size_t)b + 1], t.cdd[6 * (size_t)b + 2],
+                          t.cdd[6 * (size_t)b + 3], t.cdd[6 * (size_t)b + 4], t.cdd[6 * (size_t)b + 5]};
+    const double ih = t.invh[b];
+    accumulate_window<P, KIND, 0, CW>(t, st, ch, ih, lane, out, (size_t)b * S);
+    if constexpr (CW < S) accumulate_window<P, KIND, CW, S - CW>(t, st, ch, ih, lane, out, (size_t)b * S);
+}
+
+// ---- M2P over W (AdaptiveFMM3D.hpp:95, FmmExpansions3D.hpp:180-200) --------------
+// One wave per target leaf, lanes split into groups as in P2P: group g handles
+// W entries g, g+G, ... with their multipoles staged in LDS; partial fields are
+// reduced across groups in a fixed order and added to far[].
+template <int P>
+__global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict__ M, double* __restrict__ far,
+                                           double* __restrict__ fpot) {
+    constexpr int S = nterms(P);
+    __shared__ double mw[8][S];
+    __shared__ double red[4][64];
+    const int li = blockIdx.x;
+    const int wa = t.w_off[li], nw = t.w_off[li + 1] - wa;
+    if (nw == 0) return;
+    const int b = t.leaf_ids[li];
+    const int f = t.seg_first[b], nt = t.seg_count[b];
+    const int lane = threadIdx.x;
+    int T = 64;
+    while (T > 8 && T / 2 >= nt) T >>= 1;
+    const int Gs = 64 / T;
+    const int g = lane / T, tl = lane % T;
+    for (int tb = 0; tb < nt; tb += T) {
+        const int i = f + tb + tl;
+        const bool act = tb + tl < nt;
+        const double xi = act ? t.x[i] : 0.0, yi = act ? t.y[i] : 0.0, zi = act ? t.z[i] : 0.0;
+        double fx = 0.0, fy = 0.0, fz = 0.0, fp = 0.0;
+        for (int r = 0; r < nw; r += Gs) {
+            __syncthreads();
+            for (int q = 0; q < Gs; ++q) {
+                if (r + q >= nw) break;
+                const int w = t.w_idx[wa + r + q];
+                for (int k = lane; k < S; k += 64) mw[q][k] = M[(size_t)w * S + k];
+            }
+            __syncthreads();
+            if (act && r + g < nw) {
+                const int w = t.w_idx[wa + r + g];
+                const double* c = t.cdd + 6 * (size_t)w;
+                const double ih = t.invh[w];
+                const double dx = offset_from(xi, Centre{c[0], c[1]}) * ih;
+                const double dy = offset_from(yi, Centre{c[2], c[3]}) * ih;
+                const double dz = offset_from(zi, Centre{c[4], c[5]}) * ih;
+                double gx, gy, gz;
+                const double* mg = mw[g];
+                multipole_grad<P>([&](int k) { return mg[k]; }, dx, dy, dz, gx, gy, gz);
+                const double s2 = ih * ih;
+                fx += gx * s2;
+                fy += gy * s2;
+                fz += gz * s2;
+                if (fpot) fp += multipole_pot<P>([&](int k) { return mg[k]; }, dx, dy, dz) * ih;
+            }
+        }
+        if (Gs > 1) {
+            __syncthreads();
+            red[0][lane] = fx;
+            red[1][lane] = fy;
+            red[2][lane] = fz;
+            red[3][lane] = fp;
+            __syncthreads();
+            if (g == 0)
+                for (int q = 1; q < Gs; ++q) {
+                    fx += red[0][q * T + tl];
+                    fy += red[1][q * T + tl];
+                    fz += red[2][q * T + tl];
+                    fp += red[3][q * T + tl];
+                }
+        }
+        if (act && g == 0) {
+            far[3 * (size_t)i] += fx;
+            far[3 * (size_t)i + 1] += fy;
+            far[3 * (size_t)i + 2] += fz;
+            if (fpot) fpot[i] += fp;
+        }
     }
 }
 
@@ -128,20 +267,39 @@ __global__ void __launch_bounds__(64) k_m2l_plan(int tile, int np, int ntgt, con
     if (threadIdx.x == 0) present[(size_t)wg * kNumOffsets + o] = f ? 1 : 0;
 }
 
-template <int P>
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Operator fragment and accumulator types. FP64: v_mfma_f64_16x16x4f64 on
+// double operators. Split FP32: each operand x is carried as hi = fl32(x),
+// lo = fl32(x - hi), and a 16x16x4 product is three v_mfma_f32_16x16x4f32
+// (lo*hi + hi*lo + hi*hi) into an FP32 accumulator.
+template <bool MIX>
+struct M2LTypes {
+    using A = double;
+    using Acc = dbl4;
+};
+template <>
+struct M2LTypes<true> {
+    using A = float2;
+    using Acc = flt4;
+};
+
+template <int P, bool MIX>
 __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int ntgt,
                                                            const int* __restrict__ order,
                                                            const int* __restrict__ srcT,
                                                            const unsigned char* __restrict__ present_g,
-                                                           const double* __restrict__ Asw,
+                                                           const typename M2LTypes<MIX>::A* __restrict__ Asw,
                                                            const double* __restrict__ M, double* __restrict__ L) {
+    using AT = typename M2LTypes<MIX>::A;
+    using Acc = typename M2LTypes<MIX>::Acc;
     constexpr int S = nterms(P);
     constexpr int RB = m2l_rb(S), NKC = m2l_nkc(S), KC = m2l_kc(S), CH = m2l_chunk(S);
     constexpr int CB = m2l_cb<P>();
     constexpr int TILE = m2l_tile<P>();
     constexpr int NT = kM2LWaves * 64;
     constexpr int PER = (CH + NT - 1) / NT;
-    __shared__ double abuf[2][CH];
+    __shared__ AT abuf[2][CH];
     __shared__ unsigned char present[kNumOffsets + 1];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int j = lane & 15, kq = lane >> 4;
@@ -153,11 +311,13 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
         rank[cb] = base + (wave * CB + cb) * 16 + j;
         valid[cb] = rank[cb] < ntgt;
     }
-    dbl4 acc[CB][RB];
+    Acc acc[CB][RB];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[cb][rb][r] = 0;
     if (base < ntgt) {
         for (int t = tid; t < kNumOffsets; t += NT) present[t] = present_g[(size_t)blockIdx.x * kNumOffsets + t];
         if (tid == 0) present[kNumOffsets] = 1;  // sentinel
@@ -167,69 +327,98 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
             while (!present[o]) ++o;
             return o;  // kNumOffsets when none left
         };
+        // Software pipeline over (offset o, k chunk c) items: while the MFMAs of
+        // the current item run, the next item's operator chunk (global -> regs
+        // -> other LDS buffer) and its gathered source multipoles are in flight.
+        auto load_b = [&](const int (&src)[CB], int c, double (&bv)[CB][KC]) {
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) {
+                const double* ms = M + (size_t)(src[cb] >= 0 ? src[cb] : 0) * S;
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) {
+                    const int k = 4 * (c * KC + kk) + kq;
+                    bv[cb][kk] = (src[cb] >= 0 && k < S) ? ms[k] : 0.0;
+                }
+            }
+        };
+        auto load_src = [&](int o, int (&src)[CB]) {
+            bool any = false;
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) {
+                src[cb] = (o < kNumOffsets && valid[cb]) ? srcT[(size_t)o * np + rank[cb]] : -1;
+                any |= src[cb] >= 0;
+            }
+            return __ballot(any) != 0ull;
+        };
         int o = next_present(-1);
-        int snext[CB];
-#pragma unroll
-        for (int cb = 0; cb < CB; ++cb) snext[cb] = -1;
+        int scur[CB], snext[CB];
+        double bcur[CB][KC], bnext[CB][KC];
+        bool has_cur = false, has_next = false;
         if (o < kNumOffsets) {
-            const double* src = Asw + (size_t)o * NKC * CH;
+            const AT* src = Asw + (size_t)o * NKC * CH;
             for (int e = tid; e < CH; e += NT) abuf[0][e] = src[e];
-#pragma unroll
-            for (int cb = 0; cb < CB; ++cb) snext[cb] = valid[cb] ? srcT[(size_t)o * np + rank[cb]] : -1;
+            has_cur = load_src(o, scur);
+            if (has_cur) load_b(scur, 0, bcur);
+            has_next = load_src(next_present(o), snext);
         }
         __syncthreads();
         int buf = 0, c = 0;
-        int scur[CB];
-        bool wave_has = false;
         while (o < kNumOffsets) {
-            if (c == 0) {
-                bool any = false;
-#pragma unroll
-                for (int cb = 0; cb < CB; ++cb) {
-                    scur[cb] = snext[cb];
-                    any |= scur[cb] >= 0;
-                }
-                wave_has = __ballot(any) != 0ull;
-                const int o2 = next_present(o);
-#pragma unroll
-                for (int cb = 0; cb < CB; ++cb)
-                    snext[cb] = (o2 < kNumOffsets && valid[cb]) ? srcT[(size_t)o2 * np + rank[cb]] : -1;
-            }
             int on = o, cn = c + 1;
             if (cn == NKC) {
                 cn = 0;
                 on = next_present(o);
             }
-            double pre[PER];
+            AT pre[PER];
             if (on < kNumOffsets) {
-                const double* src = Asw + ((size_t)on * NKC + cn) * CH;
+                const AT* src = Asw + ((size_t)on * NKC + cn) * CH;
 #pragma unroll
                 for (int i = 0; i < PER; ++i) {
                     const int e = tid + NT * i;
-                    pre[i] = (e < CH) ? src[e] : 0.0;
+                    pre[i] = (e < CH) ? src[e] : AT{};
                 }
             }
-            if (wave_has) {
-                double bv[CB][KC];
+            // sources and B of the next item
+            const bool nxt_has = (cn == 0) ? has_next : has_cur;
+            if (on < kNumOffsets && nxt_has) {
+                if (cn == 0) load_b(snext, 0, bnext);
+                else load_b(scur, cn, bnext);
+            }
+            if (has_cur) {
+                const AT* ab = abuf[buf] + lane;
+                if constexpr (MIX) {
+                    float bh[CB][KC], bl[CB][KC];
 #pragma unroll
-                for (int cb = 0; cb < CB; ++cb) {
-                    const double* ms = M + (size_t)(scur[cb] >= 0 ? scur[cb] : 0) * S;
+                    for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
-                    for (int kk = 0; kk < KC; ++kk) {
-                        const int k = 4 * (c * KC + kk) + kq;
-                        bv[cb][kk] = (scur[cb] >= 0 && k < S) ? ms[k] : 0.0;
-                    }
+                        for (int kk = 0; kk < KC; ++kk) {
+                            bh[cb][kk] = (float)bcur[cb][kk];
+                            bl[cb][kk] = (float)(bcur[cb][kk] - (double)bh[cb][kk]);
+                        }
+#pragma unroll
+                    for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                        for (int rb = 0; rb < RB; ++rb) {
+                            const float2 a = ab[(rb * KC + kk) * 64];
+#pragma unroll
+                            for (int cb = 0; cb < CB; ++cb) {
+                                acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bh[cb][kk], acc[cb][rb], 0, 0, 0);
+                                acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bl[cb][kk], acc[cb][rb], 0, 0, 0);
+                                acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bh[cb][kk], acc[cb][rb], 0, 0, 0);
+                            }
+                        }
+                } else {
+#pragma unroll
+                    for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                        for (int rb = 0; rb < RB; ++rb) {
+                            const double a = ab[(rb * KC + kk) * 64];
+#pragma unroll
+                            for (int cb = 0; cb < CB; ++cb)
+                                acc[cb][rb] =
+                                    __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[cb][kk], acc[cb][rb], 0, 0, 0);
+                        }
                 }
-                const double* ab = abuf[buf] + lane;
-#pragma unroll
-                for (int kk = 0; kk < KC; ++kk)
-#pragma unroll
-                    for (int rb = 0; rb < RB; ++rb) {
-                        const double a = ab[(rb * KC + kk) * 64];
-#pragma unroll
-                        for (int cb = 0; cb < CB; ++cb)
-                            acc[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[cb][kk], acc[cb][rb], 0, 0, 0);
-                    }
             }
             if (on < kNumOffsets) {
 #pragma unroll
@@ -240,6 +429,16 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
             }
             __syncthreads();
             buf ^= 1;
+            if (cn == 0 && on < kNumOffsets) {
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb) scur[cb] = snext[cb];
+                has_cur = has_next;
+                has_next = load_src(next_present(on), snext);
+            }
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) bcur[cb][kk] = bnext[cb][kk];
             o = on;
             c = cn;
         }
@@ -252,8 +451,9 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = 16 * rb + kq + 4 * r;
-                if (row < S) L[(size_t)t * S + row] = acc[cb][rb][r];
+                // D(i, j) of lane j + 16 kq, register r: i = kq + 4r (f64), 4kq + r (f32)
+                const int row = MIX ? 16 * rb + 4 * kq + r : 16 * rb + kq + 4 * r;
+                if (row < S) L[(size_t)t * S + row] = (double)acc[cb][rb][r];
             }
     }
 }
@@ -330,56 +530,10 @@ __global__ void __launch_bounds__(64) k_level_mfma(int n, const int* __restrict_
         }
 }
 
-// ---- P2L: one wave per box with an X list; rows of q*conj(S) in LDS.
-template <int P>
-__global__ void __launch_bounds__(64) k_p2l(TreeView t, double* __restrict__ L) {
-    constexpr int S = nterms(P);
-    constexpr int K = (S + 63) / 64;
-    constexpr int R = lds_rows<S>();
-    __shared__ double rows[R * S];
-    const int b = blockIdx.x;
-    const int xa = t.x_off[b], xe = t.x_off[b + 1];
-    if (xa == xe) return;
-    const int lane = threadIdx.x;
-    const double ch[6] = {t.cdd[6 * (size_t)b], t.cdd[6 * (size_t)b + 1], t.cdd[6 * (size_t)b + 2],
-                          t.cdd[6 * (size_t)b + 3], t.cdd[6 * (size_t)b + 4], t.cdd[6 * (size_t)b + 5]};
-    const double ih = t.invh[b];
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    for (int e = xa; e < xe; ++e) {
-        const int c = t.x_idx[e];
-        const int f = t.seg_first[c], cnt = t.seg_count[c];
-        for (int base = 0; base < cnt; base += R) {
-            double* row = rows + lane * S;
-            const int i = f + base + lane;
-            if (lane < R && base + lane < cnt) {
-                const double dx = offset_from(t.x[i], Centre{ch[0], ch[1]}) * ih;
-                const double dy = offset_from(t.y[i], Centre{ch[2], ch[3]}) * ih;
-                const double dz = offset_from(t.z[i], Centre{ch[4], ch[5]}) * ih;
-                p2l_point<P>(t.m[i], dx, dy, dz, [&](int k, double v) { row[k] = v; });
-            }
-            __syncthreads();
-            const int rows_used = min(R, cnt - base);
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int cc = lane + 64 * k;
-                if (cc < S)
-                    for (int r = 0; r < rows_used; ++r) acc[k] += rows[r * S + cc];
-            }
-            __syncthreads();
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int r = lane + 64 * k;
-        if (r < S) L[(size_t)b * S + r] += acc[k];
-    }
-}
-
 // ---- L2P: one thread per sorted body (far field from its leaf's local).
 template <int P>
-__global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict__ L, double* __restrict__ far) {
+__global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict__ L, double* __restrict__ far,
+                                            double* __restrict__ fpot) {
     constexpr int S = nterms(P);
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= t.n) return;
@@ -401,38 +555,7 @@ __global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict
     far[3 * (size_t)i] = gx * s2;
     far[3 * (size_t)i + 1] = gy * s2;
     far[3 * (size_t)i + 2] = gz * s2;
-}
-
-// ---- M2P over W: adds each W box's multipole gradient (W is per leaf rank).
-template <int P>
-__global__ void __launch_bounds__(kT) k_far_w(TreeView t, const int* __restrict__ leaf_rank,
-                                             const double* __restrict__ M, double* __restrict__ far) {
-    constexpr int S = nterms(P);
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= t.n) return;
-    const int li = leaf_rank[t.leaf_of[i]];
-    const int wa = t.w_off[li], we = t.w_off[li + 1];
-    if (wa == we) return;
-    const double xi = t.x[i], yi = t.y[i], zi = t.z[i];
-    double fx = far[3 * (size_t)i], fy = far[3 * (size_t)i + 1], fz = far[3 * (size_t)i + 2];
-    for (int e = wa; e < we; ++e) {
-        const int w = t.w_idx[e];
-        const double* c = t.cdd + 6 * (size_t)w;
-        const double ih = t.invh[w];
-        const double dx = offset_from(xi, Centre{c[0], c[1]}) * ih;
-        const double dy = offset_from(yi, Centre{c[2], c[3]}) * ih;
-        const double dz = offset_from(zi, Centre{c[4], c[5]}) * ih;
-        const double* mw = M + (size_t)w * S;
-        double gx, gy, gz;
-        multipole_grad<P>([&](int k) { return mw[k]; }, dx, dy, dz, gx, gy, gz);
-        const double s2 = ih * ih;
-        fx += gx * s2;
-        fy += gy * s2;
-        fz += gz * s2;
-    }
-    far[3 * (size_t)i] = fx;
-    far[3 * (size_t)i + 1] = fy;
-    far[3 * (size_t)i + 2] = fz;
+    if (fpot) fpot[i] = local_pot<P>([&](int k) { return lb[k]; }, dx, dy, dz) * t.invh[b];
 }
 
 // ---- P2P over U + final combine (AdaptiveFMM3D.hpp:96-106) ------------------
@@ -454,16 +577,22 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
     return y;
 }
 
-__global__ void __launch_bounds__(64) k_p2p(TreeView t, double G, double eps2, const double* __restrict__ far,
-                                           double* __restrict__ acc) {
+// MIX: the pair difference stays FP64 (it carries the cancellation), then
+// r^2, v_rsq_f32 and the pair term run in FP32; each lane sums its terms of
+// one 64-body source tile in FP32 and adds that partial to its FP64 accumulator.
+template <bool MIX, bool POT>
+__global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict__ far, double* __restrict__ acc,
+                                           const double* __restrict__ fpot, double* __restrict__ phi) {
     __shared__ double sx[64], sy[64], sz[64], sm[64];
     __shared__ int sj[64];
     __shared__ int ufirst[kP2PMaxU], ucount[kP2PMaxU];
-    __shared__ double red[3][64];
+    __shared__ double red[4][64];
     const int li = blockIdx.x;
     const int b = t.leaf_ids[li];
     const int f = t.seg_first[b], nt = t.seg_count[b];
     const int lane = threadIdx.x;
+    const int q = t.bsys[b];
+    const double G = t.sys_G[q], eps2 = t.sys_eps2[q];
     const int ua = t.u_off[li], nu = t.u_off[li + 1] - ua;
     const bool cached = nu <= kP2PMaxU;
     if (cached)
@@ -491,7 +620,7 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, double G, double eps2, c
         const int i = f + tb + tl;
         const bool act = tb + tl < nt;
         const double xi = act ? t.x[i] : 0.0, yi = act ? t.y[i] : 0.0, zi = act ? t.z[i] : 0.0;
-        double ax = 0.0, ay = 0.0, az = 0.0;
+        double ax = 0.0, ay = 0.0, az = 0.0, ap = 0.0;
         int e = 0, off = 0;  // stream position: range e, offset off
         while (e < nu) {
             // lane takes stream position (e, off) + lane
@@ -533,16 +662,41 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, double G, double eps2, c
                 }
             }
             __syncthreads();
-            for (int k = g; k < 64; k += Gs) {
-                const int j = sj[k];
-                if (j < 0) break;
-                const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
-                const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-                const double rin = rsqrt_f64(r2);
-                const double s = (j != i) ? sm[k] * (rin * rin * rin) : 0.0;
-                ax += s * dx;
-                ay += s * dy;
-                az += s * dz;
+            if constexpr (MIX) {
+                const float e2 = (float)eps2;
+                float fx = 0.f, fy = 0.f, fz = 0.f, fp = 0.f;
+                for (int k = g; k < 64; k += Gs) {
+                    const int j = sj[k];
+                    if (j < 0) break;
+                    const float dx = (float)(sx[k] - xi), dy = (float)(sy[k] - yi), dz = (float)(sz[k] - zi);
+                    const float r2 = dx * dx + dy * dy + dz * dz + e2;
+                    const float rin = __builtin_amdgcn_rsqf(r2);
+                    // the self pair is selected away, not multiplied by zero
+                    // (with eps = 0 its 1/r is infinite)
+                    const float mj = (float)sm[k];
+                    const float s = (j != i) ? mj * (rin * rin * rin) : 0.f;
+                    fx += s * dx;
+                    fy += s * dy;
+                    fz += s * dz;
+                    if constexpr (POT) fp += (j != i) ? mj * rin : 0.f;
+                }
+                ax += (double)fx;
+                ay += (double)fy;
+                az += (double)fz;
+                ap += (double)fp;
+            } else {
+                for (int k = g; k < 64; k += Gs) {
+                    const int j = sj[k];
+                    if (j < 0) break;
+                    const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+                    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                    const double rin = rsqrt_f64(r2);
+                    const double s = (j != i) ? sm[k] * (rin * rin * rin) : 0.0;
+                    ax += s * dx;
+                    ay += s * dy;
+                    az += s * dz;
+                    if constexpr (POT) ap += (j != i) ? sm[k] * rin : 0.0;
+                }
             }
             __syncthreads();
         }
@@ -550,12 +704,14 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, double G, double eps2, c
             red[0][lane] = ax;
             red[1][lane] = ay;
             red[2][lane] = az;
+            if constexpr (POT) red[3][lane] = ap;
             __syncthreads();
             if (g == 0) {
                 for (int q = 1; q < Gs; ++q) {
                     ax += red[0][q * T + tl];
                     ay += red[1][q * T + tl];
                     az += red[2][q * T + tl];
+                    if constexpr (POT) ap += red[3][q * T + tl];
                 }
             }
             __syncthreads();
@@ -565,26 +721,22 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, double G, double eps2, c
             acc[3 * (size_t)o] = G * (far[3 * (size_t)i] + ax);
             acc[3 * (size_t)o + 1] = G * (far[3 * (size_t)i + 1] + ay);
             acc[3 * (size_t)o + 2] = G * (far[3 * (size_t)i + 2] + az);
+            if constexpr (POT) phi[o] = -G * (fpot[i] + ap);
         }
     }
 }
 
-__global__ void k_leaf_rank(int nleaves, const int* leaf_ids, int* rank) {
-    const int li = blockIdx.x * kT + threadIdx.x;
-    if (li < nleaves) rank[leaf_ids[li]] = li;
-}
-
 template <int P>
-void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, double G,
-         double eps, double* acc, hipEvent_t* ev) {
+void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, bool mixed,
+         double* acc, double* phi, hipEvent_t* ev) {
     constexpr int S = nterms(P);
     const int nb = t.nb;
     double* M = pool.get<double>("Mexp", (size_t)nb * S);
     double* L = pool.get<double>("Lexp", (size_t)nb * S);
     double* far = pool.get<double>("far", 3 * (size_t)t.n);
-    int* leaf_rank = pool.get<int>("leaf_rank", nb);
+    double* fpot = phi ? pool.get<double>("farpot", t.n) : nullptr;
     if (ev) AFMM_CHECK(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_p2m<P>, dim3(t.nleaves), dim3(64), 0, s, t, M);
+    hipLaunchKernelGGL((k_accumulate<P, 0>), dim3(t.nleaves), dim3(64), 0, s, t, M);
     for (int l = t.max_level - 1; l >= 0; --l) {
         const int off = meta.nonleaf_off[l], cnt = meta.nonleaf_off[l + 1] - off;
         if (cnt > 0)
@@ -606,23 +758,32 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         hipLaunchKernelGGL(k_m2l_src, dim3(blocks(nb)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv, t.ic, srcT);
         hipLaunchKernelGGL(k_m2l_plan, dim3(nwg * kNumOffsets), dim3(64), 0, s, TILE, np, nb, srcT, present);
         if (ev) AFMM_CHECK(hipEventRecord(ev[5], s));
-        hipLaunchKernelGGL(k_m2l_mfma<P>, dim3(nwg), dim3(kM2LWaves * 64), 0, s, nb, np, nb, t.lo_order, srcT, present,
-                           tab.m2l_sw, M, L);
+        if (mixed)
+            hipLaunchKernelGGL((k_m2l_mfma<P, true>), dim3(nwg), dim3(kM2LWaves * 64), 0, s, nb, np, nb, t.lo_order,
+                               srcT, present, tab.m2l_hl, M, L);
+        else
+            hipLaunchKernelGGL((k_m2l_mfma<P, false>), dim3(nwg), dim3(kM2LWaves * 64), 0, s, nb, np, nb, t.lo_order,
+                               srcT, present, tab.m2l_sw, M, L);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
-    if (t.n_x > 0) hipLaunchKernelGGL(k_p2l<P>, dim3(nb), dim3(64), 0, s, t, L);
+    if (t.n_x > 0) hipLaunchKernelGGL((k_accumulate<P, 1>), dim3(nb), dim3(64), 0, s, t, L);
     for (int l = 1; l <= t.max_level; ++l) {
         const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
         hipLaunchKernelGGL((k_level_mfma<P, 1>), dim3((cnt + 15) / 16), dim3(64), 0, s, cnt, t.lo_order + off,
                            t.child_first, t.child_mask, t.parent, t.octant, tab.l2l_sw, L, L);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[3], s));
-    hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far);
-    if (t.n_w > 0) {
-        hipLaunchKernelGGL(k_leaf_rank, dim3(blocks(t.nleaves)), dim3(kT), 0, s, t.nleaves, t.leaf_ids, leaf_rank);
-        hipLaunchKernelGGL(k_far_w<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, leaf_rank, M, far);
-    }
-    hipLaunchKernelGGL(k_p2p, dim3(t.nleaves), dim3(64), 0, s, t, G, eps * eps, far, acc);
+    hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far, fpot);
+    if (t.n_w > 0) hipLaunchKernelGGL(k_m2p<P>, dim3(t.nleaves), dim3(64), 0, s, t, M, far, fpot);
+    const dim3 pg(t.nleaves), pb(64);
+    if (mixed && phi)
+        hipLaunchKernelGGL((k_p2p<true, true>), pg, pb, 0, s, t, far, acc, fpot, phi);
+    else if (mixed)
+        hipLaunchKernelGGL((k_p2p<true, false>), pg, pb, 0, s, t, far, acc, fpot, phi);
+    else if (phi)
+        hipLaunchKernelGGL((k_p2p<false, true>), pg, pb, 0, s, t, far, acc, fpot, phi);
+    else
+        hipLaunchKernelGGL((k_p2p<false, false>), pg, pb, 0, s, t, far, acc, fpot, phi);
     if (ev) AFMM_CHECK(hipEventRecord(ev[4], s));
     AFMM_CHECK(hipGetLastError());
 }
@@ -630,10 +791,11 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
 }  // namespace
 
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
-             double G, double eps, double* d_acc, hipEvent_t* ev) {
+             bool mixed, double* d_acc, double* d_phi, hipEvent_t* ev) {
+    if (mixed && !tab.m2l_hl) throw std::runtime_error("split FP32 M2L operators not loaded");
     switch (p) {
 #define AFMM_CASE(PP) \
-    case PP: run<PP>(pool, s, tv, meta, tab, G, eps, d_acc, ev); return;
+    case PP: run<PP>(pool, s, tv, meta, tab, mixed, d_acc, d_phi, ev); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
 #undef AFMM_CASE

@@ -90,8 +90,12 @@ constexpr int kSortRounds = 8;
 constexpr int kSortTile = kSortThreads * kSortRounds;  // 2048 items per block
 constexpr int kBuckets = 512;
 
+// digit g: 0 = klo bits [0, 9); 1..7 = khi bits [9(g-1), 9g); 8, 9 = klo bits
+// [9, 18), [18, 27) (the system index of a forest, above every path digit).
 __device__ __forceinline__ int digit_of(uint64_t khi, uint32_t klo, int g) {
-    return g == 0 ? int(klo & 511u) : int((khi >> (9 * (g - 1))) & 511u);
+    if (g == 0) return int(klo & 511u);
+    if (g >= 8) return int((klo >> (9 * (g - 7))) & 511u);
+    return int((khi >> (9 * (g - 1))) & 511u);
 }
 
 __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* khi, const uint32_t* klo,
@@ -182,9 +186,9 @@ void scan_exclusive(const int* d_in, int* d_out, int n, int* d_total, int* d_tmp
 
 int sort_blocks(int n) { return (n + kSortTile - 1) / kSortTile; }
 
-int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s) {
-    int cur = 0;
-    if (n <= 1) return 0;
+int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, int start) {
+    int cur = start;
+    if (n <= 1) return cur;
     const int nblocks = sort_blocks(n);
     for (int k = 0; k < ndigits; ++k) {
         const int g = first_digit + k;

@@ -55,9 +55,11 @@ __device__ __forceinline__ void take_hi(Ext& a, Ext b) {
     if (b.v > a.v || (b.v == a.v && b.i < a.i)) a = b;
 }
 
-constexpr int kBBoxBlocks = 512;
+constexpr int kBBoxChunk = 8192;
 
-__global__ void __launch_bounds__(kT) k_bbox_partial(const double* pos, const double* mass, int n,
+// One block per chunk [cbeg[c], cend[c]) of one system's bodies.
+__global__ void __launch_bounds__(kT) k_bbox_partial(const double* pos, const double* mass,
+                                                    const int* __restrict__ cbeg, const int* __restrict__ cend,
                                                     double* pv, int* pi, int* pbad) {
     Ext e[6];
     for (int a = 0; a < 3; ++a) {
@@ -65,7 +67,8 @@ __global__ void __launch_bounds__(kT) k_bbox_partial(const double* pos, const do
         e[3 + a] = Ext{-1e300, -1};
     }
     int bad = 0;
-    for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
+    const int i0 = cbeg[blockIdx.x], i1 = cend[blockIdx.x];
+    for (int i = i0 + threadIdx.x; i < i1; i += kT) {
         for (int a = 0; a < 3; ++a) {
             const double v = pos[3 * (size_t)i + a];
             bad += !isfinite(v);
@@ -99,13 +102,16 @@ __global__ void __launch_bounds__(kT) k_bbox_partial(const double* pos, const do
     }
 }
 
-// AdaptiveOctree::build (:71-81) from the partials; also the key depth.
+// AdaptiveOctree::build (:71-81) of system blockIdx.x from its chunks'
+// partials [cfirst[q], cfirst[q+1]); also the key depth.
 __global__ void __launch_bounds__(kT) k_bbox_final(const double* pv, const int* pi, const int* pbad, int nparts,
-                                                  double eps_floor, RootInfo* root) {
+                                                  const int* __restrict__ cfirst,
+                                                  const double* __restrict__ eps_floor, RootInfo* roots) {
     Ext e[6];
     for (int a = 0; a < 3; ++a) e[a] = Ext{1e300, -1}, e[3 + a] = Ext{-1e300, -1};
     int bad = 0;
-    for (int b = threadIdx.x; b < nparts; b += kT) {
+    const int q = blockIdx.x;
+    for (int b = cfirst[q] + threadIdx.x; b < cfirst[q + 1]; b += kT) {
         for (int k = 0; k < 6; ++k) {
             const Ext x{pv[k * nparts + b], pi[k * nparts + b]};
             if (k < 3) take_lo(e[k], x);
@@ -146,13 +152,13 @@ __global__ void __launch_bounds__(kT) k_bbox_final(const double* pv, const int* 
     // Levels 0..L-1 may split (2*half_l > eps floor, level < 24): keys carry L digits.
     int L = 0;
     double h = half;
-    while (L < kMaxDepth && 2.0 * h > eps_floor) {
+    while (L < kMaxDepth && 2.0 * h > eps_floor[q]) {
         ++L;
         h = 0.5 * h;
     }
     r.key_levels = L;
     r.nonfinite = sb[0];
-    *root = r;
+    roots[q] = r;
 }
 
 // ---- path keys ----------------------------------------------------------------
@@ -161,11 +167,23 @@ __device__ __forceinline__ int level_digit(uint64_t khi, uint32_t klo, int l) {
     return l <= 21 ? int((khi >> (63 - 3 * l)) & 7u) : int((klo >> (3 * (24 - l))) & 7u);
 }
 
-__global__ void __launch_bounds__(kT) k_keys(const double* pos, int n, const RootInfo* rootp,
-                                            uint64_t* khi, uint32_t* klo, int* val) {
+// System of body i: the q with sys_off[q] <= i < sys_off[q+1].
+__device__ __forceinline__ int system_of(int i, const int* sys_off, int nsys) {
+    int lo = 0, hi = nsys - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sys_off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kT) k_keys(const double* pos, int n, int nsys, const int* __restrict__ sys_off,
+                                            const RootInfo* roots, uint64_t* khi, uint32_t* klo, int* val) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
-    const RootInfo r = *rootp;
+    const int q = system_of(i, sys_off, nsys);
+    const RootInfo r = roots[q];
     const double x = pos[3 * (size_t)i], y = pos[3 * (size_t)i + 1], z = pos[3 * (size_t)i + 2];
     double cx = r.c[0], cy = r.c[1], cz = r.c[2], h = r.half;
     uint64_t hi = 0;
@@ -180,24 +198,25 @@ __global__ void __launch_bounds__(kT) k_keys(const double* pos, int n, const Roo
         else lo |= uint32_t(o) << (3 * (24 - l));
     }
     khi[i] = hi;
-    klo[i] = lo;
+    klo[i] = lo | (uint32_t(q) << 9);
     val[i] = i;
 }
 
 // ---- topology, one level at a time --------------------------------------------
 // For each box of level l: split decision and child ranges by binary search on
 // the level-(l+1) digit inside its (sorted) segment.
-__global__ void __launch_bounds__(kT) k_split(int off, int cnt, int l, int cap, double eps_floor,
-                                             const RootInfo* rootp, const int* seg_first,
+__global__ void __launch_bounds__(kT) k_split(int off, int cnt, int l, int cap, const double* eps_floor,
+                                             const RootInfo* roots, const int* bsys, const int* seg_first,
                                              const int* seg_count, const uint64_t* khi, const uint32_t* klo,
                                              int* bnd, int* nch) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= cnt) return;
     const int b = off + i;
     const int f = seg_first[b], c = seg_count[b];
-    const double half = ldexp(rootp->half, -l);
+    const int q = bsys[b];
+    const double half = ldexp(roots[q].half, -l);
     int* bd = bnd + 9 * (size_t)i;
-    const bool split = c > cap && !(2.0 * half <= eps_floor) && l < kMaxDepth;
+    const bool split = c > cap && !(2.0 * half <= eps_floor[q]) && l < kMaxDepth;
     if (!split) {
         nch[i] = 0;
         return;
@@ -221,7 +240,7 @@ __global__ void __launch_bounds__(kT) k_split(int off, int cnt, int l, int cap, 
 __global__ void __launch_bounds__(kT) k_emit(int off, int cnt, int l, int next_off, const int* bnd,
                                             const int* nch, const int* cbase, int* seg_first,
                                             int* seg_count, int* level, int* parent, int* octant,
-                                            int* child_first, int* nchild, unsigned char* mask) {
+                                            int* child_first, int* nchild, unsigned char* mask, int* bsys) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= cnt) return;
     const int b = off + i;
@@ -246,23 +265,32 @@ __global__ void __launch_bounds__(kT) k_emit(int off, int cnt, int l, int next_o
         level[c] = l + 1;
         parent[c] = b;
         octant[c] = o;
+        bsys[c] = bsys[b];
     }
     mask[b] = mk;
 }
 
-__global__ void k_init_root(int n, int* seg_first, int* seg_count, int* level, int* parent, int* octant) {
-    if (threadIdx.x != 0) return;
-    seg_first[0] = 0;
-    seg_count[0] = n;
-    level[0] = 0;
-    parent[0] = -1;
-    octant[0] = 0;
+// Box q of level 0 is the root of system q (bodies [sys_off[q], sys_off[q+1])).
+__global__ void k_init_roots(int nsys, const int* sys_off, int* seg_first, int* seg_count, int* level, int* parent,
+                             int* octant, int* bsys) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nsys) return;
+    seg_first[q] = sys_off[q];
+    seg_count[q] = sys_off[q + 1] - sys_off[q];
+    level[q] = 0;
+    parent[q] = -1;
+    octant[q] = 0;
+    bsys[q] = q;
 }
 
 // ---- reference numbering -------------------------------------------------------
 __global__ void k_fill(int* a, int n, int v) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i < n) a[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0ull;
 }
 
 __global__ void __launch_bounds__(kT) k_count_starts(int nb, const int* seg_first, const int* level,
@@ -284,13 +312,17 @@ __global__ void __launch_bounds__(kT) k_preorder(int nb, const int* seg_first, c
     nch_by_pre[p] = nchild[b];
 }
 
-__global__ void __launch_bounds__(kT) k_refid(int nb, const int* pre, const int* base_by_pre,
-                                             const int* child_first, const int* nchild, int* refid) {
+// Ids restart at 0 in every system: its root (box q, pre-order rank pre[q])
+// precedes all of its boxes, so base_by_pre[pre[q]] counts the boxes of the
+// systems before it that are some box's child.
+__global__ void __launch_bounds__(kT) k_refid(int nb, const int* pre, const int* base_by_pre, const int* parent,
+                                             const int* bsys, const int* child_first, const int* nchild,
+                                             int* refid) {
     const int b = blockIdx.x * kT + threadIdx.x;
     if (b >= nb) return;
-    if (b == 0) refid[0] = 0;
+    if (parent[b] < 0) refid[b] = 0;
     const int k = nchild[b];
-    const int first = 1 + base_by_pre[pre[b]];
+    const int first = 1 + base_by_pre[pre[b]] - base_by_pre[pre[bsys[b]]];
     for (int r = 0; r < k; ++r) refid[child_first[b] + r] = first + r;
 }
 
@@ -354,12 +386,13 @@ __global__ void __launch_bounds__(kT) k_gather(int n, const int* perm, const dou
 // ---- box geometry -----------------------------------------------------------------
 // Replays the reference's centre arithmetic along the box's path (digits of
 // any of its bodies' keys), and derives the exact centre and integer coords.
-__global__ void __launch_bounds__(kT) k_geometry(int nb, const RootInfo* rootp, const int* seg_first,
-                                                const int* level, const uint64_t* khi, const uint32_t* klo,
-                                                double* cfp, double* cdd, double* half, double* invh, int* ic) {
+__global__ void __launch_bounds__(kT) k_geometry(int nb, const RootInfo* roots, const int* bsys,
+                                                const int* seg_first, const int* level, const uint64_t* khi,
+                                                const uint32_t* klo, double* cfp, double* cdd, double* half,
+                                                double* invh, int* ic) {
     const int b = blockIdx.x * kT + threadIdx.x;
     if (b >= nb) return;
-    const RootInfo r = *rootp;
+    const RootInfo r = roots[bsys[b]];
     const int L = level[b];
     const uint64_t kh = khi[seg_first[b]];
     const uint32_t kl = klo[seg_first[b]];
@@ -397,7 +430,8 @@ __device__ __forceinline__ bool adjacent(const double* cfp, const double* half, 
 // children of the parent's colleagues, colleague order then octant order.
 // One wave per box: lane k takes candidate k = 8*slot + octant (<= 216), and
 // ballots keep the emitted order (= ascending id = Morton order in a level).
-__global__ void __launch_bounds__(64) k_coll_v(int off, double tol, const int* __restrict__ parent,
+__global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restrict__ roots,
+                                              const int* __restrict__ bsys, const int* __restrict__ parent,
                                               const int* __restrict__ child_first,
                                               const unsigned char* __restrict__ mask, const double* __restrict__ cfp,
                                               const double* __restrict__ half, const int* __restrict__ ic,
@@ -405,15 +439,16 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, double tol, const int* _
                                               int* __restrict__ nv, int* __restrict__ overflow) {
     const int b = off + blockIdx.x;
     const int lane = threadIdx.x;
-    if (b == 0) {
+    const int par = parent[b];
+    if (par < 0) {  // a root: colleague[root] = {root} (AdaptiveOctree.hpp:143)
         if (lane == 0) {
-            coll[0] = 0;
-            ncoll[0] = 1;
-            nv[0] = 0;
+            coll[kCollSlots * (size_t)b] = b;
+            ncoll[b] = 1;
+            nv[b] = 0;
         }
         return;
     }
-    const int par = parent[b];
+    const double tol = roots[bsys[b]].tol;
     const int npc = ncoll[par];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int kc = 0, kv = 0;
@@ -463,73 +498,186 @@ __device__ __forceinline__ int dfs_next(int node, int stop, const int* parent, c
     return -1;
 }
 
-// U and W of each leaf b in one pass (AdaptiveOctree.hpp:163-169, :176-189).
-// U = every non-empty leaf adjacent to b. A coarser or same-level one is a
-// leaf colleague of b's ancestor at its own level (it contains a box adjacent
-// to b); a finer one lies under a non-leaf colleague of b. The descent into
-// each non-leaf colleague (colleague order, octant DFS) is the reference's
-// descendW: a separated box goes to W and is not entered, an adjacent leaf is
-// a finer U entry, an adjacent non-leaf is entered. U is then put in the
-// reference's DFS-from-root order, which for disjoint leaves is ascending
-// first body. mode 0 counts, mode 1 writes.
-__global__ void __launch_bounds__(kT) k_uwlist(int nleaves, const int* __restrict__ leaf_ids, double tol,
+__device__ __forceinline__ int wave_excl_scan(int v, int lane) {
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    return incl - v;
+}
+__device__ __forceinline__ int wave_total(int v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// U and W of each leaf b (AdaptiveOctree.hpp:163-169, :176-189), one wave per
+// leaf. U = every non-empty leaf adjacent to b: a coarser or same-level one is a
+// leaf colleague of b's ancestor at its own level (it contains the ancestor's
+// neighbourhood cell touching b); a finer one lies under a non-leaf colleague of
+// b. Lanes split the (ancestor, colleague slot) candidates; lane s descends
+// non-leaf colleague s the reference's way (descendW: a separated box goes to W
+// and is not entered, an adjacent leaf is a finer U entry, an adjacent non-leaf
+// is entered, octant order). W keeps colleague order then DFS order through a
+// wave prefix sum; U is then ranked by first body, which for disjoint leaves
+// is the reference's DFS-from-root order. mode 0 counts, mode 1 writes.
+constexpr int kUWMaxAnc = kMaxDepth + 1;
+constexpr int kUWSortCap = 1024;
+
+__global__ void __launch_bounds__(64) k_uwlist(int nleaves, const int* __restrict__ leaf_ids,
+                                              const RootInfo* __restrict__ roots, const int* __restrict__ bsys,
                                               const int* __restrict__ parent, const int* __restrict__ child_first,
                                               const int* __restrict__ nchild, const int* __restrict__ seg_first,
-                                              const int* __restrict__ level, const double* __restrict__ cfp,
-                                              const double* __restrict__ half, const int* __restrict__ coll,
-                                              const int* __restrict__ ncoll, int mode, int* __restrict__ ucnt,
-                                              int* __restrict__ wcnt, const int* __restrict__ uoff,
-                                              const int* __restrict__ woff, int* __restrict__ uidx,
-                                              int* __restrict__ widx) {
-    const int li = blockIdx.x * kT + threadIdx.x;
-    if (li >= nleaves) return;
+                                              const int* __restrict__ seg_count,
+                                              const double* __restrict__ cfp, const double* __restrict__ half,
+                                              const int* __restrict__ coll, const int* __restrict__ ncoll, int mode,
+                                              int* __restrict__ ucnt, int* __restrict__ wcnt,
+                                              const int* __restrict__ uoff, const int* __restrict__ woff,
+                                              int* __restrict__ uidx, int* __restrict__ widx,
+                                              int* __restrict__ usrc) {
+    __shared__ int anc[kUWMaxAnc], cum[kUWMaxAnc + 1];
+    __shared__ int nanc;
+    __shared__ int ubuf[kUWSortCap];
+    const int li = blockIdx.x;
     const int b = leaf_ids[li];
-    int ku = 0, kw = 0;
+    const int lane = threadIdx.x;
+    const double tol = roots[bsys[b]].tol;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (lane == 0) {
+        int k = 0, a = b, run = 0;
+        for (;;) {
+            anc[k] = a;
+            cum[k] = run;
+            run += ncoll[a];
+            ++k;
+            if (parent[a] < 0) break;
+            a = parent[a];
+        }
+        cum[k] = run;
+        nanc = k;
+    }
+    __syncthreads();
+    const int ncand = cum[nanc];
     int* ud = mode ? uidx + uoff[li] : nullptr;
     int* wd = mode ? widx + woff[li] : nullptr;
     // coarser and same-level adjacent leaves
-    for (int a = b;; a = parent[a]) {
-        for (int sl = 0; sl < ncoll[a]; ++sl) {
-            const int c = coll[kCollSlots * (size_t)a + sl];
-            if (nchild[c] == 0 && adjacent(cfp, half, tol, c, b)) {
-                if (mode) ud[ku] = c;
-                ++ku;
+    int ku = 0;
+    for (int base = 0; base < ncand; base += 64) {
+        const int q = base + lane;
+        int c = -1;
+        if (q < ncand) {
+            int k = 0;
+            while (cum[k + 1] <= q) ++k;
+            c = coll[kCollSlots * (size_t)anc[k] + (q - cum[k])];
+            if (!(nchild[c] == 0 && adjacent(cfp, half, tol, c, b))) c = -1;
+        }
+        const uint64_t bal = __ballot(c >= 0);
+        if (mode && c >= 0) ud[ku + __popcll(bal & lt)] = c;
+        ku += __popcll(bal);
+    }
+    // finer U and W: lane s descends non-leaf colleague s
+    const int nc = ncoll[b];
+    int myu = 0, myw = 0;
+    int c = -1;
+    if (lane < nc) {
+        c = coll[kCollSlots * (size_t)b + lane];
+        if (c == b || nchild[c] == 0) c = -1;
+    }
+    for (int pass = 0; pass < (mode ? 2 : 1); ++pass) {
+        int uo = 0, wo = 0;
+        if (pass == 1) {
+            uo = ku + wave_excl_scan(myu, lane);
+            wo = wave_excl_scan(myw, lane);
+        }
+        int nu_ = 0, nw_ = 0;
+        if (c >= 0) {
+            int node = child_first[c];
+            while (node >= 0) {
+                if (!adjacent(cfp, half, tol, node, b)) {
+                    if (pass == 1) wd[wo + nw_] = node;
+                    ++nw_;
+                    node = dfs_next(node, c, parent, child_first, nchild);
+                } else if (nchild[node] > 0) {
+                    node = child_first[node];
+                } else {
+                    if (pass == 1) ud[uo + nu_] = node;
+                    ++nu_;
+                    node = dfs_next(node, c, parent, child_first, nchild);
+                }
             }
         }
-        if (a == 0) break;
+        myu = nu_;
+        myw = nw_;
     }
-    // finer adjacent leaves (U) and W, descending non-leaf colleagues of b
-    for (int sl = 0; sl < ncoll[b]; ++sl) {
-        const int c = coll[kCollSlots * (size_t)b + sl];
-        if (c == b || nchild[c] == 0) continue;
-        int node = child_first[c];
-        while (node >= 0) {
-            if (!adjacent(cfp, half, tol, node, b)) {
-                if (mode) wd[kw] = node;
-                ++kw;
-                node = dfs_next(node, c, parent, child_first, nchild);
-            } else if (nchild[node] > 0) {
-                node = child_first[node];
-            } else {
-                if (mode) ud[ku] = node;
-                ++ku;
-                node = dfs_next(node, c, parent, child_first, nchild);
-            }
-        }
-    }
+    const int totu = ku + wave_total(myu), totw = wave_total(myw);
     if (!mode) {
-        ucnt[li] = ku;
-        wcnt[li] = kw;
+        if (lane == 0) {
+            ucnt[li] = totu;
+            wcnt[li] = totw;
+        }
         return;
     }
-    for (int i = 1; i < ku; ++i) {  // ascending first body = DFS order
-        const int v = ud[i], key = seg_first[v];
-        int j2 = i - 1;
-        while (j2 >= 0 && seg_first[ud[j2]] > key) {
-            ud[j2 + 1] = ud[j2];
-            --j2;
+    // U in ascending first body (keys are distinct: disjoint leaves)
+    __threadfence_block();
+    __syncthreads();
+    // bodies in the U leaves (the P2P source count of this leaf; order-free)
+    int ns = 0;
+    for (int i = lane; i < totu; i += 64) ns += seg_count[ud[i]];
+    ns = wave_total(ns);
+    if (lane == 0) usrc[li] = ns;
+    if (totu <= kUWSortCap) {
+        for (int i = lane; i < totu; i += 64) ubuf[i] = ud[i];
+        __syncthreads();
+        for (int i = lane; i < totu; i += 64) {
+            const int v = ubuf[i], key = seg_first[v];
+            int r = 0;
+            for (int k2 = 0; k2 < totu; ++k2) r += seg_first[ubuf[k2]] < key;
+            ud[r] = v;
         }
-        ud[j2 + 1] = v;
+    } else if (lane == 0) {
+        for (int i = 1; i < totu; ++i) {
+            const int v = ud[i], key = seg_first[v];
+            int j2 = i - 1;
+            while (j2 >= 0 && seg_first[ud[j2]] > key) {
+                ud[j2 + 1] = ud[j2];
+                --j2;
+            }
+            ud[j2 + 1] = v;
+        }
+    }
+}
+
+// Totals for the statistics: colleague and V entries, the largest leaf and
+// sum_leaf n_leaf * usrc_leaf (integer atomics, so the order does not matter).
+__global__ void __launch_bounds__(kT) k_list_totals(int nb, int nleaves, const int* __restrict__ ncoll,
+                                                   const int* __restrict__ nv, const int* __restrict__ leaf_ids,
+                                                   const int* __restrict__ seg_count, const int* __restrict__ usrc,
+                                                   unsigned long long* __restrict__ tot) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    unsigned long long c = 0, v = 0, occ = 0, pairs = 0;
+    if (i < nb) {
+        c = (unsigned long long)ncoll[i];
+        v = (unsigned long long)nv[i];
+    }
+    if (i < nleaves) {
+        const int nt = seg_count[leaf_ids[i]];
+        occ = (unsigned long long)nt;
+        pairs = (unsigned long long)nt * (unsigned long long)usrc[i];
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        c += __shfl_xor(c, m, 64);
+        v += __shfl_xor(v, m, 64);
+        pairs += __shfl_xor(pairs, m, 64);
+        const unsigned long long o = __shfl_xor(occ, m, 64);
+        occ = o > occ ? o : occ;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&tot[0], c);
+        atomicAdd(&tot[1], v);
+        atomicMax(&tot[2], occ);
+        atomicAdd(&tot[3], pairs);
     }
 }
 
@@ -574,21 +722,49 @@ int read_int(const int* d, hipStream_t s) {
 
 }  // namespace
 
-void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int n, int cap,
-                double eps_floor, TreeView& tv, TreeMeta& meta, RootInfo& root) {
-    // ---- root cube
-    double* pv = pool.get<double>("bbox_v", 6 * kBBoxBlocks);
-    int* pi = pool.get<int>("bbox_i", 6 * kBBoxBlocks);
-    int* pbad = pool.get<int>("bbox_bad", kBBoxBlocks);
-    RootInfo* d_root = pool.get<RootInfo>("root", 1);
-    const int nbb = std::min(kBBoxBlocks, blocks(n));
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kT), 0, s, d_pos, d_mass, n, pv, pi, pbad);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kT), 0, s, pv, pi, pbad, nbb, eps_floor, d_root);
-    AFMM_CHECK(hipMemcpyAsync(&root, d_root, sizeof(RootInfo), hipMemcpyDeviceToHost, s));
-    AFMM_CHECK(hipStreamSynchronize(s));
-    if (root.nonfinite) return;
+void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
+                  const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
+                  std::vector<RootInfo>& roots) {
+    const int n = h_sys_off[nsys];
+    // ---- per-system root cubes: chunks of at most kBBoxChunk bodies of one system
+    std::vector<int> hb;  // [cbeg | cend | cfirst(nsys+1) | sys_off(nsys+1)] as one upload
+    std::vector<int> cb, ce, cf(nsys + 1, 0);
+    for (int q = 0; q < nsys; ++q) {
+        cf[q] = (int)cb.size();
+        for (int i = h_sys_off[q]; i < h_sys_off[q + 1]; i += kBBoxChunk) {
+            cb.push_back(i);
+            ce.push_back(std::min(i + kBBoxChunk, h_sys_off[q + 1]));
+        }
+    }
+    cf[nsys] = (int)cb.size();
+    const int nch = (int)cb.size();
+    hb.insert(hb.end(), cb.begin(), cb.end());
+    hb.insert(hb.end(), ce.begin(), ce.end());
+    hb.insert(hb.end(), cf.begin(), cf.end());
+    hb.insert(hb.end(), h_sys_off, h_sys_off + nsys + 1);
+    int* d_hb = pool.get<int>("sys_ints", hb.size());
+    double* d_eps_floor = pool.get<double>("sys_eps_floor", nsys);
+    AFMM_CHECK(hipMemcpyAsync(d_hb, hb.data(), sizeof(int) * hb.size(), hipMemcpyHostToDevice, s));
+    AFMM_CHECK(hipMemcpyAsync(d_eps_floor, h_eps_floor, sizeof(double) * nsys, hipMemcpyHostToDevice, s));
+    const int* d_cbeg = d_hb;
+    const int* d_cend = d_hb + nch;
+    const int* d_cfirst = d_hb + 2 * nch;
+    const int* d_sys_off = d_hb + 2 * nch + nsys + 1;
+    double* pv = pool.get<double>("bbox_v", 6 * (size_t)nch);
+    int* pi = pool.get<int>("bbox_i", 6 * (size_t)nch);
+    int* pbad = pool.get<int>("bbox_bad", nch);
+    RootInfo* d_roots = pool.get<RootInfo>("roots", nsys);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nch), dim3(kT), 0, s, d_pos, d_mass, d_cbeg, d_cend, pv, pi, pbad);
+    hipLaunchKernelGGL(k_bbox_final, dim3(nsys), dim3(kT), 0, s, pv, pi, pbad, nch, d_cfirst, d_eps_floor, d_roots);
+    roots.resize(nsys);
+    AFMM_CHECK(hipMemcpyAsync(roots.data(), d_roots, sizeof(RootInfo) * nsys, hipMemcpyDeviceToHost, s));
+    AFMM_CHECK(hipStreamSynchronize(s));  // also retires the uploads of hb
+    for (const RootInfo& r : roots)
+        if (r.nonfinite) return;
     tv.n = n;
-    tv.tol = root.tol;
+    tv.nsys = nsys;
+    tv.roots = d_roots;
+    tv.tol = roots[0].tol;
 
     // ---- pass 1: sort bodies by octant path
     const int nsb = sort_blocks(n);
@@ -602,10 +778,15 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     const int scan_cap = std::max(n, 512 * nsb) + 1;
     int* scan_tmp = pool.get<int>("scan_tmp", 2 * scan_tmp_ints(scan_cap) + 16);
     sb.scan_tmp = scan_tmp;
-    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kT), 0, s, d_pos, n, d_root, sb.khi[0], sb.klo[0], sb.val[0]);
-    const int L = root.key_levels;
+    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kT), 0, s, d_pos, n, nsys, d_sys_off, d_roots, sb.khi[0],
+                       sb.klo[0], sb.val[0]);
+    int L = 0;
+    for (const RootInfo& r : roots) L = std::max(L, r.key_levels);
     const int ndig = (L + 2) / 3;
-    const int r1 = radix_sort(sb, n, 8 - ndig, ndig, s);
+    // system digits (klo bits 9..26) after the path digits: sorted by (system, path)
+    const int sdig = nsys <= 1 ? 0 : (nsys <= 512 ? 1 : 2);
+    int r1 = radix_sort(sb, n, 8 - ndig, ndig, s);
+    if (sdig > 0) r1 = radix_sort(sb, n, 8, sdig, s, r1);
     // keep pass-1 keys/perm: copy into stable names
     uint64_t* khi = pool.get<uint64_t>("key1_hi", n);
     uint32_t* klo = pool.get<uint32_t>("key1_lo", n);
@@ -625,21 +806,23 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
         tv.child_first = pool.get<int>("child_first", c, s, keep);
         tv.nchild = pool.get<int>("nchild", c, s, keep);
         tv.child_mask = pool.get<unsigned char>("child_mask", c, s, keep);
+        tv.bsys = pool.get<int>("bsys", c, s, keep);
     };
+    cap_boxes = std::max(cap_boxes, (size_t)nsys + 1024);
     box_arrays(cap_boxes, false);
-    hipLaunchKernelGGL(k_init_root, dim3(1), dim3(64), 0, s, n, tv.seg_first, tv.seg_count, tv.level, tv.parent,
-                       tv.octant);
+    hipLaunchKernelGGL(k_init_roots, dim3((nsys + 255) / 256), dim3(256), 0, s, nsys, d_sys_off, tv.seg_first,
+                       tv.seg_count, tv.level, tv.parent, tv.octant, tv.bsys);
     int* d_total = pool.get<int>("total", 4);
     meta.level_off.assign(1, 0);
-    meta.level_off.push_back(1);
+    meta.level_off.push_back(nsys);
     int l = 0;
     for (;; ++l) {
         const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
         int* bnd = pool.get<int>("split_bnd", 9 * (size_t)cnt);
         int* nch = pool.get<int>("split_nch", cnt);
         int* cbase = pool.get<int>("split_base", cnt);
-        hipLaunchKernelGGL(k_split, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, l, cap, eps_floor, d_root,
-                           tv.seg_first, tv.seg_count, khi, klo, bnd, nch);
+        hipLaunchKernelGGL(k_split, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, l, cap, d_eps_floor, d_roots,
+                           tv.bsys, tv.seg_first, tv.seg_count, khi, klo, bnd, nch);
         scan_exclusive(nch, cbase, cnt, d_total, scan_tmp, s);
         const int next = read_int(d_total, s);
         const int next_off = off + cnt;
@@ -649,7 +832,7 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
         }
         hipLaunchKernelGGL(k_emit, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, l, next_off, bnd, nch, cbase,
                            tv.seg_first, tv.seg_count, tv.level, tv.parent, tv.octant, tv.child_first,
-                           tv.nchild, tv.child_mask);
+                           tv.nchild, tv.child_mask, tv.bsys);
         if (next == 0) break;
         meta.level_off.push_back(next_off + next);
     }
@@ -673,8 +856,8 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     hipLaunchKernelGGL(k_preorder, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.seg_first, tv.level, start_scan,
                        minlev, tv.nchild, pre, nch_pre);
     scan_exclusive(nch_pre, base_pre, nb, nullptr, scan_tmp, s);
-    hipLaunchKernelGGL(k_refid, dim3(blocks(nb)), dim3(kT), 0, s, nb, pre, base_pre, tv.child_first, tv.nchild,
-                       tv.refid);
+    hipLaunchKernelGGL(k_refid, dim3(blocks(nb)), dim3(kT), 0, s, nb, pre, base_pre, tv.parent, tv.bsys,
+                       tv.child_first, tv.nchild, tv.refid);
 
     // ---- leaves, pass 2 (input order inside each leaf), gather
     int* flags = pool.get<int>("leaf_flag", nb);
@@ -704,8 +887,8 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     tv.half = pool.get<double>("half", nb);
     tv.invh = pool.get<double>("invh", nb);
     tv.ic = pool.get<int>("ic", 3 * (size_t)nb);
-    hipLaunchKernelGGL(k_geometry, dim3(blocks(nb)), dim3(kT), 0, s, nb, d_root, tv.seg_first, tv.level, khi, klo,
-                       tv.cfp, tv.cdd, tv.half, tv.invh, tv.ic);
+    hipLaunchKernelGGL(k_geometry, dim3(blocks(nb)), dim3(kT), 0, s, nb, d_roots, tv.bsys, tv.seg_first, tv.level,
+                       khi, klo, tv.cfp, tv.cdd, tv.half, tv.invh, tv.ic);
 
     // ---- (level, octant) order; non-leaf boxes per level
     {
@@ -746,7 +929,7 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, overflow, 1, 0);
     for (int lv = 0; lv <= tv.max_level; ++lv) {
         const int off = meta.level_off[lv], cnt = meta.level_off[lv + 1] - off;
-        hipLaunchKernelGGL(k_coll_v, dim3(cnt), dim3(64), 0, s, off, root.tol, tv.parent, tv.child_first,
+        hipLaunchKernelGGL(k_coll_v, dim3(cnt), dim3(64), 0, s, off, d_roots, tv.bsys, tv.parent, tv.child_first,
                            tv.child_mask, tv.cfp, tv.half, tv.ic, tv.coll, tv.ncoll, tv.vl, tv.nv, overflow);
     }
     if (read_int(overflow, s) != 0) throw std::runtime_error("interaction-list slot overflow");
@@ -757,9 +940,10 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     int* wcnt = pool.get<int>("wcnt", nl + 1);
     tv.u_off = pool.get<int>("u_off", nl + 1);
     tv.w_off = pool.get<int>("w_off", nl + 1);
-    hipLaunchKernelGGL(k_uwlist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
-                       tv.child_first, tv.nchild, tv.seg_first, tv.level, tv.cfp, tv.half, tv.coll, tv.ncoll, 0,
-                       ucnt, wcnt, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_uwlist, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
+                       tv.child_first,
+                       tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, 0, ucnt, wcnt, nullptr, nullptr,
+                       nullptr, nullptr, nullptr);
     scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
     scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp + scan_tmp_ints(scan_cap), s);
     {
@@ -772,9 +956,14 @@ void build_tree(Pool& pool, hipStream_t s, const double* d_pos, const double* d_
     }
     tv.u_idx = pool.get<int>("u_idx", tv.n_u);
     tv.w_idx = pool.get<int>("w_idx", tv.n_w);
-    hipLaunchKernelGGL(k_uwlist, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, root.tol, tv.parent,
-                       tv.child_first, tv.nchild, tv.seg_first, tv.level, tv.cfp, tv.half, tv.coll, tv.ncoll, 1,
-                       nullptr, nullptr, tv.u_off, tv.w_off, tv.u_idx, tv.w_idx);
+    hipLaunchKernelGGL(k_uwlist, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
+                       tv.child_first,
+                       tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, 1, nullptr, nullptr, tv.u_off,
+                       tv.w_off, tv.u_idx, tv.w_idx, ucnt);
+    tv.list_totals = pool.get<unsigned long long>("list_totals", 4);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, tv.list_totals, 4);
+    hipLaunchKernelGGL(k_list_totals, dim3(blocks(std::max(nb, nl))), dim3(kT), 0, s, nb, nl, tv.ncoll, tv.nv,
+                       tv.leaf_ids, tv.seg_count, ucnt, tv.list_totals);
 
     int* xcnt = pool.get<int>("xcnt", nb + 1);
     int* xcur = pool.get<int>("xcur", nb + 1);

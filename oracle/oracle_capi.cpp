@@ -160,6 +160,26 @@ int orc_fmm(int64_t n, const double* pos, const double* mass, double G, double e
     return 0;
 }
 
+// VelocityVerlet3D (Integrator3D.hpp:19-40) driven by the oracle FMM:
+// initialize, then nsteps of drift x += v*dt + a*(0.5*dt*dt) (per component,
+// in that order), new forces, kick v += (a + a')*(0.5*dt). pos and vel are
+// updated in place; acc receives the last accelerations.
+int orc_leapfrog(int64_t n, double* pos, double* vel, const double* mass, double G, double eps, int p, int cap,
+                 double dt, int nsteps, double* acc) {
+    if (n < 0 || p < 1 || cap < 1 || nsteps < 0) return -1;
+    std::vector<double> a(3 * (std::size_t)n), an(3 * (std::size_t)n);
+    adaptive_fmm(pos, mass, (std::size_t)n, G, eps, p, cap, a.data(), nullptr);
+    const double half_dt2 = 0.5 * dt * dt, half_dt = 0.5 * dt;
+    for (int k = 0; k < nsteps; ++k) {
+        for (std::size_t i = 0; i < 3 * (std::size_t)n; ++i) pos[i] = pos[i] + (vel[i] * dt + a[i] * half_dt2);
+        adaptive_fmm(pos, mass, (std::size_t)n, G, eps, p, cap, an.data(), nullptr);
+        for (std::size_t i = 0; i < 3 * (std::size_t)n; ++i) vel[i] = vel[i] + (a[i] + an[i]) * half_dt;
+        a.swap(an);
+    }
+    for (std::size_t i = 0; i < 3 * (std::size_t)n; ++i) acc[i] = a[i];
+    return 0;
+}
+
 int orc_direct(int64_t n, const double* pos, const double* mass, double G, double eps, double* acc) {
     if (n < 0) return -1;
     direct(pos, mass, (std::size_t)n, G, eps, acc);
@@ -184,6 +204,56 @@ int orc_direct_targets(int64_t n, const double* pos, const double* mass, double 
         }
         acc[3 * t] = G * ax, acc[3 * t + 1] = G * ay, acc[3 * t + 2] = G * az;
     }
+    return 0;
+}
+
+// Softened potential at a subset of targets (Kernel3D.hpp:39-42 per pair):
+// phi[t] = -G sum_{j != idx[t]} m_j / sqrt(|d|^2 + eps^2). Over all targets,
+// 0.5 sum_i m_i phi_i is Diagnostics3D's pair-sum potential energy.
+int orc_direct_potential_targets(int64_t n, const double* pos, const double* mass, double G, double eps,
+                                 int64_t nt, const int64_t* idx, double* phi) {
+    const double eps2 = eps * eps;
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t i = idx[t];
+        double s = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                         dz = pos[3 * j + 2] - pos[3 * i + 2];
+            s += mass[j] / std::sqrt(dx * dx + dy * dy + dz * dz + eps2);
+        }
+        phi[t] = -G * s;
+    }
+    return 0;
+}
+
+// computeDiagnostics (Diagnostics3D.hpp:20-38): out = {kinetic, potential,
+// total, momentum xyz, angular momentum xyz}; the potential is the i<j pair
+// sum in the reference's loop order.
+int orc_diagnostics(int64_t n, const double* pos, const double* vel, const double* mass, double G, double eps,
+                    double* out) {
+    double ke = 0, pe = 0, P[3] = {0, 0, 0}, Lm[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+        const double* v = vel + 3 * i;
+        const double* r = pos + 3 * i;
+        const double m = mass[i];
+        ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        P[0] += m * v[0], P[1] += m * v[1], P[2] += m * v[2];
+        Lm[0] += m * (r[1] * v[2] - r[2] * v[1]);
+        Lm[1] += m * (r[2] * v[0] - r[0] * v[2]);
+        Lm[2] += m * (r[0] * v[1] - r[1] * v[0]);
+    }
+    const double eps2 = eps * eps;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = i + 1; j < n; ++j) {
+            const double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                         dz = pos[3 * j + 2] - pos[3 * i + 2];
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            pe += -G * mass[i] * mass[j] / std::sqrt(r2);
+        }
+    out[0] = ke, out[1] = pe, out[2] = ke + pe;
+    out[3] = P[0], out[4] = P[1], out[5] = P[2];
+    out[6] = Lm[0], out[7] = Lm[1], out[8] = Lm[2];
     return 0;
 }
 

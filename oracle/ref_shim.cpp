@@ -3,7 +3,7 @@
 // reference's own header-only CPU path, compiled by oracle/Makefile with
 // -I <reference>/src3d into oracle/_ref/libref.so. Nothing of the reference is
 // copied: this file only calls AdaptiveOctree, AdaptiveFMM3D, DirectSolver3D,
-// makePlummerSphere and the fmm:: operators through their public interface.
+// makePlummerSphere, computeDiagnostics, VelocityVerlet3D and the fmm:: operators through their public interface.
 // Exports mirror oracle_capi.cpp with the ref_ prefix.
 // ============================================================================
 #include <cstdint>
@@ -11,9 +11,11 @@
 
 #include "AdaptiveFMM3D.hpp"
 #include "AdaptiveOctree.hpp"
+#include "Diagnostics3D.hpp"
 #include "DirectSolver3D.hpp"
 #include "FmmExpansions3D.hpp"
 #include "InitialConditions3D.hpp"
+#include "Integrator3D.hpp"
 
 #include "tree_export.hpp"
 
@@ -65,8 +67,35 @@ int ref_fmm(int64_t n, const double* pos, const double* mass, double G, double e
     return 0;
 }
 
+int ref_leapfrog(int64_t n, double* pos, double* vel, const double* mass, double G, double eps, int p, int cap,
+                 double dt, int nsteps, double* acc) {
+    Bodies3D b = toBodies(n, pos, mass);
+    for (int64_t i = 0; i < n; ++i) b.vel[i] = Vec3{vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+    AdaptiveFMM3D solver(KernelParams3D{G, eps}, p, cap);
+    VelocityVerlet3D integ(solver);
+    integ.initialize(b);
+    for (int k = 0; k < nsteps; ++k) integ.step(b, dt);
+    for (int64_t i = 0; i < n; ++i) {
+        pos[3 * i] = b.pos[i].x, pos[3 * i + 1] = b.pos[i].y, pos[3 * i + 2] = b.pos[i].z;
+        vel[3 * i] = b.vel[i].x, vel[3 * i + 1] = b.vel[i].y, vel[3 * i + 2] = b.vel[i].z;
+    }
+    fromAcc(integ.accelerations(), acc);
+    return 0;
+}
+
 int ref_direct(int64_t n, const double* pos, const double* mass, double G, double eps, double* acc) {
     fromAcc(DirectSolver3D(KernelParams3D{G, eps}).computeAccelerations(toBodies(n, pos, mass)), acc);
+    return 0;
+}
+
+int ref_diagnostics(int64_t n, const double* pos, const double* vel, const double* mass, double G, double eps,
+                    double* out) {
+    Bodies3D b = toBodies(n, pos, mass);
+    for (int64_t i = 0; i < n; ++i) b.vel[i] = Vec3{vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+    const Diagnostics3D d = computeDiagnostics(b, KernelParams3D{G, eps});
+    const double v[9] = {d.kinetic, d.potential, d.total, d.momentum.x, d.momentum.y, d.momentum.z,
+                         d.angular.x, d.angular.y, d.angular.z};
+    for (int k = 0; k < 9; ++k) out[k] = v[k];
     return 0;
 }
 

@@ -87,6 +87,9 @@ class CpuFmm:
         for op in ("local_eval", "multipole_eval"):
             f(op).argtypes = [_dp, C.c_int, _dp, _dp, _dp, _dp]
         f("plummer").argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint, _dp, _dp, _dp]
+        f("diagnostics").argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, _dp]
+        f("leapfrog").argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                  C.c_double, C.c_int, _dp]
         del L
 
     def _f(self, name):
@@ -108,6 +111,27 @@ class CpuFmm:
         acc = np.empty_like(pos)
         assert self._f("direct")(len(mass), ptr(pos), ptr(mass), G, eps, ptr(acc)) == 0
         return acc
+
+    def diagnostics(self, pos, vel, mass, G=1.0, eps=0.0):
+        """computeDiagnostics (Diagnostics3D.hpp:20-38): dict of kinetic,
+        potential, total, momentum (3), angular (3)."""
+        pos = np.ascontiguousarray(pos, np.float64)
+        vel = np.ascontiguousarray(vel, np.float64)
+        mass = np.ascontiguousarray(mass, np.float64)
+        out = np.empty(9)
+        assert self._f("diagnostics")(len(mass), ptr(pos), ptr(vel), ptr(mass), G, eps, ptr(out)) == 0
+        return {"kinetic": out[0], "potential": out[1], "total": out[2], "momentum": out[3:6],
+                "angular": out[6:9]}
+
+    def leapfrog(self, pos, vel, mass, G, eps, p, cap, dt, nsteps):
+        """VelocityVerlet3D over the CPU FMM: (pos, vel, acc) after nsteps."""
+        pos = np.array(pos, np.float64, order="C")
+        vel = np.array(vel, np.float64, order="C")
+        mass = np.ascontiguousarray(mass, np.float64)
+        acc = np.empty_like(pos)
+        assert self._f("leapfrog")(len(mass), ptr(pos), ptr(vel), ptr(mass), G, eps, p, cap, dt, nsteps,
+                                   ptr(acc)) == 0
+        return pos, vel, acc
 
     def tree(self, pos, mass, cap=64, eps_floor=0.0):
         pos = np.ascontiguousarray(pos, np.float64)
@@ -209,6 +233,20 @@ def operator_convergence(dir_first):
     out = np.empty(40)
     lib.orc_operator_convergence(int(dir_first), ptr(out))
     return out.reshape(4, 5, 2)
+
+
+def direct_potential_targets(pos, mass, idx, G=1.0, eps=0.0):
+    """Oracle softened potential (Kernel3D.hpp:39-42) at bodies idx."""
+    lib = oracle().lib
+    lib.orc_direct_potential_targets.argtypes = [C.c_int64, _dp, _dp, C.c_double, C.c_double, C.c_int64,
+                                                 C.POINTER(C.c_int64), _dp]
+    pos = np.ascontiguousarray(pos, np.float64)
+    mass = np.ascontiguousarray(mass, np.float64)
+    idx = np.ascontiguousarray(idx, np.int64)
+    phi = np.empty(len(idx))
+    lib.orc_direct_potential_targets(len(mass), ptr(pos), ptr(mass), G, eps, len(idx),
+                                     idx.ctypes.data_as(C.POINTER(C.c_int64)), ptr(phi))
+    return phi
 
 
 def rel_err(ref, test):

@@ -1,0 +1,96 @@
+"""GPU checks of batched evaluation (afmm_eval_batch): many independent
+systems in one forest, one launch per FMM phase. Every system's result must
+be bit-identical to evaluating it alone (its tree, lists and arithmetic are
+the same; MFMA tiles and P2P waves never mix a target's sums with another
+system's), in both precisions, with per-system G and eps."""
+import numpy as np
+import pytest
+
+import cabi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def afmm():
+    return cabi.product()
+
+
+def systems(seed, count, nmin=1, nmax=6000):
+    """Uneven systems: Plummer spheres and offset uniform cubes, with a few
+    tiny ones (1 and 2 bodies, and one at or below the leaf capacity)."""
+    orc = cabi.oracle()
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        n = int(np.exp(rng.uniform(np.log(max(nmin, 2)), np.log(nmax))))
+        if q == 1:
+            n = 1
+        elif q == 2:
+            n = 2
+        elif q == 3:
+            n = 40
+        if q % 2 == 0 and n > 2:
+            pos, _, m = orc.plummer(n, seed=int(rng.integers(1 << 30)))
+        else:
+            pos = rng.uniform(-1, 1, (n, 3)) * rng.uniform(0.1, 10) + rng.uniform(-50, 50, 3)
+            m = rng.uniform(0.5, 1.5, n) / n
+        out.append((np.ascontiguousarray(pos), np.ascontiguousarray(m)))
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_equals_separate_runs(afmm, prec):
+    precision = afmm.PRECISION_FP64 if prec == "fp64" else afmm.PRECISION_MIXED
+    sys_ = systems(7, 24)
+    rng = np.random.default_rng(3)
+    G = rng.uniform(0.5, 2.0, len(sys_))
+    eps = rng.choice([0.0, 0.005, 0.01, 0.05], len(sys_))
+    off = np.concatenate([[0], np.cumsum([len(m) for _, m in sys_])])
+    pos = np.concatenate([p for p, _ in sys_])
+    mass = np.concatenate([m for _, m in sys_])
+    print(f"batch {prec}: {len(sys_)} systems, {off[-1]} bodies", flush=True)
+    s = afmm.AdaptiveFMM3D(p=6, threshold=32, precision=precision)
+    acc, phi = s.computeBatch(pos, mass, off, G, eps, potential=True)
+    st = s.stats()
+    for q, (p, m) in enumerate(sys_):
+        one = afmm.AdaptiveFMM3D(G=G[q], eps=eps[q], p=6, threshold=32, precision=precision)
+        a1, f1 = one.computeAccelerationsAndPotential(p, m)
+        assert np.array_equal(acc[off[q]:off[q + 1]], a1), q
+        assert np.array_equal(phi[off[q]:off[q + 1]], f1), q
+    assert st["n_bodies"] == off[-1]
+
+
+def test_batch_trees_match_oracle_and_stats(afmm):
+    """Summed box/leaf counts of the forest equal those of the oracle's
+    per-system trees, and each system's forces match the oracle to 1e-12."""
+    sys_ = systems(11, 10, nmax=3000)
+    off = np.concatenate([[0], np.cumsum([len(m) for _, m in sys_])])
+    pos = np.concatenate([p for p, _ in sys_])
+    mass = np.concatenate([m for _, m in sys_])
+    s = afmm.AdaptiveFMM3D(G=1.0, eps=0.01, p=5, threshold=16)
+    acc = s.computeBatch(pos, mass, off)
+    st = s.stats()
+    orc = cabi.oracle()
+    nb = nl = 0
+    for q, (p, m) in enumerate(sys_):
+        t = orc.tree(p, m, 16, 0.01)
+        nb += len(t["parent"])
+        nl += len(t["leaves"])
+        if len(m) > 1:
+            ref, _ = orc.fmm(p, m, 1.0, 0.01, 5, 16)
+            mx, _ = cabi.rel_err(ref, acc[off[q]:off[q + 1]])
+            assert mx <= 1e-12, (q, mx)
+    assert (st["n_boxes"], st["n_leaves"]) == (nb, nl)
+
+
+def test_batch_argument_errors(afmm):
+    s = afmm.AdaptiveFMM3D(p=4)
+    pos = np.zeros((4, 3))
+    m = np.ones(4)
+    for off in ([1, 4], [0, 2, 2, 4], [0, 3, 2, 4]):
+        with pytest.raises(afmm.AfmmError) as e:
+            s.computeBatch(pos, m, off)
+        assert e.value.code == -1
+    with pytest.raises(afmm.AfmmError):
+        s.computeBatch(pos, m, [0, 2, 4], eps=[0.1, -1.0])

@@ -1,0 +1,119 @@
+"""GPU checks of the FP64 direct yardstick and of the mixed-precision form.
+
+  * afmm_direct (DirectSolver3D.hpp:15-34 on a target subsample, plus the
+    softened potential of Kernel3D.hpp:39-42) equals the oracle's direct sums
+    to 1e-12 relative;
+  * AFMM_PRECISION_MIXED builds the same tree as FP64 (the tree never depends
+    on precision) and its accelerations differ from the FP64 GPU result by
+    FP32 rounding only: relative RMS <= 1e-6 and, with the reference's relErr
+    floor (adaptive_gpu_harness.cu:36-46), max <= 1e-4;
+  * against FP64 direct summation on a target subsample its error stays
+    within 1e-6 (relative RMS) of the FP64 form's own truncation error;
+  * with one leaf the FMM is P2P alone, so mixed vs direct isolates the
+    FP32 pair arithmetic: relative RMS <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+import cabi
+
+pytestmark = pytest.mark.gpu
+
+MIX_VS_FP64_RMS = 1e-6   # relative RMS, mixed vs FP64 GPU on the same tree
+MIX_VS_FP64_MAX = 1e-4   # relErr max (floor 1e-6 max|a|)
+MIX_EXTRA_RMS = 1e-6     # allowed growth of relative RMS vs FP64 direct
+
+
+@pytest.fixture(scope="module")
+def afmm():
+    return cabi.product()
+
+
+def test_direct_matches_oracle(afmm):
+    orc = cabi.oracle()
+    pos, _, m = orc.plummer(20000, seed=5)
+    idx = np.random.default_rng(1).choice(len(m), 700, replace=False)
+    for eps in (0.0, 0.01):
+        print(f"direct n=20000 eps={eps}", flush=True)
+        s = afmm.AdaptiveFMM3D(G=1.7, eps=eps, p=4)
+        acc, phi = s.direct(pos, m, idx, potential=True)
+        ref = cabi.direct_targets(pos, m, idx, 1.7, eps)
+        rphi = cabi.direct_potential_targets(pos, m, idx, 1.7, eps)
+        mx, _ = cabi.rel_err(ref, acc)
+        assert mx <= 1e-12
+        assert np.abs(phi - rphi).max() <= 1e-12 * np.abs(rphi).max()
+    with pytest.raises(afmm.AfmmError):
+        s.direct(pos, m, [len(m)])
+
+
+@pytest.mark.parametrize("p", [4, 8, 12])
+def test_mixed_vs_fp64_same_tree(afmm, p):
+    orc = cabi.oracle()
+    pos, _, m = orc.plummer(60000, seed=21 + p)
+    print(f"mixed vs fp64 plummer 60000 p={p}", flush=True)
+    f64 = afmm.AdaptiveFMM3D(G=1.0, eps=0.01, p=p, threshold=64)
+    mix = afmm.AdaptiveFMM3D(G=1.0, eps=0.01, p=p, threshold=64, precision=afmm.PRECISION_MIXED)
+    a64 = f64.computeAccelerations(pos, m)
+    amx = mix.computeAccelerations(pos, m)
+    t64, tmx = f64.tree(), mix.tree()
+    for k in ("orig_index", "pfirst", "pcount", "v_idx", "u_idx", "w_idx", "x_idx"):
+        assert np.array_equal(t64[k], tmx[k]), k
+    mx, rms = cabi.rel_err(a64, amx)
+    print(f"p={p} mixed vs fp64: max={mx:.3e} rms={rms:.3e}", flush=True)
+    assert rms <= MIX_VS_FP64_RMS and mx <= MIX_VS_FP64_MAX
+    # both against FP64 direct on a subsample
+    idx = np.random.default_rng(p).choice(len(m), 1024, replace=False)
+    ref = f64.direct(pos, m, idx)
+    _, r64 = cabi.rel_err(ref, a64[idx])
+    _, rmx = cabi.rel_err(ref, amx[idx])
+    print(f"p={p} vs direct: fp64 rms={r64:.3e} mixed rms={rmx:.3e}", flush=True)
+    assert rmx <= r64 + MIX_EXTRA_RMS
+    # deterministic
+    assert np.array_equal(amx, mix.computeAccelerations(pos, m))
+
+
+def test_mixed_single_leaf_is_fp32_p2p(afmm):
+    pos, m = cabi.uniform_cube(60, 7)
+    for eps in (0.0, 0.05):
+        mix = afmm.AdaptiveFMM3D(G=1.0, eps=eps, p=4, threshold=64, precision=afmm.PRECISION_MIXED)
+        acc = mix.computeAccelerations(pos, m)
+        ref = cabi.oracle().direct(pos, m, 1.0, eps)
+        _, rms = cabi.rel_err(ref, acc)
+        print(f"single leaf eps={eps}: rms={rms:.3e}", flush=True)
+        assert rms <= 1e-6
+
+
+@pytest.mark.parametrize("p,tol", [(4, 2e-4), (8, 2e-6), (12, 5e-8)])
+def test_potential_vs_direct(afmm, p, tol):
+    """FMM potential (L2P + M2P + P2P) against the FP64 direct potential at
+    eps = 0, where the FMM's only error is truncation (with eps > 0 the far
+    field is unsoftened, as for the reference's accelerations, which biases
+    a softened comparison by ~eps^2/r^2): relative RMS and the energy
+    0.5 sum m phi against the oracle's Diagnostics3D pair sum. The
+    accelerations are unchanged by asking for the potential (bit-identical).
+    A softened run checks the near field: it equals the unsoftened one
+    except through the U-list pairs."""
+    orc = cabi.oracle()
+    pos, vel, m = orc.plummer(20000, seed=30 + p)
+    print(f"potential plummer 20000 p={p}", flush=True)
+    s = afmm.AdaptiveFMM3D(G=1.0, eps=0.0, p=p, threshold=32)
+    acc, phi = s.computeAccelerationsAndPotential(pos, m)
+    assert np.array_equal(acc, s.computeAccelerations(pos, m))
+    idx = np.random.default_rng(p).choice(len(m), 1500, replace=False)
+    ref = cabi.direct_potential_targets(pos, m, idx, 1.0, 0.0)
+    rms = np.sqrt(np.sum((phi[idx] - ref) ** 2) / np.sum(ref ** 2))
+    pe = 0.5 * np.dot(m, phi)
+    pe_ref = orc.diagnostics(pos, vel, m, 1.0, 0.0)["potential"]
+    print(f"p={p}: phi rms={rms:.3e} PE rel={abs(pe - pe_ref) / abs(pe_ref):.3e}", flush=True)
+    assert rms <= tol
+    assert abs(pe - pe_ref) <= tol * abs(pe_ref)
+
+
+def test_potential_single_leaf_is_direct(afmm):
+    pos, m = cabi.uniform_cube(50, 9)
+    idx = np.arange(50)
+    for prec, tol in ((afmm.PRECISION_FP64, 1e-13), (afmm.PRECISION_MIXED, 1e-6)):
+        s = afmm.AdaptiveFMM3D(G=0.7, eps=0.02, p=4, threshold=64, precision=prec)
+        _, phi = s.computeAccelerationsAndPotential(pos, m)
+        ref = cabi.direct_potential_targets(pos, m, idx, 0.7, 0.02)
+        assert np.abs(phi - ref).max() <= tol * np.abs(ref).max()
