@@ -26,40 +26,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;  // every lane holds the total (fixed butterfly order)
 }
 
-// A concatenated stream of the bodies of a list of boxes (their sorted-body
-// ranges), read 64 bodies per step: lane l of step k gets body 64k + l.
-struct BodyStream {
-    const int* list;  // box ids
-    int n;            // list length
-    const int *seg_first, *seg_count;
-    int e = 0, off = 0;  // current position: list entry e, offset off
-    __device__ bool done() const { return e >= n; }
-    // body index of stream position (current + lane), or -1 past the end
-    __device__ int body_at(int lane) const {
-        int ee = e, oo = off + lane;
-        while (ee < n) {
-            const int c = list[ee];
-            const int cnt = seg_count[c];
-            if (oo < cnt) return seg_first[c] + oo;
-            oo -= cnt;
-            ++ee;
-        }
-        return -1;
-    }
-    __device__ void advance(int k) {
-        while (e < n) {
-            const int cnt = seg_count[list[e]];
-            if (off + k < cnt) {
-                off += k;
-                return;
-            }
-            k -= cnt - off;
-            off = 0;
-            ++e;
-        }
-    }
-};
-
 // ---- P2M and P2L: per-body coefficient accumulation -------------------------------
 // One wave per target box. Lanes take the source bodies 64 at a time from the
 // packed stream; each lane accumulates q conj(R(d)) (P2M, FmmExpansions3D.hpp:
@@ -72,25 +38,89 @@ struct BodyStream {
 template <int S>
 constexpr int coef_window() { return S <= 96 ? S : (S + 1) / 2; }
 
+// Bodies of a list of boxes staged through LDS in chunks of kStreamChunk list
+// entries: their (first body, count) and the running body offset, so that
+// stream position q maps to its body by a binary search in LDS rather than by
+// walking the list in global memory.
+constexpr int kStreamChunk = 256;
+struct StreamLds {
+    int first[kStreamChunk];
+    int pre[kStreamChunk + 1];
+};
+
+// Loads entries [e0, e0 + m) of `list` (m <= kStreamChunk) and returns the
+// number of bodies they hold. All 64 lanes of the (single-wave) block call it.
+__device__ __forceinline__ int stage_chunk(StreamLds& sl, const int* list, int e0, int m, const int* seg_first,
+                                           const int* seg_count, int lane) {
+    int cnt[kStreamChunk / 64];
+    int run = 0;
+#pragma unroll
+    for (int r = 0; r < kStreamChunk / 64; ++r) {
+        const int e = lane * (kStreamChunk / 64) + r;  // lane owns 4 consecutive entries
+        int c = 0;
+        if (e < m) {
+            const int box = list[e0 + e];
+            sl.first[e] = seg_first[box];
+            c = seg_count[box];
+        }
+        cnt[r] = run;
+        run += c;
+    }
+    // exclusive scan of the per-lane totals
+    int incl = run;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    const int base = incl - run;
+#pragma unroll
+    for (int r = 0; r < kStreamChunk / 64; ++r) {
+        const int e = lane * (kStreamChunk / 64) + r;
+        if (e < kStreamChunk) sl.pre[e] = base + cnt[r];
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (lane == 0) sl.pre[kStreamChunk] = total;
+    __syncthreads();
+    return total;
+}
+
+// Body of stream position q (< total) of the staged chunk with m entries.
+__device__ __forceinline__ int stream_body(const StreamLds& sl, int m, int q) {
+    int lo = 0, hi = m - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sl.pre[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return sl.first[lo] + (q - sl.pre[lo]);
+}
+
 template <int P, int KIND, int C0, int CW>
-__device__ __forceinline__ void accumulate_window(const TreeView& t, BodyStream st, const double* ch, double ih,
-                                                  int lane, double* __restrict__ out, size_t out_base) {
+__device__ __forceinline__ void accumulate_window(const TreeView& t, const int* list, int nlist, const double* ch,
+                                                  double ih, int lane, StreamLds& sl, double* __restrict__ out,
+                                                  size_t out_base) {
     constexpr int S = nterms(P);
     double acc[CW];
 #pragma unroll
     for (int k = 0; k < CW; ++k) acc[k] = 0.0;
-    while (!st.done()) {
-        const int i = st.body_at(lane);
-        st.advance(64);
-        if (i >= 0) {
-            const double dx = offset_from(t.x[i], Centre{ch[0], ch[1]}) * ih;
-            const double dy = offset_from(t.y[i], Centre{ch[2], ch[3]}) * ih;
-            const double dz = offset_from(t.z[i], Centre{ch[4], ch[5]}) * ih;
-            auto add = [&](int k, double v) {
-                if (k >= C0 && k < C0 + CW) acc[k - C0] += v;
-            };
-            if (KIND == 0) p2m_point<P>(t.m[i], dx, dy, dz, add);
-            else p2l_point<P>(t.m[i], dx, dy, dz, add);
+    for (int e0 = 0; e0 < nlist; e0 += kStreamChunk) {
+        const int m = min(kStreamChunk, nlist - e0);
+        __syncthreads();  // previous chunk's readers are done
+        const int total = stage_chunk(sl, list, e0, m, t.seg_first, t.seg_count, lane);
+        for (int q0 = 0; q0 < total; q0 += 64) {
+            const int q = q0 + lane;
+            if (q < total) {
+                const int i = stream_body(sl, m, q);
+                const double dx = offset_from(t.x[i], Centre{ch[0], ch[1]}) * ih;
+                const double dy = offset_from(t.y[i], Centre{ch[2], ch[3]}) * ih;
+                const double dz = offset_from(t.z[i], Centre{ch[4], ch[5]}) * ih;
+                auto add = [&](int k, double v) {
+                    if (k >= C0 && k < C0 + CW) acc[k - C0] += v;
+                };
+                if (KIND == 0) p2m_point<P>(t.m[i], dx, dy, dz, add);
+                else p2l_point<P>(t.m[i], dx, dy, dz, add);
+            }
         }
     }
 #pragma unroll
@@ -107,34 +137,35 @@ template <int P, int KIND>
 __global__ void __launch_bounds__(64) k_accumulate(TreeView t, double* __restrict__ out) {
     constexpr int S = nterms(P);
     constexpr int CW = coef_window<S>();
+    __shared__ StreamLds sl;
     const int lane = threadIdx.x;
-    int b;
-    BodyStream st;
-    st.seg_first = t.seg_first;
-    st.seg_count = t.seg_count;
+    int b, nlist;
+    const int* list;
     if (KIND == 0) {
         b = t.leaf_ids[blockIdx.x];
-        st.list = t.leaf_ids + blockIdx.x;
-        st.n = 1;
+        list = t.leaf_ids + blockIdx.x;
+        nlist = 1;
     } else {
         b = blockIdx.x;
         const int xa = t.x_off[b];
-        st.list = t.x_idx + xa;
-        st.n = t.x_off[b + 1] - xa;
-        if (st.n == 0) return;
+        list = t.x_idx + xa;
+        nlist = t.x_off[b + 1] - xa;
+        if (nlist == 0) return;
     }
     const double ch[6] = {t.cdd[6 * (size_t)b], t.cdd[6 * (size_t)b + 1], t.cdd[6 * (size_t)b + 2],
                           t.cdd[6 * (size_t)b + 3], t.cdd[6 * (size_t)b + 4], t.cdd[6 * (size_t)b + 5]};
     const double ih = t.invh[b];
-    accumulate_window<P, KIND, 0, CW>(t, st, ch, ih, lane, out, (size_t)b * S);
-    if constexpr (CW < S) accumulate_window<P, KIND, CW, S - CW>(t, st, ch, ih, lane, out, (size_t)b * S);
+    accumulate_window<P, KIND, 0, CW>(t, list, nlist, ch, ih, lane, sl, out, (size_t)b * S);
+    if constexpr (CW < S) accumulate_window<P, KIND, CW, S - CW>(t, list, nlist, ch, ih, lane, sl, out, (size_t)b * S);
 }
 
 // ---- M2P over W (AdaptiveFMM3D.hpp:95, FmmExpansions3D.hpp:180-200) --------------
-// One wave per target leaf, lanes split into groups as in P2P: group g handles
-// W entries g, g+G, ... with their multipoles staged in LDS; partial fields are
-// reduced across groups in a fixed order and added to far[].
-template <int P>
+// One wave per target leaf. The wave is split into G = floor(64/T) lane groups
+// of T = max(n_leaf, 8) lanes (T = 64 above 64 bodies); group g handles W
+// entries g, g+G, ... with their multipoles staged in LDS. Partial fields are
+// reduced across groups in a fixed order and added to far[] (and, with POT,
+// the potential to fpot[]).
+template <int P, bool POT>
 __global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict__ M, double* __restrict__ far,
                                            double* __restrict__ fpot) {
     constexpr int S = nterms(P);
@@ -146,24 +177,23 @@ __global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict
     const int b = t.leaf_ids[li];
     const int f = t.seg_first[b], nt = t.seg_count[b];
     const int lane = threadIdx.x;
-    int T = 64;
-    while (T > 8 && T / 2 >= nt) T >>= 1;
+    const int T = nt > 64 ? 64 : (nt < 8 ? 8 : nt);
     const int Gs = 64 / T;
     const int g = lane / T, tl = lane % T;
     for (int tb = 0; tb < nt; tb += T) {
         const int i = f + tb + tl;
-        const bool act = tb + tl < nt;
+        const bool act = g < Gs && tb + tl < nt;
         const double xi = act ? t.x[i] : 0.0, yi = act ? t.y[i] : 0.0, zi = act ? t.z[i] : 0.0;
         double fx = 0.0, fy = 0.0, fz = 0.0, fp = 0.0;
         for (int r = 0; r < nw; r += Gs) {
             __syncthreads();
-            for (int q = 0; q < Gs; ++q) {
-                if (r + q >= nw) break;
-                const int w = t.w_idx[wa + r + q];
-                for (int k = lane; k < S; k += 64) mw[q][k] = M[(size_t)w * S + k];
+            const int nq = min(Gs, nw - r);
+            for (int e = lane; e < nq * S; e += 64) {
+                const int q = e / S, c = e - q * S;
+                mw[q][c] = M[(size_t)t.w_idx[wa + r + q] * S + c];
             }
             __syncthreads();
-            if (act && r + g < nw) {
+            if (act && g < nq) {
                 const int w = t.w_idx[wa + r + g];
                 const double* c = t.cdd + 6 * (size_t)w;
                 const double ih = t.invh[w];
@@ -177,7 +207,7 @@ __global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict
                 fx += gx * s2;
                 fy += gy * s2;
                 fz += gz * s2;
-                if (fpot) fp += multipole_pot<P>([&](int k) { return mg[k]; }, dx, dy, dz) * ih;
+                if constexpr (POT) fp += multipole_pot<P>([&](int k) { return mg[k]; }, dx, dy, dz) * ih;
             }
         }
         if (Gs > 1) {
@@ -199,7 +229,7 @@ __global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict
             far[3 * (size_t)i] += fx;
             far[3 * (size_t)i + 1] += fy;
             far[3 * (size_t)i + 2] += fz;
-            if (fpot) fpot[i] += fp;
+            if constexpr (POT) fpot[i] += fp;
         }
     }
 }
@@ -459,75 +489,86 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
 }
 
 // ---- M2M / L2L on FP64 matrix cores ----------------------------------------------
-// out[t] (+)= sum_o T[o] in[src(o, t)] over the 8 octant operators, one wave
-// per 16 targets of one level.
+// out[t] (+)= sum_o T[o] in[src(o, t)] over the 8 octant operators for 16
+// targets of one level per workgroup of 8 waves: wave o applies octant o's
+// operator (v_mfma_f64_16x16x4_f64, all A fragments and B values of a K-chunk
+// loaded before its MFMAs), and the 8 partial 16 x S tiles are summed in LDS
+// in octant order (deterministic).
 //   MODE 0 (M2M, FmmExpansions3D.hpp:242-265): targets are the level's
 //     non-leaf boxes, src(o, t) = child of t in octant o; overwrites Mhat[t].
-//   MODE 1 (L2L, :307-330): targets are the level's boxes in octant order (so a
-//     tile mostly shares one operator), src(o, t) = parent if octant(t) == o;
-//     adds into Lhat[t].
+//   MODE 1 (L2L, :307-330): targets are the level's boxes in octant order,
+//     src(o, t) = parent if octant(t) == o (one operator per target); adds
+//     into Lhat[t].
+constexpr int kLevelWaves = 8;
 template <int P, int MODE>
-__global__ void __launch_bounds__(64) k_level_mfma(int n, const int* __restrict__ tgt,
-                                                  const int* __restrict__ child_first,
-                                                  const unsigned char* __restrict__ mask,
-                                                  const int* __restrict__ parent, const int* __restrict__ octant,
-                                                  const double* __restrict__ Tsw, const double* __restrict__ in,
-                                                  double* __restrict__ out) {
+__global__ void __launch_bounds__(kLevelWaves * 64) k_level_mfma(int n, const int* __restrict__ tgt,
+                                                                const int* __restrict__ child_first,
+                                                                const unsigned char* __restrict__ mask,
+                                                                const int* __restrict__ parent,
+                                                                const int* __restrict__ octant,
+                                                                const double* __restrict__ Tsw,
+                                                                const double* __restrict__ in,
+                                                                double* __restrict__ out) {
     constexpr int S = nterms(P);
     constexpr int RB = m2l_rb(S), NKC = m2l_nkc(S), KC = m2l_kc(S), CH = m2l_chunk(S);
-    const int lane = threadIdx.x, j = lane & 15, kq = lane >> 4;
+    __shared__ double red[16 * RB * 16];  // [row][target], rows padded to RB*16
+    const int tid = threadIdx.x, o = tid >> 6, lane = tid & 63, j = lane & 15, kq = lane >> 4;
     const int ti = blockIdx.x * 16 + j;
     const bool valid = ti < n;
     const int t = valid ? tgt[ti] : -1;
-    unsigned mk = 0;
-    int cf = 0, oct = -1, par = -1;
+    int src = -1;
     if (valid) {
         if (MODE == 0) {
-            mk = mask[t];
-            cf = child_first[t];
-        } else {
-            oct = octant[t];
-            par = parent[t];
+            const unsigned mk = mask[t];
+            if ((mk >> o) & 1u) src = child_first[t] + __popc(mk & ((1u << o) - 1u));
+        } else if (octant[t] == o) {
+            src = parent[t];
         }
     }
+    for (int e = tid; e < 16 * RB * 16; e += kLevelWaves * 64) red[e] = 0.0;
     dbl4 acc[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) acc[rb] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int o = 0; o < 8; ++o) {
-        int src = -1;
-        if (MODE == 0) {
-            if ((mk >> o) & 1u) src = cf + __popc(mk & ((1u << o) - 1u));
-        } else {
-            if (oct == o) src = par;
-        }
-        if (__ballot(src >= 0) == 0ull) continue;
+    const bool active = __ballot(src >= 0) != 0ull;
+    if (active) {
         const double* __restrict__ ms = in + (size_t)(src >= 0 ? src : 0) * S;
         for (int c = 0; c < NKC; ++c) {
-            double bv[KC];
+            double bv[KC], av[RB * KC];
+            const double* __restrict__ A = Tsw + ((size_t)o * NKC + c) * CH + lane;
+#pragma unroll
+            for (int e = 0; e < RB * KC; ++e) av[e] = A[e * 64];
 #pragma unroll
             for (int kk = 0; kk < KC; ++kk) {
                 const int k = 4 * (c * KC + kk) + kq;
                 bv[kk] = (src >= 0 && k < S) ? ms[k] : 0.0;
             }
-            const double* __restrict__ A = Tsw + ((size_t)o * NKC + c) * CH + lane;
 #pragma unroll
             for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(rb * KC + kk) * 64], bv[kk], acc[rb], 0, 0, 0);
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb * KC + kk], bv[kk], acc[rb], 0, 0, 0);
         }
     }
-    if (!valid) return;
+    // octant-ordered sum of the partial tiles
+    for (int w = 0; w < kLevelWaves; ++w) {
+        __syncthreads();
+        if (w == o && active) {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+            for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 16 * rb + kq + 4 * r;
-            if (row < S) {
-                if (MODE == 0) out[(size_t)t * S + row] = acc[rb][r];
-                else out[(size_t)t * S + row] += acc[rb][r];
-            }
+                for (int r = 0; r < 4; ++r) red[(16 * rb + kq + 4 * r) * 16 + j] += acc[rb][r];
         }
+    }
+    __syncthreads();
+    // write: thread e covers (row, target) pairs
+    for (int e = tid; e < S * 16; e += kLevelWaves * 64) {
+        const int row = e >> 4, jj = e & 15;
+        const int tj = blockIdx.x * 16 + jj;
+        if (tj >= n) continue;
+        const int tt = tgt[tj];
+        if (MODE == 0) out[(size_t)tt * S + row] = red[row * 16 + jj];
+        else out[(size_t)tt * S + row] += red[row * 16 + jj];
+    }
 }
 
 // ---- L2P: one thread per sorted body (far field from its leaf's local).
@@ -561,8 +602,8 @@ __global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict
 // ---- P2P over U + final combine (AdaptiveFMM3D.hpp:96-106) ------------------
 // One wave per target leaf. The source bodies of U(b) (contiguous ranges in
 // sorted order) are streamed as one concatenated sequence in full 64-body LDS
-// tiles. The wave is split into G = 64/T lane groups (T = smallest power of two
-// >= the leaf's body count, at least 8); group g takes the tile's sources
+// tiles. The wave is split into G = floor(64/T) lane groups of T = the leaf's
+// body count (at least 8, at most 64); group g takes the tile's sources
 // g, g+G, ... so short leaves still keep all lanes busy. Partial sums are
 // reduced across groups in a fixed order (deterministic). The kernel is
 // (|d|^2 + eps^2)^{-3/2} from v_rsq_f64 refined by two Newton steps; the self
@@ -612,13 +653,13 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict
             cnt = t.seg_count[c];
         }
     };
-    int T = 64;
-    while (T > 8 && T / 2 >= nt) T >>= 1;
+    const int T = nt > 64 ? 64 : (nt < 8 ? 8 : nt);
     const int Gs = 64 / T;
     const int g = lane / T, tl = lane % T;
+    const int kfirst = g < Gs ? g : 64;  // lanes past the last whole group idle
     for (int tb = 0; tb < nt; tb += T) {
         const int i = f + tb + tl;
-        const bool act = tb + tl < nt;
+        const bool act = g < Gs && tb + tl < nt;
         const double xi = act ? t.x[i] : 0.0, yi = act ? t.y[i] : 0.0, zi = act ? t.z[i] : 0.0;
         double ax = 0.0, ay = 0.0, az = 0.0, ap = 0.0;
         int e = 0, off = 0;  // stream position: range e, offset off
@@ -665,7 +706,7 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict
             if constexpr (MIX) {
                 const float e2 = (float)eps2;
                 float fx = 0.f, fy = 0.f, fz = 0.f, fp = 0.f;
-                for (int k = g; k < 64; k += Gs) {
+                for (int k = kfirst; k < 64; k += Gs) {
                     const int j = sj[k];
                     if (j < 0) break;
                     const float dx = (float)(sx[k] - xi), dy = (float)(sy[k] - yi), dz = (float)(sz[k] - zi);
@@ -685,7 +726,7 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict
                 az += (double)fz;
                 ap += (double)fp;
             } else {
-                for (int k = g; k < 64; k += Gs) {
+                for (int k = kfirst; k < 64; k += Gs) {
                     const int j = sj[k];
                     if (j < 0) break;
                     const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
@@ -740,7 +781,8 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     for (int l = t.max_level - 1; l >= 0; --l) {
         const int off = meta.nonleaf_off[l], cnt = meta.nonleaf_off[l + 1] - off;
         if (cnt > 0)
-            hipLaunchKernelGGL((k_level_mfma<P, 0>), dim3((cnt + 15) / 16), dim3(64), 0, s, cnt, t.nonleaf_ids + off,
+            hipLaunchKernelGGL((k_level_mfma<P, 0>), dim3((cnt + 15) / 16), dim3(kLevelWaves * 64), 0, s, cnt,
+                               t.nonleaf_ids + off,
                                t.child_first, t.child_mask, t.parent, t.octant, tab.m2m_sw, M, M);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[1], s));
@@ -769,12 +811,16 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (t.n_x > 0) hipLaunchKernelGGL((k_accumulate<P, 1>), dim3(nb), dim3(64), 0, s, t, L);
     for (int l = 1; l <= t.max_level; ++l) {
         const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
-        hipLaunchKernelGGL((k_level_mfma<P, 1>), dim3((cnt + 15) / 16), dim3(64), 0, s, cnt, t.lo_order + off,
+        hipLaunchKernelGGL((k_level_mfma<P, 1>), dim3((cnt + 15) / 16), dim3(kLevelWaves * 64), 0, s, cnt,
+                           t.lo_order + off,
                            t.child_first, t.child_mask, t.parent, t.octant, tab.l2l_sw, L, L);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far, fpot);
-    if (t.n_w > 0) hipLaunchKernelGGL(k_m2p<P>, dim3(t.nleaves), dim3(64), 0, s, t, M, far, fpot);
+    if (t.n_w > 0) {
+        if (fpot) hipLaunchKernelGGL((k_m2p<P, true>), dim3(t.nleaves), dim3(64), 0, s, t, M, far, fpot);
+        else hipLaunchKernelGGL((k_m2p<P, false>), dim3(t.nleaves), dim3(64), 0, s, t, M, far, fpot);
+    }
     const dim3 pg(t.nleaves), pb(64);
     if (mixed && phi)
         hipLaunchKernelGGL((k_p2p<true, true>), pg, pb, 0, s, t, far, acc, fpot, phi);

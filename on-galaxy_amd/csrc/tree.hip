@@ -681,35 +681,89 @@ __global__ void __launch_bounds__(kT) k_list_totals(int nb, int nleaves, const i
     }
 }
 
-// X = transpose of W (:172-173).
-__global__ void __launch_bounds__(kT) k_xcount(int nleaves, const int* w_off, const int* w_idx, int* xcnt) {
-    const int li = blockIdx.x * kT + threadIdx.x;
-    if (li >= nleaves) return;
-    for (int e = w_off[li]; e < w_off[li + 1]; ++e) atomicAdd(&xcnt[w_idx[e]], 1);
-}
-__global__ void __launch_bounds__(kT) k_xfill(int nleaves, const int* leaf_ids, const int* w_off,
-                                             const int* w_idx, const int* x_off, int* xcur, int* x_idx) {
-    const int li = blockIdx.x * kT + threadIdx.x;
-    if (li >= nleaves) return;
-    const int b = leaf_ids[li];
-    for (int e = w_off[li]; e < w_off[li + 1]; ++e) {
-        const int w = w_idx[e];
-        x_idx[x_off[w] + atomicAdd(&xcur[w], 1)] = b;
+// X of every box w (AdaptiveOctree.hpp:172-173), one wave per box, without
+// transposing W: w is in W(c) exactly when c is a leaf colleague of a proper
+// ancestor A of w (c != A), c is adjacent to parent(w) and c is not adjacent
+// to w (descendW enters only boxes adjacent to c, and a box adjacent to
+// parent(w) is adjacent to every ancestor of it). Lanes split the (ancestor,
+// colleague slot) candidates; each list is then ranked by reference id, the
+// reference's order (X filled from `leaves` in ascending id). mode 0 counts,
+// mode 1 writes.
+constexpr int kXSortCap = 1024;
+__global__ void __launch_bounds__(64) k_xlist(int nb, const RootInfo* __restrict__ roots, const int* __restrict__ bsys,
+                                             const int* __restrict__ parent, const int* __restrict__ nchild,
+                                             const int* __restrict__ refid, const double* __restrict__ cfp,
+                                             const double* __restrict__ half, const int* __restrict__ coll,
+                                             const int* __restrict__ ncoll, int mode, int* __restrict__ xcnt,
+                                             const int* __restrict__ xoff, int* __restrict__ xidx) {
+    __shared__ int anc[kUWMaxAnc], cum[kUWMaxAnc + 1];
+    __shared__ int nanc;
+    __shared__ int xbuf[kXSortCap];
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int pw = parent[w];
+    if (pw < 0) {
+        if (mode == 0 && lane == 0) xcnt[w] = 0;
+        return;
     }
-}
-// Each X list in ascending reference id of its leaves.
-__global__ void __launch_bounds__(kT) k_xsort(int nb, const int* x_off, const int* refid, int* x_idx) {
-    const int w = blockIdx.x * kT + threadIdx.x;
-    if (w >= nb) return;
-    const int a = x_off[w], e = x_off[w + 1];
-    for (int i = a + 1; i < e; ++i) {
-        const int v = x_idx[i], key = refid[v];
-        int j = i - 1;
-        while (j >= a && refid[x_idx[j]] > key) {
-            x_idx[j + 1] = x_idx[j];
-            --j;
+    const double tol = roots[bsys[w]].tol;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (lane == 0) {
+        int k = 0, a = pw, run = 0;
+        for (;;) {
+            anc[k] = a;
+            cum[k] = run;
+            run += ncoll[a];
+            ++k;
+            if (parent[a] < 0) break;
+            a = parent[a];
         }
-        x_idx[j + 1] = v;
+        cum[k] = run;
+        nanc = k;
+    }
+    __syncthreads();
+    const int ncand = cum[nanc];
+    int* xd = mode ? xidx + xoff[w] : nullptr;
+    int kx = 0;
+    for (int base = 0; base < ncand; base += 64) {
+        const int q = base + lane;
+        int c = -1;
+        if (q < ncand) {
+            int k = 0;
+            while (cum[k + 1] <= q) ++k;
+            const int A = anc[k];
+            c = coll[kCollSlots * (size_t)A + (q - cum[k])];
+            if (c == A || nchild[c] != 0 || !adjacent(cfp, half, tol, c, pw) || adjacent(cfp, half, tol, c, w)) c = -1;
+        }
+        const uint64_t bal = __ballot(c >= 0);
+        if (mode && c >= 0) xd[kx + __popcll(bal & lt)] = c;
+        kx += __popcll(bal);
+    }
+    if (!mode) {
+        if (lane == 0) xcnt[w] = kx;
+        return;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (kx <= kXSortCap) {
+        for (int i = lane; i < kx; i += 64) xbuf[i] = xd[i];
+        __syncthreads();
+        for (int i = lane; i < kx; i += 64) {
+            const int v = xbuf[i], key = refid[v];
+            int r = 0;
+            for (int k2 = 0; k2 < kx; ++k2) r += refid[xbuf[k2]] < key;
+            xd[r] = v;
+        }
+    } else if (lane == 0) {
+        for (int i = 1; i < kx; ++i) {
+            const int v = xd[i], key = refid[v];
+            int j = i - 1;
+            while (j >= 0 && refid[xd[j]] > key) {
+                xd[j + 1] = xd[j];
+                --j;
+            }
+            xd[j + 1] = v;
+        }
     }
 }
 
@@ -966,17 +1020,14 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
                        tv.leaf_ids, tv.seg_count, ucnt, tv.list_totals);
 
     int* xcnt = pool.get<int>("xcnt", nb + 1);
-    int* xcur = pool.get<int>("xcur", nb + 1);
     tv.x_off = pool.get<int>("x_off", nb + 1);
-    hipLaunchKernelGGL(k_fill, dim3(blocks(nb)), dim3(kT), 0, s, xcnt, nb, 0);
-    hipLaunchKernelGGL(k_fill, dim3(blocks(nb)), dim3(kT), 0, s, xcur, nb, 0);
-    hipLaunchKernelGGL(k_xcount, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.w_off, tv.w_idx, xcnt);
+    hipLaunchKernelGGL(k_xlist, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid, tv.cfp,
+                       tv.half, tv.coll, tv.ncoll, 0, xcnt, nullptr, nullptr);
     scan_exclusive(xcnt, tv.x_off, nb, tv.x_off + nb, scan_tmp, s);
     tv.n_x = read_int(tv.x_off + nb, s);
     tv.x_idx = pool.get<int>("x_idx", tv.n_x);
-    hipLaunchKernelGGL(k_xfill, dim3(blocks(nl)), dim3(kT), 0, s, nl, tv.leaf_ids, tv.w_off, tv.w_idx, tv.x_off,
-                       xcur, tv.x_idx);
-    hipLaunchKernelGGL(k_xsort, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.x_off, tv.refid, tv.x_idx);
+    hipLaunchKernelGGL(k_xlist, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid, tv.cfp,
+                       tv.half, tv.coll, tv.ncoll, 1, nullptr, tv.x_off, tv.x_idx);
     AFMM_CHECK(hipGetLastError());
 }
 

@@ -15,7 +15,8 @@ merger()    the two-galaxy recipe: per galaxy mass 1 in equal-mass bodies,
             pericentre 1. Disk velocities (optional, needs a force solver):
             circular speed from the solver's own radial force, binned in R,
             plus Gaussian dispersions sigma_R = 3.36 Q G Sigma / kappa (Q=1.2),
-            sigma_phi = sigma_R kappa / (2 Omega), sigma_z^2 = pi G Sigma z0.
+            sigma_phi = sigma_R kappa / (2 Omega), sigma_z^2 = pi G Sigma z0,
+            with kappa^2 kept within [Omega^2, 4 Omega^2].
             The reference's nearest recipe (gpu/GalaxyIC.hpp:19-125) uses a
             Plummer bulge and Gaussian thickness instead.
 """
@@ -128,9 +129,15 @@ def _disk_velocities(pos, acc, is_disk, seed, G=1.0, Rd=1.0, z0=0.1, Md=0.8, Q=1
     idx = np.clip(np.searchsorted(edges, R[d]) - 1, 0, 63)
     vc2_bin = np.bincount(idx, vc2_body, 64) / np.maximum(np.bincount(idx, None, 64), 1)
     vc2 = np.interp(R[d], mid, vc2_bin)
-    om2_bin = vc2_bin / mid ** 2
+    # kappa^2 = R dOmega^2/dR + 4 Omega^2 from the binned, lightly smoothed
+    # rotation curve, kept within [Omega^2, 4 Omega^2] (Keplerian to solid
+    # body): with few bodies per bin the finite difference is noisy and could
+    # otherwise reach zero and blow sigma_R up.
+    vc2_s = np.convolve(np.pad(vc2_bin, 1, mode="edge"), np.ones(3) / 3.0, mode="valid")
+    om2_bin = vc2_s / mid ** 2
     dom2 = np.gradient(om2_bin, mid)
-    kap2_bin = np.maximum(mid * dom2 + 4.0 * om2_bin, 1e-12)
+    kap2_bin = np.clip(mid * dom2 + 4.0 * om2_bin, om2_bin, 4.0 * om2_bin)
+    kap2_bin = np.maximum(kap2_bin, 1e-12)
     kap = np.sqrt(np.interp(R[d], mid, kap2_bin))
     vc = np.sqrt(vc2)
     om = vc / R[d]
