@@ -40,8 +40,13 @@ typedef enum {
  * accumulation per target; tree, P2M/M2M/L2L/P2L, L2P/M2P and the pair
  * differences stay FP64. The CUDA twin's reduced precision is its `real`
  * template (gpu/AdaptiveFMM3DGPU.cu:219-334). */
+/* MIXED_BF16: as MIXED, but M2L carries each operand as three bf16 pieces
+ * (hi + mid + lo, ~2^-24 relative) and takes the six products at or above
+ * that order on v_mfma_f32_16x16x32_bf16, FP32 accumulation: the same
+ * compensated split on the matrix type MI355X runs 16x faster than FP32. */
 #define AFMM_PRECISION_FP64 0
 #define AFMM_PRECISION_MIXED 1
+#define AFMM_PRECISION_MIXED_BF16 2
 
 /* Solver parameters: AdaptiveFMM3D(KernelParams3D{G, eps}, p, threshold)
  * (AdaptiveFMM3D.hpp:22-23; Kernel3D.hpp:24-27). The eps also floors the leaf
@@ -52,7 +57,7 @@ typedef struct {
     double eps;
     int32_t p;          /* expansion order, 1..AFMM_MAX_ORDER */
     int32_t leaf_cap;   /* split a box while it holds > leaf_cap bodies (64) */
-    int32_t precision;  /* AFMM_PRECISION_FP64 or AFMM_PRECISION_MIXED */
+    int32_t precision;  /* AFMM_PRECISION_FP64, _MIXED or _MIXED_BF16 */
     int32_t device;     /* HIP device ordinal */
 } afmm_params;
 
@@ -176,7 +181,7 @@ typedef struct afmm_server afmm_server;
 typedef struct {
     int64_t hbm_budget;  /* bytes of device memory jobs may claim (<= 0: 240 GiB) */
     int32_t leaf_cap;    /* leaf capacity of every job (64) */
-    int32_t precision;   /* AFMM_PRECISION_FP64 or AFMM_PRECISION_MIXED */
+    int32_t precision;   /* AFMM_PRECISION_FP64, _MIXED or _MIXED_BF16 */
     int32_t device;      /* HIP device ordinal */
     int32_t max_batch;   /* most jobs in one batched step of an order (<= 0: no limit) */
 } afmm_server_params;

@@ -24,6 +24,7 @@ _bp = C.POINTER(C.c_int8)
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_MIXED_BF16 = 2
 
 
 class AfmmError(RuntimeError):
@@ -163,7 +164,7 @@ class AdaptiveFMM3D:
                             self.lib.afmm_strerror(rc).decode())
 
     def name(self):
-        prec = "mixed" if self.params.precision == PRECISION_MIXED else "FP64"
+        prec = {PRECISION_MIXED: "mixed", PRECISION_MIXED_BF16: "mixed-bf16"}.get(self.params.precision, "FP64")
         return f"AdaptiveFMM3D(MI355X HIP, {prec})"
 
     def computeAccelerations(self, pos, mass):

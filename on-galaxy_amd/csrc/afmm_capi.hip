@@ -31,7 +31,9 @@ int check_params(const afmm_params* p) {
     if (!(std::isfinite(p->G)) || !(std::isfinite(p->eps)) || p->eps < 0) return AFMM_ERR_ARG;
     if (p->p < 1 || p->p > kMaxOrder) return AFMM_ERR_ORDER;
     if (p->leaf_cap < 1) return AFMM_ERR_LEAF_CAP;
-    if (p->precision != AFMM_PRECISION_FP64 && p->precision != AFMM_PRECISION_MIXED) return AFMM_ERR_PRECISION;
+    if (p->precision != AFMM_PRECISION_FP64 && p->precision != AFMM_PRECISION_MIXED &&
+        p->precision != AFMM_PRECISION_MIXED_BF16)
+        return AFMM_ERR_PRECISION;
     return AFMM_OK;
 }
 
@@ -67,8 +69,8 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     tv.sys_G = d_ge;
     tv.sys_eps2 = d_ge + nsys;
     AFMM_CHECK(hipEventRecord(c->ev[1], s));
-    FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw, c->d_m2l_hl};
-    run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision == AFMM_PRECISION_MIXED, acc, phi, c->ev + 2);
+    FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw, c->d_m2l_hl, c->d_m2l_bf};
+    run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2);
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
     c->meta = meta;
@@ -160,7 +162,7 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
         const Tables& T = tables_for_order(prm->p);
         const std::vector<double> m2m = mfma_a_layout(T.m2m, T.S, 8);
         const std::vector<double> l2l = mfma_a_layout(T.l2l, T.S, 8);
-        const std::vector<double> m2l = mfma_a_layout(T.m2l, T.S, kNumOffsets);
+        const std::vector<double> m2l = m2l_a_layout(T.m2l, T.S, kNumOffsets);
         c->d_m2m_sw = c->pool.get<double>("T_m2m_sw", m2m.size());
         c->d_l2l_sw = c->pool.get<double>("T_l2l_sw", l2l.size());
         c->d_m2l_sw = c->pool.get<double>("T_m2l_sw", m2l.size());
@@ -175,6 +177,11 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
             }
             c->d_m2l_hl = c->pool.get<float2>("T_m2l_hl", hl.size());
             AFMM_CHECK(hipMemcpy(c->d_m2l_hl, hl.data(), sizeof(float2) * hl.size(), hipMemcpyHostToDevice));
+        }
+        if (prm->precision == AFMM_PRECISION_MIXED_BF16) {
+            const std::vector<uint16_t> bf = mfma_bf16_layout(T.m2l, T.S, kNumOffsets);
+            c->d_m2l_bf = c->pool.get<uint4>("T_m2l_bf", bf.size() / 8);
+            AFMM_CHECK(hipMemcpy(c->d_m2l_bf, bf.data(), sizeof(uint16_t) * bf.size(), hipMemcpyHostToDevice));
         }
         return AFMM_OK;
     });

@@ -17,7 +17,8 @@ struct afmm_ctx {
     std::string err;
     // tables
     double *d_m2m_sw = nullptr, *d_l2l_sw = nullptr, *d_m2l_sw = nullptr;
-    float2* d_m2l_hl = nullptr;  // split FP32 M2L operators (mixed precision only)
+    float2* d_m2l_hl = nullptr;  // split FP32 M2L operators (AFMM_PRECISION_MIXED only)
+    uint4* d_m2l_bf = nullptr;   // three-piece bf16 M2L operators (AFMM_PRECISION_MIXED_BF16 only)
     // last evaluation
     bool have_tree = false;
     afmm::TreeView tv{};

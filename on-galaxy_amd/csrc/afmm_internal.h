@@ -116,6 +116,11 @@ struct TreeView {
     int* ic;             // [3B] integer coordinates at the box's level
     int* leaf_ids;       // [nleaves] BFS ids of the leaves
     int* lo_order;       // [B] boxes sorted by (level, octant); level l is [level_off[l], level_off[l+1])
+    int* m2l_order;      // [B] boxes sorted by (octant, level): the M2L target order
+    // M2L target tiles over m2l_order, cut inside octant groups: tile k is
+    // ranks [m2l_tiles{256,128}[k], [k+1]), at most 256 (128) targets each
+    int *m2l_tiles256, *m2l_tiles128;
+    int m2l_ntiles256 = 0, m2l_ntiles128 = 0;
     int* nonleaf_ids;    // [B - nleaves] non-leaf boxes in BFS order
     // lists
     int *coll, *ncoll;   // [27B], [B]
@@ -141,6 +146,10 @@ int scan_tmp_ints(int n);
 // Exclusive prefix sum of n int32 values (in place allowed); total to *d_total
 // when non-null.
 void scan_exclusive(const int* d_in, int* d_out, int n, int* d_total, int* d_tmp, hipStream_t s);
+// The same over a count held on the device: rng[1] - rng[0] items (at most
+// nmax); writes rng[1] + total to *d_end. d_tmp needs scan_tmp_ints(nmax).
+void scan_exclusive_range(const int* d_in, int* d_out, const int* d_rng, int nmax, int* d_end, int* d_tmp,
+                          hipStream_t s);
 
 struct SortBufs {
     uint64_t* khi[2];
@@ -177,14 +186,18 @@ struct FmmTables {
     const double *m2m_sw, *l2l_sw;  // device, 8 octant operators in the MFMA A-fragment layout
     const double* m2l_sw;           // device, 343 offset operators in the same layout
     const float2* m2l_hl;           // device, the same as FP32 (hi, lo) pairs, or null
+    const uint4* m2l_bf;            // device, three-piece bf16 layout (mfma_bf16_layout), or null
 };
+// precision of an evaluation (the values of AFMM_PRECISION_*)
+constexpr int kPrecFP64 = 0, kPrecMixed = 1, kPrecMixedBf16 = 2;
 // Runs the FMM passes on a built tree (forest) and writes G*(far + near) to
 // d_acc in input order (AoS n x 3) and, when d_phi is non-null, the potential
 // -G*(far + near) per body, with G and eps of each body's system (tv.sys_G,
-// tv.sys_eps2). mixed: M2L as split-FP32 MFMA and P2P in FP32 with FP64
-// accumulation; everything else FP64.
+// tv.sys_eps2). prec kPrecMixed: M2L as split-FP32 MFMA; kPrecMixedBf16: M2L
+// as three-piece bf16 MFMA; both with P2P in FP32 and FP64 accumulation;
+// everything else FP64.
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
-             const FmmTables& tab, bool mixed, double* d_acc, double* d_phi,
+             const FmmTables& tab, int prec, double* d_acc, double* d_phi,
              hipEvent_t* ev /* 6 or null */);
 
 // ---- integrate.hip -----------------------------------------------------------

@@ -171,11 +171,19 @@ AFMM_HD void local_grad(Coef L, double dx, double dy, double dz, double& gx, dou
 
 // Gradient of sum_{n<=P} M_n^m S_n^m at the scaled point d: the reference's
 // multipoleField (FmmExpansions3D.hpp:180-200) folded onto m >= 0 the same way.
-template <int P, class Coef>
+// With POT, *pot also receives the potential sum_{n<=P} M_n^m S_n^m
+// (multipolePotential, :166-177) from the same harmonic stream.
+template <int P, bool POT = false, class Coef>
 AFMM_HD void multipole_grad(Coef M, double dx, double dy, double dz, double& gx, double& gy,
-                            double& gz) {
-    double pr = 0.0, pi = 0.0, zz = 0.0;
+                            double& gz, double* pot = nullptr) {
+    double pr = 0.0, pi = 0.0, zz = 0.0, ps = 0.0;
     singular_stream<P + 1>(dx, dy, dz, [&](int n2, int a, double re, double im) {
+        if constexpr (POT) {
+            if (n2 <= P) {
+                const double mr = M(rid(n2, a)), mi = a ? M(rid(n2, a) + 1) : 0.0;
+                ps += (a ? 2.0 : 1.0) * (re * mr - im * mi);
+            }
+        }
         if (n2 == 0) return;
         const int n = n2 - 1;
         if (a <= n) {
@@ -199,6 +207,7 @@ AFMM_HD void multipole_grad(Coef M, double dx, double dy, double dz, double& gx,
     gx = pr;
     gy = pi;
     gz = zz;
+    if constexpr (POT) *pot = ps;
 }
 
 // Potentials of the expansions at the scaled point d: sum_{n<=P} L_n^m R_n^m
@@ -264,5 +273,37 @@ AFMM_HD constexpr int m2l_kcmax(int S) { return 60 / m2l_rb(S) > 0 ? 60 / m2l_rb
 AFMM_HD constexpr int m2l_nkc(int S) { return (m2l_kb(S) + m2l_kcmax(S) - 1) / m2l_kcmax(S); }
 AFMM_HD constexpr int m2l_kc(int S) { return (m2l_kb(S) + m2l_nkc(S) - 1) / m2l_nkc(S); }
 AFMM_HD constexpr int m2l_chunk(int S) { return m2l_rb(S) * m2l_kc(S) * 64; }
+
+// M2L operator tiling (FP64 and split-FP32 MFMA). When S leaves 1-4 rows past
+// a multiple of 16 (p = 5, 6, 8, 9, ...), those tail rows run on the VALU
+// instead of a 16-row MFMA block that would be mostly padding: MRB row blocks
+// on MFMA plus TAIL rows. The k steps of 4 split into NKC chunks of KC steps
+// (MRB*KC <= 60, the split with the fewest padded steps); a chunk is MRB*KC
+// A fragments of 64 values followed by the tail rows' TAIL*KC*4 values.
+AFMM_HD constexpr int m2l_mrb(int S) { return (S > 16 && S % 16 != 0 && S % 16 <= 4) ? S / 16 : (S + 15) / 16; }
+AFMM_HD constexpr int m2l_tail(int S) { return S > 16 * m2l_mrb(S) ? S - 16 * m2l_mrb(S) : 0; }
+AFMM_HD constexpr int m2l_mnkc(int S) {
+    const int kb = m2l_kb(S), kcmax = 60 / m2l_mrb(S) > 0 ? 60 / m2l_mrb(S) : 1;
+    const int n0 = (kb + kcmax - 1) / kcmax;
+    int best = n0;
+    for (int n = n0; n <= n0 + 1 && n <= kb; ++n)
+        if (n * ((kb + n - 1) / n) < best * ((kb + best - 1) / best)) best = n;
+    return best;
+}
+AFMM_HD constexpr int m2l_mkc(int S) { return (m2l_kb(S) + m2l_mnkc(S) - 1) / m2l_mnkc(S); }
+AFMM_HD constexpr int m2l_mchunk(int S) { return (m2l_mrb(S) * 64 + m2l_tail(S) * 4) * m2l_mkc(S); }
+
+// M2L operator tiling for v_mfma_f32_16x16x32_bf16 with operands split into
+// three bf16 pieces: RB row blocks of 16, k blocks of 32 grouped into NKC
+// chunks of KC blocks. A fragment set (row block, k block) is 3 pieces x 64
+// lanes x 8 bf16 = 3072 bytes; RB*KC <= 24 keeps two chunk buffers under
+// 150 KB of LDS.
+constexpr int kBfPieces = 3;
+AFMM_HD constexpr int m2l_bf_kb(int S) { return (S + 31) / 32; }
+AFMM_HD constexpr int m2l_bf_kcmax(int S) { return 24 / m2l_rb(S) > 0 ? 24 / m2l_rb(S) : 1; }
+AFMM_HD constexpr int m2l_bf_nkc(int S) { return (m2l_bf_kb(S) + m2l_bf_kcmax(S) - 1) / m2l_bf_kcmax(S); }
+AFMM_HD constexpr int m2l_bf_kc(int S) { return (m2l_bf_kb(S) + m2l_bf_nkc(S) - 1) / m2l_bf_nkc(S); }
+// 16-byte units (8 bf16) per chunk
+AFMM_HD constexpr int m2l_bf_chunk(int S) { return m2l_rb(S) * m2l_bf_kc(S) * kBfPieces * 64; }
 
 }  // namespace afmm

@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <map>
 #include <mutex>
 
@@ -161,6 +162,76 @@ std::vector<double> mfma_a_layout(const std::vector<double>& cm, int S, int nmat
                             out[((size_t)t * NKC + c) * CH + ((size_t)rb * KC + kk) * 64 + lane] =
                                 cm[(size_t)t * S * S + (size_t)col * S + row];
                     }
+    return out;
+}
+
+std::vector<double> m2l_a_layout(const std::vector<double>& cm, int S, int nmat) {
+    const int MRB = m2l_mrb(S), TR = m2l_tail(S), NKC = m2l_mnkc(S), KC = m2l_mkc(S), CH = m2l_mchunk(S);
+    std::vector<double> out((size_t)nmat * NKC * CH, 0.0);
+    auto at = [&](int t, int row, int col) {
+        return (row < S && col < S) ? cm[(size_t)t * S * S + (size_t)col * S + row] : 0.0;
+    };
+    for (int t = 0; t < nmat; ++t)
+        for (int c = 0; c < NKC; ++c) {
+            double* ch = out.data() + ((size_t)t * NKC + c) * CH;
+            for (int rb = 0; rb < MRB; ++rb)
+                for (int kk = 0; kk < KC; ++kk)
+                    for (int lane = 0; lane < 64; ++lane)
+                        ch[((size_t)rb * KC + kk) * 64 + lane] =
+                            at(t, 16 * rb + lane % 16, 4 * (c * KC + kk) + lane / 16);
+            double* tail = ch + (size_t)MRB * KC * 64;
+            for (int tr = 0; tr < TR; ++tr)
+                for (int kk = 0; kk < KC; ++kk)
+                    for (int kq = 0; kq < 4; ++kq)
+                        tail[((size_t)tr * KC + kk) * 4 + kq] = at(t, 16 * MRB + tr, 4 * (c * KC + kk) + kq);
+        }
+    return out;
+}
+
+namespace {
+uint16_t bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+double bf16_value(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+}  // namespace
+
+void bf16_split3(double x, uint16_t out[3]) {
+    double r = x;
+    for (int k = 0; k < kBfPieces; ++k) {
+        out[k] = bf16_bits((float)r);
+        r -= bf16_value(out[k]);
+    }
+}
+
+std::vector<uint16_t> mfma_bf16_layout(const std::vector<double>& cm, int S, int nmat) {
+    const int RB = m2l_rb(S), KB = m2l_bf_kb(S), NKC = m2l_bf_nkc(S), KC = m2l_bf_kc(S);
+    const size_t CHU = (size_t)m2l_bf_chunk(S);
+    std::vector<uint16_t> out((size_t)nmat * NKC * CHU * 8, 0);
+    uint16_t pc[kBfPieces];
+    for (int t = 0; t < nmat; ++t)
+        for (int c = 0; c < NKC; ++c)
+            for (int rb = 0; rb < RB; ++rb)
+                for (int kk = 0; kk < KC; ++kk) {
+                    const int kb = c * KC + kk;
+                    for (int lane = 0; lane < 64; ++lane)
+                        for (int e = 0; e < 8; ++e) {
+                            const int row = 16 * rb + lane % 16, col = 32 * kb + 8 * (lane / 16) + e;
+                            const double v = (row < S && col < S && kb < KB)
+                                                 ? cm[(size_t)t * S * S + (size_t)col * S + row] : 0.0;
+                            bf16_split3(v, pc);
+                            for (int q = 0; q < kBfPieces; ++q)
+                                out[(((((size_t)t * NKC + c) * RB + rb) * KC + kk) * kBfPieces + q) * 512 +
+                                    (size_t)lane * 8 + e] = pc[q];
+                        }
+                }
     return out;
 }
 

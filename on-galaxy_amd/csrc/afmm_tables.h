@@ -12,6 +12,7 @@
 // ============================================================================
 #pragma once
 
+#include <cstdint>
 #include <vector>
 
 namespace afmm {
@@ -32,5 +33,16 @@ const Tables& tables_for_order(int p);
 // S x S (chunking of afmm_math.h m2l_*). A chunk is contiguous and one
 // fragment is one 512-byte run of 64 lanes.
 std::vector<double> mfma_a_layout(const std::vector<double>& colmajor, int S, int nmat);
+// M2L operators in the FP64 M2L kernel's chunks: MFMA fragments plus VALU tail rows.
+std::vector<double> m2l_a_layout(const std::vector<double>& colmajor, int S, int nmat);
+
+// The same matrices for v_mfma_f32_16x16x32_bf16, each entry split into three
+// bf16 pieces (x = hi + mid + lo to about 2^-24 relative; bf16 round to nearest
+// even of the running remainder): [matrix][k chunk c][row block rb][k block kk]
+// [piece][lane][8], value T(row = 16 rb + lane%16, col = 32 (c*KC + kk) +
+// 8 (lane/16) + e), zero outside S x S (chunking of afmm_math.h m2l_bf_*).
+std::vector<uint16_t> mfma_bf16_layout(const std::vector<double>& colmajor, int S, int nmat);
+// Three-piece bf16 split of x (bits), as above.
+void bf16_split3(double x, uint16_t out[3]);
 
 }  // namespace afmm

@@ -19,11 +19,55 @@ namespace {
 constexpr int kT = 256;
 inline int blocks(long long n, int t = kT) { return (int)std::max<long long>(1, (n + t - 1) / t); }
 
+// ---- XCD-aware workgroup order ------------------------------------------------
+// MI355X deals workgroups round-robin to its 8 XCDs, each with its own L2.
+// Kernels whose neighbouring work items (tiles or leaves in tree order) read
+// the same neighbour data launch xcd_grid(n) workgroups and take item
+// xcd_item(): each XCD then runs one contiguous range of items, in order, and
+// the shared reads hit its L2. Items >= n return at once.
+// Per-leaf kernels, whose cost varies from leaf to leaf in spatial clusters,
+// use chunks of K items instead (xcd_chunk_*): chunks go round-robin to the
+// XCDs, which keeps each XCD's share balanced and its reads local within a chunk.
+constexpr int kXcds = 8;
+inline int xcd_grid(int n) { return (n + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ int xcd_item() {
+    return (int)(blockIdx.x % kXcds) * (int)(gridDim.x / kXcds) + (int)(blockIdx.x / kXcds);
+}
+constexpr int kLeafChunk = 32;
+inline int xcd_chunk_grid(int n) { return (n + kXcds * kLeafChunk - 1) / (kXcds * kLeafChunk) * (kXcds * kLeafChunk); }
+__device__ __forceinline__ int xcd_chunk_item() {
+    const int x = (int)(blockIdx.x % kXcds), slot = (int)(blockIdx.x / kXcds);
+    return ((slot / kLeafChunk) * kXcds + x) * kLeafChunk + slot % kLeafChunk;
+}
+
 // ---- wave helpers -------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
+// Sums each of a lane's H register values over the wave by recursive halving:
+// at mask M = 32, 16, ..., 1 a lane keeps the lower ceil(H/2) of its values
+// when (lane & M) == 0, else the upper ones, and adds its partner's copy of
+// them. Every coefficient's total is one fixed sum tree (deterministic), and a
+// wave exchanges about H values instead of 6 H for H separate butterflies. On
+// return lane l holds the totals of coefficients first .. first + cnt - 1 in
+// v[0 .. cnt) (cnt <= 2 for H <= 128).
+template <int H, int M>
+__device__ __forceinline__ void wave_halving_sum(double* v, int lane, int& first, int& cnt) {
+    if constexpr (M >= 1) {
+        constexpr int H2 = (H + 1) / 2;
+        const bool up = (lane & M) != 0;
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;  // every lane holds the total (fixed butterfly order)
+        for (int q = 0; q < H2; ++q) {
+            const double lo = v[q];
+            const double hi = (H2 + q < H) ? v[H2 + q] : 0.0;
+            const double keep = up ? hi : lo;
+            v[q] = keep + __shfl_xor(up ? lo : hi, M, 64);
+        }
+        if (up) {
+            first += H2;
+            cnt = cnt > H2 ? cnt - H2 : 0;
+        } else {
+            cnt = cnt < H2 ? cnt : H2;
+        }
+        wave_halving_sum<H2, M / 2>(v, lane, first, cnt);
+    }
 }
 
 // ---- P2M and P2L: per-body coefficient accumulation -------------------------------
@@ -31,8 +75,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // packed stream; each lane accumulates q conj(R(d)) (P2M, FmmExpansions3D.hpp:
 // 148-163) or q conj(S(d)) (P2L, AdaptiveFMM3D.hpp:117-128) of its bodies into
 // a register window of the S coefficients; the windows are then summed across
-// the wave in a fixed butterfly order. Orders above 8 take two windows (two
-// passes over the stream) to bound registers.
+// the wave by recursive halving (fixed order). Orders above 8 take two windows
+// (two passes over the stream) to bound registers.
 //   KIND 0: targets = leaves, sources = the leaf's own bodies, Mhat = result.
 //   KIND 1: targets = boxes, sources = bodies of X(box) leaves, Lhat += result.
 template <int S>
@@ -48,41 +92,38 @@ struct StreamLds {
     int pre[kStreamChunk + 1];
 };
 
-// Loads entries [e0, e0 + m) of `list` (m <= kStreamChunk) and returns the
-// number of bodies they hold. All 64 lanes of the (single-wave) block call it.
+// Loads entries [e0, e0 + m) of `list` (m <= kStreamChunk) into LDS and
+// returns the number of bodies they hold (one-wave workgroup).
 __device__ __forceinline__ int stage_chunk(StreamLds& sl, const int* list, int e0, int m, const int* seg_first,
                                            const int* seg_count, int lane) {
-    int cnt[kStreamChunk / 64];
-    int run = 0;
+    {
+        int cnt[kStreamChunk / 64];
+        int run = 0;
 #pragma unroll
-    for (int r = 0; r < kStreamChunk / 64; ++r) {
-        const int e = lane * (kStreamChunk / 64) + r;  // lane owns 4 consecutive entries
-        int c = 0;
-        if (e < m) {
-            const int box = list[e0 + e];
-            sl.first[e] = seg_first[box];
-            c = seg_count[box];
+        for (int r = 0; r < kStreamChunk / 64; ++r) {
+            const int e = lane * (kStreamChunk / 64) + r;  // lane owns 4 consecutive entries
+            int c = 0;
+            if (e < m) {
+                const int box = list[e0 + e];
+                sl.first[e] = seg_first[box];
+                c = seg_count[box];
+            }
+            cnt[r] = run;
+            run += c;
         }
-        cnt[r] = run;
-        run += c;
-    }
-    // exclusive scan of the per-lane totals
-    int incl = run;
+        int incl = run;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += t;
-    }
-    const int base = incl - run;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
+        const int base = incl - run;
 #pragma unroll
-    for (int r = 0; r < kStreamChunk / 64; ++r) {
-        const int e = lane * (kStreamChunk / 64) + r;
-        if (e < kStreamChunk) sl.pre[e] = base + cnt[r];
+        for (int r = 0; r < kStreamChunk / 64; ++r) sl.pre[lane * (kStreamChunk / 64) + r] = base + cnt[r];
+        if (lane == 63) sl.pre[kStreamChunk] = incl;
     }
-    const int total = __shfl(incl, 63, 64);
-    if (lane == 0) sl.pre[kStreamChunk] = total;
     __syncthreads();
-    return total;
+    return sl.pre[kStreamChunk];
 }
 
 // Body of stream position q (< total) of the staged chunk with m entries.
@@ -96,6 +137,8 @@ __device__ __forceinline__ int stream_body(const StreamLds& sl, int m, int q) {
     return sl.first[lo] + (q - sl.pre[lo]);
 }
 
+// Coefficients C0 .. C0 + CW - 1 of one target box: lanes take the staged
+// bodies 64 at a time, then the wave sums its windows by recursive halving.
 template <int P, int KIND, int C0, int CW>
 __device__ __forceinline__ void accumulate_window(const TreeView& t, const int* list, int nlist, const double* ch,
                                                   double ih, int lane, StreamLds& sl, double* __restrict__ out,
@@ -123,16 +166,19 @@ __device__ __forceinline__ void accumulate_window(const TreeView& t, const int* 
             }
         }
     }
+    static_assert(CW >= 2 && CW <= 128, "halving leaves at most two totals per lane");
+    int first = 0, cnt = CW;
+    wave_halving_sum<CW, 32>(acc, lane, first, cnt);
 #pragma unroll
-    for (int k = 0; k < CW; ++k) {
-        const double v = wave_sum(acc[k]);
-        if (lane == (k & 63) && C0 + k < S) {
-            if (KIND == 0) out[out_base + C0 + k] = v;
-            else out[out_base + C0 + k] += v;
+    for (int k = 0; k < 2; ++k)
+        if (k < cnt && C0 + first + k < S) {
+            if (KIND == 0) out[out_base + C0 + first + k] = acc[k];
+            else out[out_base + C0 + first + k] += acc[k];
         }
-    }
 }
 
+// One wave per box (four waves per box with X lists measured 2.7x slower:
+// most X ranges hold too few bodies to occupy them).
 template <int P, int KIND>
 __global__ void __launch_bounds__(64) k_accumulate(TreeView t, double* __restrict__ out) {
     constexpr int S = nterms(P);
@@ -141,12 +187,15 @@ __global__ void __launch_bounds__(64) k_accumulate(TreeView t, double* __restric
     const int lane = threadIdx.x;
     int b, nlist;
     const int* list;
+    const int item = xcd_chunk_item();
     if (KIND == 0) {
-        b = t.leaf_ids[blockIdx.x];
-        list = t.leaf_ids + blockIdx.x;
+        if (item >= t.nleaves) return;
+        b = t.leaf_ids[item];
+        list = t.leaf_ids + item;
         nlist = 1;
     } else {
-        b = blockIdx.x;
+        if (item >= t.nb) return;
+        b = item;
         const int xa = t.x_off[b];
         list = t.x_idx + xa;
         nlist = t.x_off[b + 1] - xa;
@@ -160,8 +209,10 @@ __global__ void __launch_bounds__(64) k_accumulate(TreeView t, double* __restric
 }
 
 // ---- M2P over W (AdaptiveFMM3D.hpp:95, FmmExpansions3D.hpp:180-200) --------------
-// One wave per target leaf. The wave is split into G = floor(64/T) lane groups
-// of T = max(n_leaf, 8) lanes (T = 64 above 64 bodies); group g handles W
+// One wave per target leaf (an 8-wave workgroup per leaf measured 15% slower:
+// most W lists are short and leave its groups idle; prefetching the next
+// multipoles into registers gained nothing). The wave is split into
+// G = floor(64/T) lane groups of T = max(n_leaf, 8) lanes (T = 64 above 64 bodies); group g handles W
 // entries g, g+G, ... with their multipoles staged in LDS. Partial fields are
 // reduced across groups in a fixed order and added to far[] (and, with POT,
 // the potential to fpot[]).
@@ -171,7 +222,8 @@ __global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict
     constexpr int S = nterms(P);
     __shared__ double mw[8][S];
     __shared__ double red[4][64];
-    const int li = blockIdx.x;
+    const int li = xcd_chunk_item();
+    if (li >= t.nleaves) return;
     const int wa = t.w_off[li], nw = t.w_off[li + 1] - wa;
     if (nw == 0) return;
     const int b = t.leaf_ids[li];
@@ -200,14 +252,14 @@ __global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict
                 const double dx = offset_from(xi, Centre{c[0], c[1]}) * ih;
                 const double dy = offset_from(yi, Centre{c[2], c[3]}) * ih;
                 const double dz = offset_from(zi, Centre{c[4], c[5]}) * ih;
-                double gx, gy, gz;
+                double gx, gy, gz, pt = 0.0;
                 const double* mg = mw[g];
-                multipole_grad<P>([&](int k) { return mg[k]; }, dx, dy, dz, gx, gy, gz);
+                multipole_grad<P, POT>([&](int k) { return mg[k]; }, dx, dy, dz, gx, gy, gz, &pt);
                 const double s2 = ih * ih;
                 fx += gx * s2;
                 fy += gy * s2;
                 fz += gz * s2;
-                if constexpr (POT) fp += multipole_pot<P>([&](int k) { return mg[k]; }, dx, dy, dz) * ih;
+                if constexpr (POT) fp += pt * ih;
             }
         }
         if (Gs > 1) {
@@ -280,19 +332,17 @@ constexpr int m2l_cb() { return P <= 8 ? 2 : 1; }  // 16-target column blocks pe
 template <int P>
 constexpr int m2l_tile() { return kM2LWaves * 16 * m2l_cb<P>(); }
 
-// present[wg][o] = 1 when some target of workgroup tile wg has a source at
-// offset o (one wave per (tile, offset); the tile's ranks are contiguous in srcT).
-__global__ void __launch_bounds__(64) k_m2l_plan(int tile, int np, int ntgt, const int* __restrict__ srcT,
-                                                unsigned char* __restrict__ present) {
+// present[wg][o] = 1 when some target of tile wg (ranks tstart[wg] ..
+// tstart[wg+1]) has a source at offset o (one wave per (tile, offset)).
+__global__ void __launch_bounds__(64) k_m2l_plan(const int* __restrict__ tstart, int np,
+                                                const int* __restrict__ srcT, unsigned char* __restrict__ present) {
     const int wg = blockIdx.x / kNumOffsets, o = blockIdx.x % kNumOffsets;
     const int dx = o / 49 - 3, dy = (o / 7) % 7 - 3, dz = o % 7 - 3;
     const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
     bool any = false;
     if (!near)
-        for (int k = threadIdx.x; k < tile; k += 64) {
-            const int r = wg * tile + k;
-            if (r < ntgt && srcT[(size_t)o * np + r] >= 0) any = true;
-        }
+        for (int r = tstart[wg] + (int)threadIdx.x; r < tstart[wg + 1]; r += 64)
+            if (srcT[(size_t)o * np + r] >= 0) any = true;
     const bool f = __ballot(any) != 0ull;
     if (threadIdx.x == 0) present[(size_t)wg * kNumOffsets + o] = f ? 1 : 0;
 }
@@ -314,8 +364,11 @@ struct M2LTypes<true> {
     using Acc = flt4;
 };
 
+__device__ __forceinline__ double a_value(double a) { return a; }
+__device__ __forceinline__ double a_value(float2 a) { return (double)a.x + (double)a.y; }
+
 template <int P, bool MIX>
-__global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int ntgt,
+__global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int np, int ntiles, const int* __restrict__ tstart,
                                                            const int* __restrict__ order,
                                                            const int* __restrict__ srcT,
                                                            const unsigned char* __restrict__ present_g,
@@ -324,22 +377,28 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
     using AT = typename M2LTypes<MIX>::A;
     using Acc = typename M2LTypes<MIX>::Acc;
     constexpr int S = nterms(P);
-    constexpr int RB = m2l_rb(S), NKC = m2l_nkc(S), KC = m2l_kc(S), CH = m2l_chunk(S);
+    constexpr int RB = m2l_mrb(S), TR = m2l_tail(S), NKC = m2l_mnkc(S), KC = m2l_mkc(S), CH = m2l_mchunk(S);
     constexpr int CB = m2l_cb<P>();
     constexpr int TILE = m2l_tile<P>();
     constexpr int NT = kM2LWaves * 64;
     constexpr int PER = (CH + NT - 1) / NT;
-    __shared__ AT abuf[2][CH];
+    // WHOLE: both buffers hold a whole operator (all NKC chunks), so the
+    // workgroup synchronises once per offset; otherwise once per chunk.
+    constexpr bool WHOLE = NKC > 1 && 2 * NKC * CH * (int)sizeof(AT) <= 150 * 1024;
+    constexpr int BUFN = WHOLE ? NKC * CH : CH;
+    __shared__ AT abuf[2][BUFN];
     __shared__ unsigned char present[kNumOffsets + 1];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int j = lane & 15, kq = lane >> 4;
-    const int base = blockIdx.x * TILE;
+    const int tile = xcd_item();
+    if (tile >= ntiles) return;
+    const int base = tstart[tile], end = tstart[tile + 1];  // end - base <= TILE
     int rank[CB];
     bool valid[CB];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
         rank[cb] = base + (wave * CB + cb) * 16 + j;
-        valid[cb] = rank[cb] < ntgt;
+        valid[cb] = rank[cb] < end;
     }
     Acc acc[CB][RB];
 #pragma unroll
@@ -348,8 +407,15 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[cb][rb][r] = 0;
-    if (base < ntgt) {
-        for (int t = tid; t < kNumOffsets; t += NT) present[t] = present_g[(size_t)blockIdx.x * kNumOffsets + t];
+    (void)TILE;
+    // tail rows 16 RB + tr: lane (j, kq) sums its k = kq (mod 4) terms
+    double tacc[CB][TR > 0 ? TR : 1];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int tr = 0; tr < (TR > 0 ? TR : 1); ++tr) tacc[cb][tr] = 0.0;
+    {
+        for (int t = tid; t < kNumOffsets; t += NT) present[t] = present_g[(size_t)tile * kNumOffsets + t];
         if (tid == 0) present[kNumOffsets] = 1;  // sentinel
         __syncthreads();
         auto next_present = [&](int o) {
@@ -384,12 +450,14 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
         int scur[CB], snext[CB];
         double bcur[CB][KC], bnext[CB][KC];
         bool has_cur = false, has_next = false;
+        int onext = kNumOffsets;  // the offset after o
         if (o < kNumOffsets) {
             const AT* src = Asw + (size_t)o * NKC * CH;
-            for (int e = tid; e < CH; e += NT) abuf[0][e] = src[e];
+            for (int e = tid; e < (WHOLE ? NKC * CH : CH); e += NT) abuf[0][e] = src[e];
             has_cur = load_src(o, scur);
             if (has_cur) load_b(scur, 0, bcur);
-            has_next = load_src(next_present(o), snext);
+            onext = next_present(o);
+            has_next = load_src(onext, snext);
         }
         __syncthreads();
         int buf = 0, c = 0;
@@ -397,11 +465,14 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
             int on = o, cn = c + 1;
             if (cn == NKC) {
                 cn = 0;
-                on = next_present(o);
+                on = onext;
             }
+            // operator prefetch: chunk c of the next offset (WHOLE), else the next item's chunk
+            const int pf_o = WHOLE ? onext : on;
+            const int pf_c = WHOLE ? c : cn;
             AT pre[PER];
-            if (on < kNumOffsets) {
-                const AT* src = Asw + ((size_t)on * NKC + cn) * CH;
+            if (pf_o < kNumOffsets) {
+                const AT* src = Asw + ((size_t)pf_o * NKC + pf_c) * CH;
 #pragma unroll
                 for (int i = 0; i < PER; ++i) {
                     const int e = tid + NT * i;
@@ -415,7 +486,7 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
                 else load_b(scur, cn, bnext);
             }
             if (has_cur) {
-                const AT* ab = abuf[buf] + lane;
+                const AT* ab = abuf[buf] + (WHOLE ? c * CH : 0) + lane;
                 if constexpr (MIX) {
                     float bh[CB][KC], bl[CB][KC];
 #pragma unroll
@@ -449,21 +520,36 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
                                     __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[cb][kk], acc[cb][rb], 0, 0, 0);
                         }
                 }
+                if constexpr (TR > 0) {
+                    const AT* tb = abuf[buf] + (WHOLE ? c * CH : 0) + RB * KC * 64 + kq;
+#pragma unroll
+                    for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                        for (int tr = 0; tr < TR; ++tr) {
+                            const double a = a_value(tb[(tr * KC + kk) * 4]);
+#pragma unroll
+                            for (int cb = 0; cb < CB; ++cb) tacc[cb][tr] = fma(a, bcur[cb][kk], tacc[cb][tr]);
+                        }
+                }
             }
-            if (on < kNumOffsets) {
+            if (pf_o < kNumOffsets) {
+                AT* dst = abuf[buf ^ 1] + (WHOLE ? pf_c * CH : 0);
 #pragma unroll
                 for (int i = 0; i < PER; ++i) {
                     const int e = tid + NT * i;
-                    if (e < CH) abuf[buf ^ 1][e] = pre[i];
+                    if (e < CH) dst[e] = pre[i];
                 }
             }
-            __syncthreads();
-            buf ^= 1;
+            if (!WHOLE || cn == 0) {
+                __syncthreads();
+                buf ^= 1;
+            }
             if (cn == 0 && on < kNumOffsets) {
 #pragma unroll
                 for (int cb = 0; cb < CB; ++cb) scur[cb] = snext[cb];
                 has_cur = has_next;
-                has_next = load_src(next_present(on), snext);
+                onext = next_present(on);
+                has_next = load_src(onext, snext);
             }
 #pragma unroll
             for (int cb = 0; cb < CB; ++cb)
@@ -473,9 +559,19 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
             c = cn;
         }
     }
+    if constexpr (TR > 0) {
+        // (kq 0 + kq 1) + (kq 2 + kq 3), in every lane
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+            for (int tr = 0; tr < TR; ++tr) {
+                tacc[cb][tr] += __shfl_xor(tacc[cb][tr], 16, 64);
+                tacc[cb][tr] += __shfl_xor(tacc[cb][tr], 32, 64);
+            }
+    }
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-        if (rank[cb] >= nb) continue;
+        if (!valid[cb]) continue;
         const int t = order[rank[cb]];
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
@@ -485,7 +581,194 @@ __global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_mfma(int nb, int np, int
                 const int row = MIX ? 16 * rb + 4 * kq + r : 16 * rb + kq + 4 * r;
                 if (row < S) L[(size_t)t * S + row] = (double)acc[cb][rb][r];
             }
+        if constexpr (TR > 0)
+            if (kq == 0)
+#pragma unroll
+                for (int tr = 0; tr < TR; ++tr) L[(size_t)t * S + 16 * RB + tr] = tacc[cb][tr];
     }
+}
+
+// ---- M2L on bf16 matrix cores with three-piece split operands -------------------
+// Each operand x is carried as bf16 pieces hi + mid + lo (x to ~2^-24
+// relative); a 16 x 16 x 32 block takes the six products whose order is at or
+// above 2^-24 (lo*hi, hi*lo, mid*mid, mid*hi, hi*mid, hi*hi) on
+// v_mfma_f32_16x16x32_bf16 into an FP32 accumulator. Operand layout (16x16x32
+// bf16): A[i][k] in lane i + 16 (k/8), element k%8; B[k][j] in lane j + 16 (k/8);
+// D[i][j] in lane j + 16 (i/4), register i%4. Multipoles are split once per
+// evaluation into Mbf[box][k block][piece][k/8 % 4] (8 bf16 per 16-byte unit).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ double bf16_to_double(unsigned short b) {
+    return (double)__uint_as_float((unsigned)b << 16);
+}
+
+template <int P>
+__global__ void __launch_bounds__(kT) k_split_bf16(int nb, const double* __restrict__ M, uint4* __restrict__ Mbf) {
+    constexpr int S = nterms(P), KB = m2l_bf_kb(S);
+    const int t = blockIdx.x * kT + threadIdx.x;
+    if (t >= nb * KB * 4) return;
+    const int box = t / (KB * 4), kb = (t / 4) % KB, kq = t % 4;
+    unsigned short pc[kBfPieces][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = 32 * kb + 8 * kq + e;
+        double r = k < S ? M[(size_t)box * S + k] : 0.0;
+#pragma unroll
+        for (int q = 0; q < kBfPieces; ++q) {
+            pc[q][e] = bf16_rne((float)r);
+            r -= bf16_to_double(pc[q][e]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kBfPieces; ++q) {
+        uint4 v;
+        v.x = pc[q][0] | ((unsigned)pc[q][1] << 16);
+        v.y = pc[q][2] | ((unsigned)pc[q][3] << 16);
+        v.z = pc[q][4] | ((unsigned)pc[q][5] << 16);
+        v.w = pc[q][6] | ((unsigned)pc[q][7] << 16);
+        Mbf[(((size_t)box * KB + kb) * kBfPieces + q) * 4 + kq] = v;
+    }
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+constexpr int kM2LBfTile = kM2LWaves * 16;
+
+template <int P>
+__global__ void __launch_bounds__(kM2LWaves * 64) k_m2l_bf16(int np, int ntiles, const int* __restrict__ tstart,
+                                                           const int* __restrict__ order,
+                                                           const int* __restrict__ srcT,
+                                                           const unsigned char* __restrict__ present_g,
+                                                           const uint4* __restrict__ Abf,
+                                                           const uint4* __restrict__ Mbf, double* __restrict__ L) {
+    constexpr int S = nterms(P);
+    constexpr int RB = m2l_rb(S), KB = m2l_bf_kb(S), NKC = m2l_bf_nkc(S), KC = m2l_bf_kc(S);
+    constexpr int CHU = m2l_bf_chunk(S);
+    constexpr int NT = kM2LWaves * 64;
+    constexpr int PER = (CHU + NT - 1) / NT;
+    __shared__ uint4 abuf[2][CHU];
+    __shared__ unsigned char present[kNumOffsets + 1];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int j = lane & 15, kq = lane >> 4;
+    const int tile = xcd_item();
+    if (tile >= ntiles) return;
+    const int base = tstart[tile], end = tstart[tile + 1];  // end - base <= kM2LBfTile
+    const int rank = base + wave * 16 + j;
+    const bool valid = rank < end;
+    flt4 acc[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[rb][r] = 0.f;
+    {
+        for (int t = tid; t < kNumOffsets; t += NT) present[t] = present_g[(size_t)tile * kNumOffsets + t];
+        if (tid == 0) present[kNumOffsets] = 1;  // sentinel
+        __syncthreads();
+        auto next_present = [&](int o) {
+            ++o;
+            while (!present[o]) ++o;
+            return o;  // kNumOffsets when none left
+        };
+        auto load_b = [&](int src, int c, uint4 (&bv)[KC][kBfPieces]) {
+            const uint4* ms = Mbf + (size_t)(src >= 0 ? src : 0) * KB * kBfPieces * 4;
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) {
+                const int kb = c * KC + kk;
+#pragma unroll
+                for (int q = 0; q < kBfPieces; ++q)
+                    bv[kk][q] = (src >= 0 && kb < KB) ? ms[(kb * kBfPieces + q) * 4 + kq] : make_uint4(0, 0, 0, 0);
+            }
+        };
+        auto load_src = [&](int o, int& src) {
+            src = (o < kNumOffsets && valid) ? srcT[(size_t)o * np + rank] : -1;
+            return __ballot(src >= 0) != 0ull;
+        };
+        int o = next_present(-1);
+        int scur = -1, snext = -1;
+        uint4 bcur[KC][kBfPieces], bnext[KC][kBfPieces];
+        bool has_cur = false, has_next = false;
+        if (o < kNumOffsets) {
+            const uint4* src = Abf + (size_t)o * NKC * CHU;
+            for (int e = tid; e < CHU; e += NT) abuf[0][e] = src[e];
+            has_cur = load_src(o, scur);
+            if (has_cur) load_b(scur, 0, bcur);
+            has_next = load_src(next_present(o), snext);
+        }
+        __syncthreads();
+        int buf = 0, c = 0;
+        while (o < kNumOffsets) {
+            int on = o, cn = c + 1;
+            if (cn == NKC) {
+                cn = 0;
+                on = next_present(o);
+            }
+            uint4 pre[PER];
+            if (on < kNumOffsets) {
+                const uint4* src = Abf + ((size_t)on * NKC + cn) * CHU;
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int e = tid + NT * i;
+                    pre[i] = (e < CHU) ? src[e] : make_uint4(0, 0, 0, 0);
+                }
+            }
+            const bool nxt_has = (cn == 0) ? has_next : has_cur;
+            if (on < kNumOffsets && nxt_has) load_b(cn == 0 ? snext : scur, cn, bnext);
+            if (has_cur) {
+                const uint4* ab = abuf[buf] + lane;
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) {
+                    const bf16x8 bh = as_bf16x8(bcur[kk][0]), bm = as_bf16x8(bcur[kk][1]),
+                                 bl = as_bf16x8(bcur[kk][2]);
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) {
+                        const uint4* af = ab + ((rb * KC + kk) * kBfPieces) * 64;
+                        const bf16x8 ah = as_bf16x8(af[0]), am = as_bf16x8(af[64]), al = as_bf16x8(af[128]);
+                        flt4 a = acc[rb];
+                        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, a, 0, 0, 0);
+                        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, a, 0, 0, 0);
+                        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, a, 0, 0, 0);
+                        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, a, 0, 0, 0);
+                        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, a, 0, 0, 0);
+                        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, a, 0, 0, 0);
+                    }
+                }
+            }
+            if (on < kNumOffsets) {
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int e = tid + NT * i;
+                    if (e < CHU) abuf[buf ^ 1][e] = pre[i];
+                }
+            }
+            __syncthreads();
+            buf ^= 1;
+            if (cn == 0 && on < kNumOffsets) {
+                scur = snext;
+                has_cur = has_next;
+                has_next = load_src(next_present(on), snext);
+            }
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int q = 0; q < kBfPieces; ++q) bcur[kk][q] = bnext[kk][q];
+            o = on;
+            c = cn;
+        }
+    }
+    if (!valid) return;
+    const int t = order[rank];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * rb + 4 * kq + r;
+            if (row < S) L[(size_t)t * S + row] = (double)acc[rb][r];
+        }
 }
 
 // ---- M2M / L2L on FP64 matrix cores ----------------------------------------------
@@ -601,15 +884,16 @@ __global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict
 
 // ---- P2P over U + final combine (AdaptiveFMM3D.hpp:96-106) ------------------
 // One wave per target leaf. The source bodies of U(b) (contiguous ranges in
-// sorted order) are streamed as one concatenated sequence in full 64-body LDS
-// tiles. The wave is split into G = floor(64/T) lane groups of T = the leaf's
-// body count (at least 8, at most 64); group g takes the tile's sources
-// g, g+G, ... so short leaves still keep all lanes busy. Partial sums are
-// reduced across groups in a fixed order (deterministic). The kernel is
-// (|d|^2 + eps^2)^{-3/2} from v_rsq_f64 refined by two Newton steps; the self
-// pair is skipped by index, as in the reference.
-constexpr int kP2PMaxU = 256;
-
+// sorted order) are streamed as one concatenated sequence in 64-body LDS
+// tiles: the ranges of up to kStreamChunk U boxes and their running body
+// offsets are staged in LDS once, and each lane finds the body of its tile
+// slot by binary search. Every lane holds R targets (R = 2 above 32 bodies,
+// else 1) of a lane group of T lanes (T = ceil(n_leaf / R), 8 <= T <= 64);
+// the wave's G = floor(64/T) groups take the tile's sources g, g+G, ... and
+// their partial sums are reduced in a fixed order (deterministic). The kernel
+// is (|d|^2 + eps^2)^{-3/2} from v_rsq_f64 refined by two Newton steps; the
+// self pair is skipped by index, as in the reference (branch-free: its
+// distance is replaced by 1 and its mass by 0).
 __device__ __forceinline__ double rsqrt_f64(double x) {
     double y = __builtin_amdgcn_rsq(x);
     const double hx = 0.5 * x;
@@ -618,158 +902,166 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
     return y;
 }
 
+template <bool POT>
+struct P2PAcc {
+    double x = 0.0, y = 0.0, z = 0.0, p = 0.0;
+};
+
 // MIX: the pair difference stays FP64 (it carries the cancellation), then
 // r^2, v_rsq_f32 and the pair term run in FP32; each lane sums its terms of
 // one 64-body source tile in FP32 and adds that partial to its FP64 accumulator.
-template <bool MIX, bool POT>
-__global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict__ far, double* __restrict__ acc,
-                                           const double* __restrict__ fpot, double* __restrict__ phi) {
-    __shared__ double sx[64], sy[64], sz[64], sm[64];
-    __shared__ int sj[64];
-    __shared__ int ufirst[kP2PMaxU], ucount[kP2PMaxU];
-    __shared__ double red[4][64];
-    const int li = blockIdx.x;
-    const int b = t.leaf_ids[li];
+template <bool MIX, bool POT, int R>
+__device__ __forceinline__ void p2p_tile(const double* sx, const double* sy, const double* sz, const double* sm,
+                                         const int* sj, int kfirst, int cnt, int Gs, const double (&xi)[R][3],
+                                         const int (&ii)[R], double eps2, P2PAcc<POT> (&a)[R]) {
+    if constexpr (MIX) {
+        const float e2 = (float)eps2;
+        float fx[R], fy[R], fz[R], fp[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) fx[r] = fy[r] = fz[r] = fp[r] = 0.f;
+#pragma unroll 1
+        for (int k = kfirst; k < cnt; k += Gs) {
+            const double xk = sx[k], yk = sy[k], zk = sz[k];
+            const float mk = (float)sm[k];
+            const int j = sj[k];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float dx = (float)(xk - xi[r][0]), dy = (float)(yk - xi[r][1]), dz = (float)(zk - xi[r][2]);
+                const bool self = j == ii[r];
+                const float r2 = self ? 1.f : dx * dx + dy * dy + dz * dz + e2;
+                const float m = self ? 0.f : mk;
+                const float rin = __builtin_amdgcn_rsqf(r2);
+                const float sc = m * (rin * rin * rin);
+                fx[r] += sc * dx;
+                fy[r] += sc * dy;
+                fz[r] += sc * dz;
+                if constexpr (POT) fp[r] += m * rin;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            a[r].x += (double)fx[r];
+            a[r].y += (double)fy[r];
+            a[r].z += (double)fz[r];
+            a[r].p += (double)fp[r];
+        }
+    } else {
+#pragma unroll 1
+        for (int k = kfirst; k < cnt; k += Gs) {
+            const double xk = sx[k], yk = sy[k], zk = sz[k], mk = sm[k];
+            const int j = sj[k];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double dx = xk - xi[r][0], dy = yk - xi[r][1], dz = zk - xi[r][2];
+                const bool self = j == ii[r];
+                const double r2 = self ? 1.0 : dx * dx + dy * dy + dz * dz + eps2;
+                const double m = self ? 0.0 : mk;
+                const double rin = rsqrt_f64(r2);
+                const double sc = m * (rin * rin * rin);
+                a[r].x += sc * dx;
+                a[r].y += sc * dy;
+                a[r].z += sc * dz;
+                if constexpr (POT) a[r].p += m * rin;
+            }
+        }
+    }
+}
+
+template <bool MIX, bool POT, int R>
+__device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const double* __restrict__ far,
+                                         double* __restrict__ acc, const double* __restrict__ fpot,
+                                         double* __restrict__ phi, StreamLds& sl, double* sx, double* sy,
+                                         double* sz, double* sm, int* sj, double (*red)[64]) {
     const int f = t.seg_first[b], nt = t.seg_count[b];
     const int lane = threadIdx.x;
     const int q = t.bsys[b];
     const double G = t.sys_G[q], eps2 = t.sys_eps2[q];
     const int ua = t.u_off[li], nu = t.u_off[li + 1] - ua;
-    const bool cached = nu <= kP2PMaxU;
-    if (cached)
-        for (int e = lane; e < nu; e += 64) {
-            const int c = t.u_idx[ua + e];
-            ufirst[e] = t.seg_first[c];
-            ucount[e] = t.seg_count[c];
-        }
-    __syncthreads();
-    auto range = [&](int e, int& first, int& cnt) {
-        if (cached) {
-            first = ufirst[e];
-            cnt = ucount[e];
-        } else {
-            const int c = t.u_idx[ua + e];
-            first = t.seg_first[c];
-            cnt = t.seg_count[c];
-        }
-    };
-    const int T = nt > 64 ? 64 : (nt < 8 ? 8 : nt);
+    const int T = min(64, max(8, (nt + R - 1) / R));
     const int Gs = 64 / T;
     const int g = lane / T, tl = lane % T;
     const int kfirst = g < Gs ? g : 64;  // lanes past the last whole group idle
-    for (int tb = 0; tb < nt; tb += T) {
-        const int i = f + tb + tl;
-        const bool act = g < Gs && tb + tl < nt;
-        const double xi = act ? t.x[i] : 0.0, yi = act ? t.y[i] : 0.0, zi = act ? t.z[i] : 0.0;
-        double ax = 0.0, ay = 0.0, az = 0.0, ap = 0.0;
-        int e = 0, off = 0;  // stream position: range e, offset off
-        while (e < nu) {
-            // lane takes stream position (e, off) + lane
-            int ee = e, oo = off + lane, fst = 0, cnt = 0;
-            bool ok = false;
-            while (ee < nu) {
-                range(ee, fst, cnt);
-                if (oo < cnt) {
-                    ok = true;
-                    break;
+    for (int tb = 0; tb < nt; tb += R * T) {
+        double xi[R][3];
+        int ii[R];
+        bool act[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = f + tb + r * T + tl;
+            act[r] = g < Gs && tb + r * T + tl < nt;
+            ii[r] = act[r] ? i : -2;
+            xi[r][0] = act[r] ? t.x[i] : 0.0;
+            xi[r][1] = act[r] ? t.y[i] : 0.0;
+            xi[r][2] = act[r] ? t.z[i] : 0.0;
+        }
+        P2PAcc<POT> a[R];
+        for (int e0 = 0; e0 < nu; e0 += kStreamChunk) {
+            const int m = min(kStreamChunk, nu - e0);
+            __syncthreads();  // previous readers of the staged ranges are done
+            const int total = stage_chunk(sl, t.u_idx + ua, e0, m, t.seg_first, t.seg_count, lane);
+            for (int q0 = 0; q0 < total; q0 += 64) {
+                const int qq = q0 + lane;
+                if (qq < total) {
+                    const int j = stream_body(sl, m, qq);
+                    sx[lane] = t.x[j];
+                    sy[lane] = t.y[j];
+                    sz[lane] = t.z[j];
+                    sm[lane] = t.m[j];
+                    sj[lane] = j;
                 }
-                oo -= cnt;
-                ++ee;
+                __syncthreads();
+                p2p_tile<MIX, POT, R>(sx, sy, sz, sm, sj, kfirst, min(64, total - q0), Gs, xi, ii, eps2, a);
+                __syncthreads();
             }
-            if (ok) {
-                const int j = fst + oo;
-                sx[lane] = t.x[j];
-                sy[lane] = t.y[j];
-                sz[lane] = t.z[j];
-                sm[lane] = t.m[j];
-                sj[lane] = j;
-            } else {
-                sj[lane] = -1;
-            }
-            // advance the stream by 64 bodies (wave-uniform)
-            {
-                int rem = 64;
-                while (e < nu) {
-                    int f2, c2;
-                    range(e, f2, c2);
-                    if (off + rem < c2) {
-                        off += rem;
-                        rem = 0;
-                        break;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (Gs > 1) {
+                red[0][lane] = a[r].x;
+                red[1][lane] = a[r].y;
+                red[2][lane] = a[r].z;
+                if constexpr (POT) red[3][lane] = a[r].p;
+                __syncthreads();
+                if (g == 0) {
+                    for (int u = 1; u < Gs; ++u) {
+                        a[r].x += red[0][u * T + tl];
+                        a[r].y += red[1][u * T + tl];
+                        a[r].z += red[2][u * T + tl];
+                        if constexpr (POT) a[r].p += red[3][u * T + tl];
                     }
-                    rem -= c2 - off;
-                    off = 0;
-                    ++e;
                 }
+                __syncthreads();
             }
-            __syncthreads();
-            if constexpr (MIX) {
-                const float e2 = (float)eps2;
-                float fx = 0.f, fy = 0.f, fz = 0.f, fp = 0.f;
-                for (int k = kfirst; k < 64; k += Gs) {
-                    const int j = sj[k];
-                    if (j < 0) break;
-                    const float dx = (float)(sx[k] - xi), dy = (float)(sy[k] - yi), dz = (float)(sz[k] - zi);
-                    const float r2 = dx * dx + dy * dy + dz * dz + e2;
-                    const float rin = __builtin_amdgcn_rsqf(r2);
-                    // the self pair is selected away, not multiplied by zero
-                    // (with eps = 0 its 1/r is infinite)
-                    const float mj = (float)sm[k];
-                    const float s = (j != i) ? mj * (rin * rin * rin) : 0.f;
-                    fx += s * dx;
-                    fy += s * dy;
-                    fz += s * dz;
-                    if constexpr (POT) fp += (j != i) ? mj * rin : 0.f;
-                }
-                ax += (double)fx;
-                ay += (double)fy;
-                az += (double)fz;
-                ap += (double)fp;
-            } else {
-                for (int k = kfirst; k < 64; k += Gs) {
-                    const int j = sj[k];
-                    if (j < 0) break;
-                    const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
-                    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
-                    const double rin = rsqrt_f64(r2);
-                    const double s = (j != i) ? sm[k] * (rin * rin * rin) : 0.0;
-                    ax += s * dx;
-                    ay += s * dy;
-                    az += s * dz;
-                    if constexpr (POT) ap += (j != i) ? sm[k] * rin : 0.0;
-                }
+            if (act[r] && g == 0) {
+                const int i = ii[r];
+                const int o = t.orig[i];
+                acc[3 * (size_t)o] = G * (far[3 * (size_t)i] + a[r].x);
+                acc[3 * (size_t)o + 1] = G * (far[3 * (size_t)i + 1] + a[r].y);
+                acc[3 * (size_t)o + 2] = G * (far[3 * (size_t)i + 2] + a[r].z);
+                if constexpr (POT) phi[o] = -G * (fpot[i] + a[r].p);
             }
-            __syncthreads();
-        }
-        if (Gs > 1) {
-            red[0][lane] = ax;
-            red[1][lane] = ay;
-            red[2][lane] = az;
-            if constexpr (POT) red[3][lane] = ap;
-            __syncthreads();
-            if (g == 0) {
-                for (int q = 1; q < Gs; ++q) {
-                    ax += red[0][q * T + tl];
-                    ay += red[1][q * T + tl];
-                    az += red[2][q * T + tl];
-                    if constexpr (POT) ap += red[3][q * T + tl];
-                }
-            }
-            __syncthreads();
-        }
-        if (act && g == 0) {
-            const int o = t.orig[i];
-            acc[3 * (size_t)o] = G * (far[3 * (size_t)i] + ax);
-            acc[3 * (size_t)o + 1] = G * (far[3 * (size_t)i + 1] + ay);
-            acc[3 * (size_t)o + 2] = G * (far[3 * (size_t)i + 2] + az);
-            if constexpr (POT) phi[o] = -G * (fpot[i] + ap);
         }
     }
 }
 
+template <bool MIX, bool POT>
+__global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict__ far, double* __restrict__ acc,
+                                           const double* __restrict__ fpot, double* __restrict__ phi) {
+    __shared__ double sx[64], sy[64], sz[64], sm[64];
+    __shared__ int sj[64];
+    __shared__ StreamLds sl;
+    __shared__ double red[4][64];
+    const int li = xcd_chunk_item();
+    if (li >= t.nleaves) return;
+    const int b = t.leaf_ids[li];
+    if (t.seg_count[b] > 32) p2p_leaf<MIX, POT, 2>(t, b, li, far, acc, fpot, phi, sl, sx, sy, sz, sm, sj, red);
+    else p2p_leaf<MIX, POT, 1>(t, b, li, far, acc, fpot, phi, sl, sx, sy, sz, sm, sj, red);
+}
+
 template <int P>
-void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, bool mixed,
+void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, int prec,
          double* acc, double* phi, hipEvent_t* ev) {
+    const bool mixed = prec != kPrecFP64;
     constexpr int S = nterms(P);
     const int nb = t.nb;
     double* M = pool.get<double>("Mexp", (size_t)nb * S);
@@ -777,7 +1069,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     double* far = pool.get<double>("far", 3 * (size_t)t.n);
     double* fpot = phi ? pool.get<double>("farpot", t.n) : nullptr;
     if (ev) AFMM_CHECK(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL((k_accumulate<P, 0>), dim3(t.nleaves), dim3(64), 0, s, t, M);
+    hipLaunchKernelGGL((k_accumulate<P, 0>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M);
     for (int l = t.max_level - 1; l >= 0; --l) {
         const int off = meta.nonleaf_off[l], cnt = meta.nonleaf_off[l + 1] - off;
         if (cnt > 0)
@@ -788,27 +1080,40 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (ev) AFMM_CHECK(hipEventRecord(ev[1], s));
     {
         // M2L targets in (level, octant) order; transposed source table; tile plan
-        constexpr int TILE = m2l_tile<P>();
-        const int np = (nb + TILE - 1) / TILE * TILE;
-        const int nwg = np / TILE;
+        // M2L targets in (octant, level) order, tiles cut inside octant groups
+        const int TILE = prec == kPrecMixedBf16 ? kM2LBfTile : m2l_tile<P>();
+        const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
+        const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
+        if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
+        const int np = nb;
         int* rank = pool.get<int>("m2l_rank", nb);
         int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
-        unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)nwg * kNumOffsets);
-        hipLaunchKernelGGL(k_rank_of, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.lo_order, rank);
+        unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
+        hipLaunchKernelGGL(k_rank_of, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.m2l_order, rank);
         hipLaunchKernelGGL(k_fill_int, dim3(blocks((long long)kNumOffsets * np)), dim3(kT), 0, s, srcT,
                            (size_t)kNumOffsets * np, -1);
         hipLaunchKernelGGL(k_m2l_src, dim3(blocks(nb)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv, t.ic, srcT);
-        hipLaunchKernelGGL(k_m2l_plan, dim3(nwg * kNumOffsets), dim3(64), 0, s, TILE, np, nb, srcT, present);
+        hipLaunchKernelGGL(k_m2l_plan, dim3(ntiles * kNumOffsets), dim3(64), 0, s, tstart, np, srcT, present);
+        uint4* Mbf = nullptr;
+        if (prec == kPrecMixedBf16) {
+            constexpr int KB = m2l_bf_kb(S);
+            Mbf = pool.get<uint4>("m2l_Mbf", (size_t)nb * KB * kBfPieces * 4);
+            hipLaunchKernelGGL(k_split_bf16<P>, dim3(blocks((long long)nb * KB * 4)), dim3(kT), 0, s, nb, M, Mbf);
+        }
         if (ev) AFMM_CHECK(hipEventRecord(ev[5], s));
-        if (mixed)
-            hipLaunchKernelGGL((k_m2l_mfma<P, true>), dim3(nwg), dim3(kM2LWaves * 64), 0, s, nb, np, nb, t.lo_order,
-                               srcT, present, tab.m2l_hl, M, L);
+        if (prec == kPrecMixedBf16)
+            hipLaunchKernelGGL(k_m2l_bf16<P>, dim3(xcd_grid(ntiles)), dim3(kM2LWaves * 64), 0, s, np, ntiles, tstart,
+                               t.m2l_order, srcT, present, tab.m2l_bf, Mbf, L);
+        else if (prec == kPrecMixed)
+            hipLaunchKernelGGL((k_m2l_mfma<P, true>), dim3(xcd_grid(ntiles)), dim3(kM2LWaves * 64), 0, s, np, ntiles,
+                               tstart, t.m2l_order, srcT, present, tab.m2l_hl, M, L);
         else
-            hipLaunchKernelGGL((k_m2l_mfma<P, false>), dim3(nwg), dim3(kM2LWaves * 64), 0, s, nb, np, nb, t.lo_order,
-                               srcT, present, tab.m2l_sw, M, L);
+            hipLaunchKernelGGL((k_m2l_mfma<P, false>), dim3(xcd_grid(ntiles)), dim3(kM2LWaves * 64), 0, s, np, ntiles,
+                               tstart, t.m2l_order, srcT, present, tab.m2l_sw, M, L);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
-    if (t.n_x > 0) hipLaunchKernelGGL((k_accumulate<P, 1>), dim3(nb), dim3(64), 0, s, t, L);
+    if (t.n_x > 0)
+        hipLaunchKernelGGL((k_accumulate<P, 1>), dim3(xcd_chunk_grid(nb)), dim3(64), 0, s, t, L);
     for (int l = 1; l <= t.max_level; ++l) {
         const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
         hipLaunchKernelGGL((k_level_mfma<P, 1>), dim3((cnt + 15) / 16), dim3(kLevelWaves * 64), 0, s, cnt,
@@ -818,10 +1123,10 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (ev) AFMM_CHECK(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far, fpot);
     if (t.n_w > 0) {
-        if (fpot) hipLaunchKernelGGL((k_m2p<P, true>), dim3(t.nleaves), dim3(64), 0, s, t, M, far, fpot);
-        else hipLaunchKernelGGL((k_m2p<P, false>), dim3(t.nleaves), dim3(64), 0, s, t, M, far, fpot);
+        if (fpot) hipLaunchKernelGGL((k_m2p<P, true>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M, far, fpot);
+        else hipLaunchKernelGGL((k_m2p<P, false>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M, far, fpot);
     }
-    const dim3 pg(t.nleaves), pb(64);
+    const dim3 pg(xcd_chunk_grid(t.nleaves)), pb(64);
     if (mixed && phi)
         hipLaunchKernelGGL((k_p2p<true, true>), pg, pb, 0, s, t, far, acc, fpot, phi);
     else if (mixed)
@@ -837,11 +1142,12 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
 }  // namespace
 
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
-             bool mixed, double* d_acc, double* d_phi, hipEvent_t* ev) {
-    if (mixed && !tab.m2l_hl) throw std::runtime_error("split FP32 M2L operators not loaded");
+             int prec, double* d_acc, double* d_phi, hipEvent_t* ev) {
+    if (prec == kPrecMixed && !tab.m2l_hl) throw std::runtime_error("split FP32 M2L operators not loaded");
+    if (prec == kPrecMixedBf16 && !tab.m2l_bf) throw std::runtime_error("bf16 M2L operators not loaded");
     switch (p) {
 #define AFMM_CASE(PP) \
-    case PP: run<PP>(pool, s, tv, meta, tab, mixed, d_acc, d_phi, ev); return;
+    case PP: run<PP>(pool, s, tv, meta, tab, prec, d_acc, d_phi, ev); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
 #undef AFMM_CASE

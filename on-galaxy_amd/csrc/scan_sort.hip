@@ -10,6 +10,8 @@
 // the four waves of the block. Everything is integer arithmetic: the result
 // is deterministic and identical to a sequential stable counting sort.
 // ============================================================================
+#include <algorithm>
+
 #include "afmm_internal.h"
 
 namespace afmm {
@@ -36,8 +38,15 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total)
     return incl - v;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_tile_sums(const int* in, int n, int* sums) {
+// rng (optional, device): the item count is rng[1] - rng[0] instead of n, and
+// blocks past it only write zero tile sums.
+__device__ __forceinline__ int scan_count(int n, const int* rng) {
+    return rng ? max(0, min(n, rng[1] - rng[0])) : n;  // n bounds the range (buffer size)
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_tile_sums(const int* in, int n, int* sums, const int* rng) {
     __shared__ int lds[kScanThreads];
+    n = scan_count(n, rng);
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int s = 0;
 #pragma unroll
@@ -48,9 +57,12 @@ __global__ void __launch_bounds__(kScanThreads) k_tile_sums(const int* in, int n
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// One block scans `m` tile sums in chunks with a running carry.
-__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int* sums, int m, int* total_out) {
+// One block scans `m` tile sums in chunks with a running carry. With rng, m is
+// the tile count of rng[1] - rng[0] items and the total is written as
+// rng[1] + total (the end offset of the next range).
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int* sums, int m, int* total_out, const int* rng) {
     __shared__ int lds[kScanThreads];
+    if (rng) m = min(m, (max(0, rng[1] - rng[0]) + kScanTile - 1) / kScanTile);
     int carry = 0;
     for (int base = 0; base < m; base += kScanThreads) {
         const int i = base + threadIdx.x;
@@ -60,12 +72,14 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_sums(int* sums, int m, in
         if (i < m) sums[i] = carry + ex;
         carry += total;
     }
-    if (threadIdx.x == 0 && total_out) *total_out = carry;
+    if (threadIdx.x == 0 && total_out) *total_out = carry + (rng ? rng[1] : 0);
 }
 
 __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int* in, int n, const int* sums,
-                                                             int* out) {
+                                                             int* out, const int* rng) {
     __shared__ int lds[kScanThreads];
+    n = scan_count(n, rng);
+    if (blockIdx.x * kScanTile >= n) return;
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
     int s = 0;
@@ -176,12 +190,21 @@ int scan_tmp_ints(int n) { return (n + kScanTile - 1) / kScanTile + 1; }
 void scan_exclusive(const int* d_in, int* d_out, int n, int* d_total, int* d_tmp, hipStream_t s) {
     const int tiles = (n + kScanTile - 1) / kScanTile;
     if (n <= 0) {
-        if (d_total) hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, 0, d_total);
+        if (d_total)
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, 0, d_total, nullptr);
         return;
     }
-    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, tiles, d_total);
-    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp, d_out);
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp, nullptr);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, tiles, d_total, nullptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp, d_out, nullptr);
+}
+
+void scan_exclusive_range(const int* d_in, int* d_out, const int* d_rng, int nmax, int* d_end, int* d_tmp,
+                          hipStream_t s) {
+    const int tiles = std::max(1, (nmax + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, d_in, nmax, d_tmp, d_rng);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, tiles, d_end, d_rng);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, d_in, nmax, d_tmp, d_out, d_rng);
 }
 
 int sort_blocks(int n) { return (n + kSortTile - 1) / kSortTile; }

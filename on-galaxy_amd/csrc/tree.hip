@@ -203,38 +203,49 @@ __global__ void __launch_bounds__(kT) k_keys(const double* pos, int n, int nsys,
 }
 
 // ---- topology, one level at a time --------------------------------------------
-// For each box of level l: split decision and child ranges by binary search on
-// the level-(l+1) digit inside its (sorted) segment.
+// For each box of level l: split decision and child ranges on the level-(l+1)
+// digit inside its (sorted) segment. Eight lanes per box (one wave holds eight
+// boxes): lane o finds the first body with digit >= o by its own binary search
+// over the segment, and the box's child count is gathered with shuffles.
 __global__ void __launch_bounds__(kT) k_split(int off, int cnt, int l, int cap, const double* eps_floor,
                                              const RootInfo* roots, const int* bsys, const int* seg_first,
                                              const int* seg_count, const uint64_t* khi, const uint32_t* klo,
                                              int* bnd, int* nch) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= cnt) return;
-    const int b = off + i;
-    const int f = seg_first[b], c = seg_count[b];
-    const int q = bsys[b];
-    const double half = ldexp(roots[q].half, -l);
-    int* bd = bnd + 9 * (size_t)i;
-    const bool split = c > cap && !(2.0 * half <= eps_floor[q]) && l < kMaxDepth;
-    if (!split) {
-        nch[i] = 0;
-        return;
+    const int gt = blockIdx.x * kT + threadIdx.x;
+    const int i = gt >> 3, o = gt & 7;
+    const bool live = i < cnt;
+    int f = 0, c = 0;
+    bool split = false;
+    if (live) {
+        const int b = off + i;
+        f = seg_first[b];
+        c = seg_count[b];
+        const int q = bsys[b];
+        const double half = ldexp(roots[q].half, -l);
+        split = c > cap && !(2.0 * half <= eps_floor[q]) && l < kMaxDepth;
     }
-    bd[0] = f;
-    bd[8] = f + c;
-    for (int o = 1; o < 8; ++o) {  // first index with digit >= o
-        int lo = bd[o - 1], hi = f + c;
+    int lo = f;
+    if (split && o > 0) {
+        int hi = f + c;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (level_digit(khi[mid], klo[mid], l + 1) < o) lo = mid + 1;
             else hi = mid;
         }
-        bd[o] = lo;
     }
-    int k = 0;
-    for (int o = 0; o < 8; ++o) k += bd[o + 1] > bd[o];
-    nch[i] = k;
+    // bound of octant o + 1 (lane o + 1 of the group; the last lane uses the segment end)
+    const int up = __shfl_down(lo, 1, 8);
+    const int end = (o == 7) ? f + c : up;
+    int nonempty = (split && end > lo) ? 1 : 0;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) nonempty += __shfl_xor(nonempty, m, 8);
+    if (!live) return;
+    int* bd = bnd + 9 * (size_t)i;
+    if (split) {
+        bd[o] = lo;
+        if (o == 7) bd[8] = f + c;
+    }
+    if (o == 0) nch[i] = split ? nonempty : 0;
 }
 
 __global__ void __launch_bounds__(kT) k_emit(int off, int cnt, int l, int next_off, const int* bnd,
@@ -268,6 +279,118 @@ __global__ void __launch_bounds__(kT) k_emit(int off, int cnt, int l, int next_o
         bsys[c] = bsys[b];
     }
     mask[b] = mk;
+}
+
+// Device-counted variants of k_split / k_emit for the level loop that never
+// waits for the host: level l holds boxes [lev[l], lev[l+1]) and the scan of
+// the child counts writes lev[l+2]. Fixed grids stride over the level; child
+// slots beyond the box arrays' capacity set *overflow and are not written.
+// Level range of lev[l] clamped to the buffers (maxcnt entries per level,
+// `capacity` boxes); a clamp can only follow an overflow, which the host sees.
+__device__ __forceinline__ int level_count(const int* lev, int l, int maxcnt, int capacity) {
+    const int off = lev[l];
+    return max(0, min(min(lev[l + 1] - off, maxcnt), capacity - off));
+}
+
+__global__ void __launch_bounds__(kT) k_split_dev(const int* __restrict__ lev, int l, int maxcnt, int capacity,
+                                                 int cap, const double* eps_floor, const RootInfo* roots,
+                                                 const int* bsys, const int* seg_first, const int* seg_count,
+                                                 const uint64_t* khi, const uint32_t* klo, int* bnd, int* nch) {
+    const int off = lev[l], cnt = level_count(lev, l, maxcnt, capacity);
+    for (long long gt = (long long)blockIdx.x * kT + threadIdx.x; gt < 8LL * ((cnt + 7) / 8 * 8);
+         gt += (long long)gridDim.x * kT) {
+        const int i = (int)(gt >> 3), o = (int)(gt & 7);
+        const bool live = i < cnt;
+        int f = 0, c = 0;
+        bool split = false;
+        if (live) {
+            const int b = off + i;
+            f = seg_first[b];
+            c = seg_count[b];
+            const int q = bsys[b];
+            const double half = ldexp(roots[q].half, -l);
+            split = c > cap && !(2.0 * half <= eps_floor[q]) && l < kMaxDepth;
+        }
+        int lo = f;
+        if (split && o > 0) {
+            int hi = f + c;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (level_digit(khi[mid], klo[mid], l + 1) < o) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        const int up = __shfl_down(lo, 1, 8);
+        const int end = (o == 7) ? f + c : up;
+        int nonempty = (split && end > lo) ? 1 : 0;
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1) nonempty += __shfl_xor(nonempty, m, 8);
+        if (!live) continue;
+        int* bd = bnd + 9 * (size_t)i;
+        if (split) {
+            bd[o] = lo;
+            if (o == 7) bd[8] = f + c;
+        }
+        if (o == 0) nch[i] = split ? nonempty : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kT) k_emit_dev(const int* __restrict__ lev, int l, int maxcnt, int capacity,
+                                                const int* bnd, const int* nch, const int* cbase, int* seg_first,
+                                                int* seg_count, int* level, int* parent, int* octant,
+                                                int* child_first, int* nchild, unsigned char* mask, int* bsys,
+                                                int* overflow) {
+    const int off = lev[l], cnt = level_count(lev, l, maxcnt, capacity), next_off = lev[l + 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && cnt != lev[l + 1] - off) atomicOr(overflow, 2);
+    for (int i = blockIdx.x * kT + threadIdx.x; i < cnt; i += gridDim.x * kT) {
+        const int b = off + i;
+        const int k = nch[i];
+        nchild[b] = k;
+        if (k == 0) {
+            child_first[b] = -1;
+            mask[b] = 0;
+            continue;
+        }
+        const int* bd = bnd + 9 * (size_t)i;
+        const int first = next_off + cbase[i];
+        child_first[b] = first;
+        unsigned char mk = 0;
+        int r = 0;
+        for (int o = 0; o < 8; ++o) {
+            if (bd[o + 1] <= bd[o]) continue;
+            const int c = first + r++;
+            mk |= (unsigned char)(1u << o);
+            if (c >= capacity) {
+                atomicOr(overflow, 1);
+                continue;
+            }
+            seg_first[c] = bd[o];
+            seg_count[c] = bd[o + 1] - bd[o];
+            level[c] = l + 1;
+            parent[c] = b;
+            octant[c] = o;
+            bsys[c] = bsys[b];
+        }
+        mask[b] = mk;
+    }
+}
+
+// Boxes [off, off + cnt) of the last level can no longer split: no children.
+__global__ void __launch_bounds__(kT) k_fill_leaf_range(int off, int cnt, int* nchild, int* child_first,
+                                                       unsigned char* mask) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= cnt) return;
+    nchild[off + i] = 0;
+    child_first[off + i] = -1;
+    mask[off + i] = 0;
+}
+
+__global__ void k_level_init(int* lev, int nsys, int* overflow) {
+    if (threadIdx.x == 0) {
+        lev[0] = 0;
+        lev[1] = nsys;
+        *overflow = 0;
+    }
 }
 
 // Box q of level 0 is the root of system q (bodies [sys_off[q], sys_off[q+1])).
@@ -334,6 +457,53 @@ __global__ void __launch_bounds__(kT) k_lo_keys(int nb, const int* level, const 
     khi[b] = uint64_t(level[b] * 8 + octant[b]);
     klo[b] = 0;
     val[b] = b;
+}
+// M2L target order: (octant, level, BFS). A box's separated offsets depend on
+// its octant (not its level), so tiles cut inside one octant group share one
+// set of at most 189 offsets.
+__global__ void __launch_bounds__(kT) k_m2l_keys(int nb, const int* level, const int* octant, uint64_t* khi,
+                                                uint32_t* klo, int* val) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    khi[b] = uint64_t(octant[b] * 32 + level[b]);  // level <= kMaxDepth < 32
+    klo[b] = 0;
+    val[b] = b;
+}
+// Boxes per octant (one block, fixed order of the per-thread partials).
+__global__ void __launch_bounds__(kT) k_octant_count(int nb, const int* octant, int* cnt) {
+    __shared__ int c[8][kT];
+    int my[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += kT) {
+        const int o = octant[b];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) my[q] += (o == q);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q][threadIdx.x] = my[q];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        int t = 0;
+        for (int i = 0; i < kT; ++i) t += c[threadIdx.x][i];
+        cnt[threadIdx.x] = t;
+    }
+}
+// M2L tile starts over m2l_order (cnt = boxes per octant): each octant group
+// split into the fewest near-equal tiles of at most T targets. The host
+// derives the tile count from the same counts with m2l_tile_count().
+__host__ __device__ inline int m2l_tiles_of(int cnt, int T) { return (cnt + T - 1) / T; }
+__global__ void k_m2l_tile_plan(const int* cnt, int T, int* tstart) {
+    if (threadIdx.x != 0) return;
+    int pos = 0, k0 = 0;
+    for (int q = 0; q < 8; ++q) {
+        const int c = cnt[q];
+        if (c <= 0) continue;
+        const int k = m2l_tiles_of(c, T), base = c / k, rem = c % k;
+        for (int i = 0; i < k; ++i) {
+            tstart[k0++] = pos;
+            pos += base + (i < rem ? 1 : 0);
+        }
+    }
+    tstart[k0] = pos;
 }
 __global__ void __launch_bounds__(kT) k_nonleaf_ids(int nb, const int* nchild, const int* leaf_scan, int* ids) {
     const int b = blockIdx.x * kT + threadIdx.x;
@@ -829,7 +999,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         sb.val[k] = pool.get<int>(k ? "val1" : "val0", n);
     }
     sb.hist = pool.get<int>("sort_hist", 512 * (size_t)nsb);
-    const int scan_cap = std::max(n, 512 * nsb) + 1;
+    int scan_cap = std::max(n, 512 * nsb) + 1;
     int* scan_tmp = pool.get<int>("scan_tmp", 2 * scan_tmp_ints(scan_cap) + 16);
     sb.scan_tmp = scan_tmp;
     hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kT), 0, s, d_pos, n, nsys, d_sys_off, d_roots, sb.khi[0],
@@ -862,20 +1032,72 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         tv.child_mask = pool.get<unsigned char>("child_mask", c, s, keep);
         tv.bsys = pool.get<int>("bsys", c, s, keep);
     };
+    // Levels 0..L-1 may split. Run them all without host round trips when the
+    // box arrays can hold the worst case (every level split as far as the
+    // bodies allow); otherwise, or on overflow, fall back to the loop that
+    // reads each level's size back.
+    size_t bound = (size_t)nsys;
+    for (int lv = 0; lv < L; ++lv) {
+        const double full = (double)nsys * std::pow(8.0, lv);
+        bound += 8 * (size_t)std::min<double>(full, (double)n / (cap + 1) + 1.0);
+    }
+    const bool device_levels = L > 0 && bound <= (size_t)8 * n + (1u << 20) && bound < (size_t)INT_MAX / 2;
     cap_boxes = std::max(cap_boxes, (size_t)nsys + 1024);
+    if (device_levels) cap_boxes = std::max(cap_boxes, bound);
     box_arrays(cap_boxes, false);
     hipLaunchKernelGGL(k_init_roots, dim3((nsys + 255) / 256), dim3(256), 0, s, nsys, d_sys_off, tv.seg_first,
                        tv.seg_count, tv.level, tv.parent, tv.octant, tv.bsys);
     int* d_total = pool.get<int>("total", 4);
+    int l = 0;
+    bool done = false;
+    {
+        if (device_levels) {
+            const int maxcnt = std::min<size_t>(bound, (size_t)n + nsys);
+            int* lev = pool.get<int>("level_dev", L + 3);
+            int* bnd = pool.get<int>("split_bnd", 9 * (size_t)maxcnt);
+            int* nch = pool.get<int>("split_nch", maxcnt);
+            int* cbase = pool.get<int>("split_base", maxcnt);
+            int* ovf = lev + L + 2;
+            int* stmp = pool.get<int>("level_scan_tmp", scan_tmp_ints(maxcnt) + 16);
+            hipLaunchKernelGGL(k_level_init, dim3(1), dim3(64), 0, s, lev, nsys, ovf);
+            const int g8 = std::max(1, std::min(4096, (int)((8LL * maxcnt + kT - 1) / kT)));
+            const int g1 = std::max(1, std::min(4096, (int)((maxcnt + kT - 1) / kT)));
+            for (int lv = 0; lv < L; ++lv) {
+                hipLaunchKernelGGL(k_split_dev, dim3(g8), dim3(kT), 0, s, lev, lv, maxcnt, (int)cap_boxes, cap,
+                                   d_eps_floor, d_roots, tv.bsys, tv.seg_first, tv.seg_count, khi, klo, bnd, nch);
+                scan_exclusive_range(nch, cbase, lev + lv, maxcnt, lev + lv + 2, stmp, s);
+                hipLaunchKernelGGL(k_emit_dev, dim3(g1), dim3(kT), 0, s, lev, lv, maxcnt, (int)cap_boxes, bnd, nch, cbase,
+                                   tv.seg_first, tv.seg_count, tv.level, tv.parent, tv.octant, tv.child_first,
+                                   tv.nchild, tv.child_mask, tv.bsys, ovf);
+            }
+            std::vector<int> hl(L + 3);
+            AFMM_CHECK(hipMemcpyAsync(hl.data(), lev, sizeof(int) * (L + 3), hipMemcpyDeviceToHost, s));
+            AFMM_CHECK(hipStreamSynchronize(s));
+            if (hl[L + 2] == 0) {
+                // level L boxes (if any) are leaves
+                hipLaunchKernelGGL(k_fill_leaf_range, dim3(blocks(hl[L + 1] - hl[L])), dim3(kT), 0, s, hl[L],
+                                   hl[L + 1] - hl[L], tv.nchild, tv.child_first, tv.child_mask);
+                meta.level_off.assign(hl.begin(), hl.begin() + L + 2);
+                while (meta.level_off.size() > 2 && meta.level_off[meta.level_off.size() - 1] ==
+                                                         meta.level_off[meta.level_off.size() - 2])
+                    meta.level_off.pop_back();
+                l = (int)meta.level_off.size() - 2;
+                done = true;
+            }
+        }
+    }
+    if (!done) {
+    hipLaunchKernelGGL(k_init_roots, dim3((nsys + 255) / 256), dim3(256), 0, s, nsys, d_sys_off, tv.seg_first,
+                       tv.seg_count, tv.level, tv.parent, tv.octant, tv.bsys);
     meta.level_off.assign(1, 0);
     meta.level_off.push_back(nsys);
-    int l = 0;
+    l = 0;
     for (;; ++l) {
         const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
         int* bnd = pool.get<int>("split_bnd", 9 * (size_t)cnt);
         int* nch = pool.get<int>("split_nch", cnt);
         int* cbase = pool.get<int>("split_base", cnt);
-        hipLaunchKernelGGL(k_split, dim3(blocks(cnt)), dim3(kT), 0, s, off, cnt, l, cap, d_eps_floor, d_roots,
+        hipLaunchKernelGGL(k_split, dim3(blocks(8LL * cnt)), dim3(kT), 0, s, off, cnt, l, cap, d_eps_floor, d_roots,
                            tv.bsys, tv.seg_first, tv.seg_count, khi, klo, bnd, nch);
         scan_exclusive(nch, cbase, cnt, d_total, scan_tmp, s);
         const int next = read_int(d_total, s);
@@ -890,9 +1112,17 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         if (next == 0) break;
         meta.level_off.push_back(next_off + next);
     }
+    }
     tv.max_level = l;
     const int nb = meta.level_off.back();
     tv.nb = nb;
+    // scans over boxes from here on: chains of single-child boxes (near
+    // duplicates, small capacities) can hold more boxes than bodies
+    if (nb + 1 > scan_cap) {
+        scan_cap = nb + 1;
+        scan_tmp = pool.get<int>("scan_tmp", 2 * scan_tmp_ints(scan_cap) + 16);
+        sb.scan_tmp = scan_tmp;
+    }
 
     // ---- reference box ids
     int* cnt_start = pool.get<int>("cnt_start", n + 1);
@@ -959,19 +1189,41 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         const int r = radix_sort(lb, nb, 1, 1, s);
         tv.lo_order = pool.get<int>("lo_order", nb);
         AFMM_CHECK(hipMemcpyAsync(tv.lo_order, lb.val[r], sizeof(int) * nb, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_m2l_keys, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.level, tv.octant, lb.khi[0],
+                           lb.klo[0], lb.val[0]);
+        const int r2 = radix_sort(lb, nb, 1, 1, s);
+        tv.m2l_order = pool.get<int>("m2l_order", nb);
+        AFMM_CHECK(hipMemcpyAsync(tv.m2l_order, lb.val[r2], sizeof(int) * nb, hipMemcpyDeviceToDevice, s));
         tv.nonleaf_ids = pool.get<int>("nonleaf_ids", nb);
         hipLaunchKernelGGL(k_nonleaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.nonleaf_ids);
         const int nlev = (int)meta.level_off.size();
         std::vector<int> bounds(meta.level_off), got(nlev);
         int* d_idx = pool.get<int>("lvl_idx", nlev);
-        int* d_out = pool.get<int>("lvl_out", nlev);
+        int* d_out = pool.get<int>("lvl_out", nlev + 8);
         AFMM_CHECK(hipMemcpyAsync(d_idx, bounds.data(), sizeof(int) * nlev, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_idx, nlev, nb, d_out);
-        AFMM_CHECK(hipMemcpyAsync(got.data(), d_out, sizeof(int) * nlev, hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_octant_count, dim3(1), dim3(kT), 0, s, nb, tv.octant, d_out + nlev);
+        got.resize(nlev + 8);
+        AFMM_CHECK(hipMemcpyAsync(got.data(), d_out, sizeof(int) * (nlev + 8), hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipStreamSynchronize(s));
         meta.nonleaf_off.resize(nlev);
         for (int l = 0; l < nlev; ++l)
             meta.nonleaf_off[l] = bounds[l] - (bounds[l] < nb ? got[l] : tv.nleaves);
+        // M2L tiles (tile starts built on the device from the same counts)
+        int tot = 0;
+        tv.m2l_ntiles256 = tv.m2l_ntiles128 = 0;
+        for (int q = 0; q < 8; ++q) {
+            const int c = got[nlev + q];
+            if (c <= 0) continue;
+            tot += c;
+            tv.m2l_ntiles256 += m2l_tiles_of(c, 256);
+            tv.m2l_ntiles128 += m2l_tiles_of(c, 128);
+        }
+        if (tot != nb) throw std::runtime_error("octant counts do not cover the boxes");
+        tv.m2l_tiles256 = pool.get<int>("m2l_tiles256", tv.m2l_ntiles256 + 1);
+        tv.m2l_tiles128 = pool.get<int>("m2l_tiles128", tv.m2l_ntiles128 + 1);
+        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(64), 0, s, d_out + nlev, 256, tv.m2l_tiles256);
+        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(64), 0, s, d_out + nlev, 128, tv.m2l_tiles128);
     }
 
     // ---- colleagues, V, M2L source table
@@ -988,7 +1240,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     }
     if (read_int(overflow, s) != 0) throw std::runtime_error("interaction-list slot overflow");
 
-    // ---- U and W (count, scan, fill), X transpose
+    // ---- U and W per leaf, X per box (count, scan, one size readback, fill)
     const int nl = tv.nleaves;
     int* ucnt = pool.get<int>("ucnt", nl + 1);
     int* wcnt = pool.get<int>("wcnt", nl + 1);
@@ -1000,16 +1252,25 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
                        nullptr, nullptr, nullptr);
     scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
     scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp + scan_tmp_ints(scan_cap), s);
+    // X counts too, so that one round trip returns all three list sizes
+    int* xcnt = pool.get<int>("xcnt", nb + 1);
+    tv.x_off = pool.get<int>("x_off", nb + 1);
+    hipLaunchKernelGGL(k_xlist, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid, tv.cfp,
+                       tv.half, tv.coll, tv.ncoll, 0, xcnt, nullptr, nullptr);
+    scan_exclusive(xcnt, tv.x_off, nb, tv.x_off + nb, scan_tmp, s);
     {
-        int tot[2];
+        int tot[3];
         AFMM_CHECK(hipMemcpyAsync(&tot[0], tv.u_off + nl, sizeof(int), hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipMemcpyAsync(&tot[1], tv.w_off + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipMemcpyAsync(&tot[2], tv.x_off + nb, sizeof(int), hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipStreamSynchronize(s));
         tv.n_u = tot[0];
         tv.n_w = tot[1];
+        tv.n_x = tot[2];
     }
     tv.u_idx = pool.get<int>("u_idx", tv.n_u);
     tv.w_idx = pool.get<int>("w_idx", tv.n_w);
+    tv.x_idx = pool.get<int>("x_idx", tv.n_x);
     hipLaunchKernelGGL(k_uwlist, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
                        tv.child_first,
                        tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, 1, nullptr, nullptr, tv.u_off,
@@ -1018,14 +1279,6 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, tv.list_totals, 4);
     hipLaunchKernelGGL(k_list_totals, dim3(blocks(std::max(nb, nl))), dim3(kT), 0, s, nb, nl, tv.ncoll, tv.nv,
                        tv.leaf_ids, tv.seg_count, ucnt, tv.list_totals);
-
-    int* xcnt = pool.get<int>("xcnt", nb + 1);
-    tv.x_off = pool.get<int>("x_off", nb + 1);
-    hipLaunchKernelGGL(k_xlist, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid, tv.cfp,
-                       tv.half, tv.coll, tv.ncoll, 0, xcnt, nullptr, nullptr);
-    scan_exclusive(xcnt, tv.x_off, nb, tv.x_off + nb, scan_tmp, s);
-    tv.n_x = read_int(tv.x_off + nb, s);
-    tv.x_idx = pool.get<int>("x_idx", tv.n_x);
     hipLaunchKernelGGL(k_xlist, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid, tv.cfp,
                        tv.half, tv.coll, tv.ncoll, 1, nullptr, tv.x_off, tv.x_idx);
     AFMM_CHECK(hipGetLastError());

@@ -39,9 +39,10 @@ def systems(seed, count, nmin=1, nmax=6000):
     return out
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed_bf16"])
 def test_batch_equals_separate_runs(afmm, prec):
-    precision = afmm.PRECISION_FP64 if prec == "fp64" else afmm.PRECISION_MIXED
+    precision = {"fp64": afmm.PRECISION_FP64, "mixed": afmm.PRECISION_MIXED,
+                 "mixed_bf16": afmm.PRECISION_MIXED_BF16}[prec]
     sys_ = systems(7, 24)
     rng = np.random.default_rng(3)
     G = rng.uniform(0.5, 2.0, len(sys_))

@@ -3,7 +3,8 @@
   * afmm_direct (DirectSolver3D.hpp:15-34 on a target subsample, plus the
     softened potential of Kernel3D.hpp:39-42) equals the oracle's direct sums
     to 1e-12 relative;
-  * AFMM_PRECISION_MIXED builds the same tree as FP64 (the tree never depends
+  * AFMM_PRECISION_MIXED (split-FP32 M2L) and AFMM_PRECISION_MIXED_BF16
+    (three-piece bf16 M2L) build the same tree as FP64 (the tree never depends
     on precision) and its accelerations differ from the FP64 GPU result by
     FP32 rounding only: relative RMS <= 1e-6 and, with the reference's relErr
     floor (adaptive_gpu_harness.cu:36-46), max <= 1e-4;
@@ -46,20 +47,22 @@ def test_direct_matches_oracle(afmm):
         s.direct(pos, m, [len(m)])
 
 
+@pytest.mark.parametrize("mode", ["fp32split", "bf16split"])
 @pytest.mark.parametrize("p", [4, 8, 12])
-def test_mixed_vs_fp64_same_tree(afmm, p):
+def test_mixed_vs_fp64_same_tree(afmm, p, mode):
     orc = cabi.oracle()
     pos, _, m = orc.plummer(60000, seed=21 + p)
-    print(f"mixed vs fp64 plummer 60000 p={p}", flush=True)
+    print(f"mixed ({mode}) vs fp64 plummer 60000 p={p}", flush=True)
+    prec = afmm.PRECISION_MIXED if mode == "fp32split" else afmm.PRECISION_MIXED_BF16
     f64 = afmm.AdaptiveFMM3D(G=1.0, eps=0.01, p=p, threshold=64)
-    mix = afmm.AdaptiveFMM3D(G=1.0, eps=0.01, p=p, threshold=64, precision=afmm.PRECISION_MIXED)
+    mix = afmm.AdaptiveFMM3D(G=1.0, eps=0.01, p=p, threshold=64, precision=prec)
     a64 = f64.computeAccelerations(pos, m)
     amx = mix.computeAccelerations(pos, m)
     t64, tmx = f64.tree(), mix.tree()
     for k in ("orig_index", "pfirst", "pcount", "v_idx", "u_idx", "w_idx", "x_idx"):
         assert np.array_equal(t64[k], tmx[k]), k
     mx, rms = cabi.rel_err(a64, amx)
-    print(f"p={p} mixed vs fp64: max={mx:.3e} rms={rms:.3e}", flush=True)
+    print(f"p={p} {mode} vs fp64: max={mx:.3e} rms={rms:.3e}", flush=True)
     assert rms <= MIX_VS_FP64_RMS and mx <= MIX_VS_FP64_MAX
     # both against FP64 direct on a subsample
     idx = np.random.default_rng(p).choice(len(m), 1024, replace=False)
@@ -72,10 +75,12 @@ def test_mixed_vs_fp64_same_tree(afmm, p):
     assert np.array_equal(amx, mix.computeAccelerations(pos, m))
 
 
-def test_mixed_single_leaf_is_fp32_p2p(afmm):
+@pytest.mark.parametrize("mode", ["fp32split", "bf16split"])
+def test_mixed_single_leaf_is_fp32_p2p(afmm, mode):
     pos, m = cabi.uniform_cube(60, 7)
+    prec = afmm.PRECISION_MIXED if mode == "fp32split" else afmm.PRECISION_MIXED_BF16
     for eps in (0.0, 0.05):
-        mix = afmm.AdaptiveFMM3D(G=1.0, eps=eps, p=4, threshold=64, precision=afmm.PRECISION_MIXED)
+        mix = afmm.AdaptiveFMM3D(G=1.0, eps=eps, p=4, threshold=64, precision=prec)
         acc = mix.computeAccelerations(pos, m)
         ref = cabi.oracle().direct(pos, m, 1.0, eps)
         _, rms = cabi.rel_err(ref, acc)

@@ -139,3 +139,24 @@ def test_errors_and_edge_cases(afmm):
     pos[7, 1] = 0.25
     acc = s.computeAccelerations(pos, m)
     assert np.isfinite(acc).all()
+
+
+def test_many_more_boxes_than_bodies(afmm):
+    """Near-duplicate pairs with leaf capacity 1 chain down to the depth cap:
+    about ten boxes per body, so every scan over boxes runs past the sizes
+    derived from the body count. Tree bit-exact and forces vs the oracle."""
+    rng = np.random.default_rng(8)
+    base = rng.uniform(-1, 1, (10000, 3))
+    pos = np.concatenate([base, base + 1e-9])
+    m = rng.uniform(0.5, 1.5, len(pos))
+    print("near-duplicate pairs 20000 cap 1", flush=True)
+    orc = cabi.oracle()
+    s = solver(afmm, 2, 1, 0.0)
+    acc = s.computeAccelerations(pos, m)
+    t, tr = s.tree(), orc.tree(pos, m, 1, 0.0)
+    assert len(t["parent"]) > 5 * len(m)
+    for k in TREE_KEYS:
+        assert np.array_equal(t[k], tr[k]), k
+    ref, _ = orc.fmm(pos, m, 1.0, 0.0, 2, 1)
+    mx, _ = cabi.rel_err(ref, acc)
+    assert mx <= TOL
