@@ -354,7 +354,7 @@ def main():
             phases.append(solver.stats())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        keys = ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_leaf", "ms_total")
+        keys = ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_leaf", "ms_total", "ms_p2p")
         avg = {k[3:]: float(np.mean([ph[k] for ph in phases])) for k in keys}
         return dt / args.steps, avg, phases[-1], solver
 
@@ -435,6 +435,12 @@ def main():
                       "n_bodies": n, "p": args.p, "leaf_cap": args.cap, "eps": args.eps,
                       "precision": "fp64", "parallelism": "single GPU"},
            "roofline": roofline,
+           # the near field on the VALU, at the conventional 20 flop per softened
+           # pair interaction, against the FP64 peak
+           "p2p_valu": {"kernel": "k_p2p (FP64 VALU)", "pairs": st["p2p_pairs"], "ms": avg["p2p"],
+                        "achieved_tflops": 20.0 * st["p2p_pairs"] / (avg["p2p"] * 1e-3) / 1e12,
+                        "peak_tflops": PEAK_FP64_TFLOPS,
+                        "frac": 20.0 * st["p2p_pairs"] / (avg["p2p"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
            "phases_ms": avg,
            "mixed": mixed,
            "accuracy": accuracy,

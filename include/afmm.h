@@ -75,6 +75,7 @@ typedef struct {
     int32_t max_level;
     int32_t max_leaf_occ;                    /* AdaptiveOctree::maxLeafOcc (:191-195) */
     double ms_tree, ms_lists, ms_upward, ms_m2l, ms_downward, ms_leaf, ms_total; /* device time */
+    double ms_p2p; /* the P2P near-field kernel alone (part of ms_leaf) */
 } afmm_stats;
 
 /* Tree export, in the reference's box numbering and list orders

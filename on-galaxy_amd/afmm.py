@@ -44,7 +44,8 @@ class Stats(C.Structure):
                 ("n_x", C.c_int64), ("p2p_pairs", C.c_int64), ("max_level", C.c_int32),
                 ("max_leaf_occ", C.c_int32), ("ms_tree", C.c_double), ("ms_lists", C.c_double),
                 ("ms_upward", C.c_double), ("ms_m2l", C.c_double), ("ms_downward", C.c_double),
-                ("ms_leaf", C.c_double), ("ms_total", C.c_double)]
+                ("ms_leaf", C.c_double), ("ms_total", C.c_double),
+                ("ms_p2p", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

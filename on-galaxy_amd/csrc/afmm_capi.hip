@@ -99,6 +99,7 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     st.ms_downward = elapsed(c->ev[4], c->ev[5]);
     st.ms_leaf = elapsed(c->ev[5], c->ev[6]);
     st.ms_total = elapsed(c->ev[0], c->ev[6]);
+    st.ms_p2p = elapsed(c->ev[8], c->ev[6]);
     return AFMM_OK;
 }
 

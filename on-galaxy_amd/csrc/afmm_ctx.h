@@ -26,7 +26,7 @@ struct afmm_ctx {
     afmm::RootInfo root{};  // of system 0
     int nsys = 1;
     afmm_stats stats{};
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[9] = {};
 };
 
 namespace afmm {

@@ -469,23 +469,28 @@ __global__ void __launch_bounds__(kT) k_m2l_keys(int nb, const int* level, const
     klo[b] = 0;
     val[b] = b;
 }
-// Boxes per octant (one block, fixed order of the per-thread partials).
-__global__ void __launch_bounds__(kT) k_octant_count(int nb, const int* octant, int* cnt) {
-    __shared__ int c[8][kT];
-    int my[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nb; b += kT) {
-        const int o = octant[b];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) my[q] += (o == q);
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) c[q][threadIdx.x] = my[q];
+// Boxes per octant (one block of 1024 threads; integer sums, any order).
+constexpr int kOctCountThreads = 1024;
+__global__ void __launch_bounds__(kOctCountThreads) k_octant_count(int nb, const int* octant, int* cnt) {
+    __shared__ int c[8];
+    if (threadIdx.x < 8) c[threadIdx.x] = 0;
     __syncthreads();
-    if (threadIdx.x < 8) {
-        int t = 0;
-        for (int i = 0; i < kT; ++i) t += c[threadIdx.x][i];
-        cnt[threadIdx.x] = t;
+    int my[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b0 = 4 * (int)threadIdx.x; b0 < nb; b0 += 4 * kOctCountThreads) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (b0 + u < nb) {
+                const int o = octant[b0 + u];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) my[q] += (o == q);
+            }
+        }
     }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (my[q]) atomicAdd(&c[q], my[q]);
+    __syncthreads();
+    if (threadIdx.x < 8) cnt[threadIdx.x] = c[threadIdx.x];
 }
 // M2L tile starts over m2l_order (cnt = boxes per octant): each octant group
 // split into the fewest near-equal tiles of at most T targets. The host
@@ -816,6 +821,16 @@ __global__ void __launch_bounds__(64) k_uwlist(int nleaves, const int* __restric
             }
             ud[j2 + 1] = v;
         }
+    }
+}
+
+// The three list sizes and the slot-overflow flag, for one readback.
+__global__ void k_list_sizes(const int* nu, const int* nw, const int* nx, const int* overflow, int* out) {
+    if (threadIdx.x == 0) {
+        out[0] = *nu;
+        out[1] = *nw;
+        out[2] = *nx;
+        out[3] = *overflow;
     }
 }
 
@@ -1202,7 +1217,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         int* d_out = pool.get<int>("lvl_out", nlev + 8);
         AFMM_CHECK(hipMemcpyAsync(d_idx, bounds.data(), sizeof(int) * nlev, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_idx, nlev, nb, d_out);
-        hipLaunchKernelGGL(k_octant_count, dim3(1), dim3(kT), 0, s, nb, tv.octant, d_out + nlev);
+        hipLaunchKernelGGL(k_octant_count, dim3(1), dim3(kOctCountThreads), 0, s, nb, tv.octant, d_out + nlev);
         got.resize(nlev + 8);
         AFMM_CHECK(hipMemcpyAsync(got.data(), d_out, sizeof(int) * (nlev + 8), hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipStreamSynchronize(s));
@@ -1238,7 +1253,8 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         hipLaunchKernelGGL(k_coll_v, dim3(cnt), dim3(64), 0, s, off, d_roots, tv.bsys, tv.parent, tv.child_first,
                            tv.child_mask, tv.cfp, tv.half, tv.ic, tv.coll, tv.ncoll, tv.vl, tv.nv, overflow);
     }
-    if (read_int(overflow, s) != 0) throw std::runtime_error("interaction-list slot overflow");
+    // (the overflow flag is read back with the list sizes below; list counts
+    // are capped at their slots, so the kernels in between stay in bounds)
 
     // ---- U and W per leaf, X per box (count, scan, one size readback, fill)
     const int nl = tv.nleaves;
@@ -1259,11 +1275,13 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
                        tv.half, tv.coll, tv.ncoll, 0, xcnt, nullptr, nullptr);
     scan_exclusive(xcnt, tv.x_off, nb, tv.x_off + nb, scan_tmp, s);
     {
-        int tot[3];
-        AFMM_CHECK(hipMemcpyAsync(&tot[0], tv.u_off + nl, sizeof(int), hipMemcpyDeviceToHost, s));
-        AFMM_CHECK(hipMemcpyAsync(&tot[1], tv.w_off + nl, sizeof(int), hipMemcpyDeviceToHost, s));
-        AFMM_CHECK(hipMemcpyAsync(&tot[2], tv.x_off + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+        int tot[4];
+        int* d_tot = pool.get<int>("list_sizes", 4);
+        hipLaunchKernelGGL(k_list_sizes, dim3(1), dim3(64), 0, s, tv.u_off + nl, tv.w_off + nl, tv.x_off + nb, overflow,
+                           d_tot);
+        AFMM_CHECK(hipMemcpyAsync(tot, d_tot, sizeof(tot), hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipStreamSynchronize(s));
+        if (tot[3] != 0) throw std::runtime_error("interaction-list slot overflow");
         tv.n_u = tot[0];
         tv.n_w = tot[1];
         tv.n_x = tot[2];
