@@ -161,9 +161,11 @@ def large_leg(afmm, ics, torch, args):
 def batched_leg(afmm, ics, tune, torch, args):
     """256 independent systems (N log-uniform over 2k-200k; even: Plummer a=1,
     odd: Hernquist a=0.5 truncated at 20a; total mass 1 each), p=8,
-    eps=0.01, mixed precision, leaf capacity from the autotuner: one batched
+    eps=0.01, mixed precision (bf16-split M2L, FP32 P2P), leaf capacity from
+    the autotuner, FMM passes replayed as one HIP graph per step: one batched
     evaluation (one launch per FMM phase for all systems) against 256
-    sequential plain (FP64, leaf capacity 64) solves of the same systems."""
+    sequential plain (FP64, leaf capacity 64) solves of the same systems. The
+    same step with stream launches is reported beside it."""
     rng = np.random.default_rng(args.seed + 3)
     ns = np.exp(rng.uniform(np.log(2000), np.log(200000), args.batch_systems)).astype(np.int64)
     parts = []
@@ -180,10 +182,11 @@ def batched_leg(afmm, ics, tune, torch, args):
     d_acc = torch.empty_like(d_pos)
     P, E = d_pos.data_ptr(), d_mass.data_ptr()
     A = d_acc.data_ptr()
-    cap, tune_ms = tune.tune_leaf_cap(afmm, off, P, E, A, 8, afmm.PRECISION_MIXED, eps=0.01)
+    cap, tune_ms = tune.tune_leaf_cap(afmm, off, P, E, A, 8, afmm.PRECISION_MIXED_BF16, eps=0.01)
 
-    def timed_batch(precision, leaf_cap, reps=3):
+    def timed_batch(precision, leaf_cap, reps=3, graph=False):
         s = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=leaf_cap, precision=precision)
+        s.set_graph(graph)
         s.compute_batch_device(off, P, E, A)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -192,7 +195,8 @@ def batched_leg(afmm, ics, tune, torch, args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps, s.stats()
 
-    t_mix, st = timed_batch(afmm.PRECISION_MIXED, cap)
+    t_mix, st = timed_batch(afmm.PRECISION_MIXED_BF16, cap, graph=True)
+    t_mix_stream, _ = timed_batch(afmm.PRECISION_MIXED_BF16, cap)
     t_b64, _ = timed_batch(afmm.PRECISION_FP64, 64)
     # 256 sequential plain solves (FP64, cap 64), device-resident slices
     s = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=64)
@@ -207,7 +211,10 @@ def batched_leg(afmm, ics, tune, torch, args):
     inter = float(np.sum(ns.astype(np.float64) * (ns - 1)))
     return {"systems": int(len(ns)), "bodies": int(off[-1]), "p": 8, "eps": 0.01,
             "leaf_cap": int(cap), "autotune_ms_total_by_cap": {str(k): round(v, 3) for k, v in tune_ms.items()},
-            "batched_mixed_s_per_step": t_mix, "batched_fp64_cap64_s_per_step": t_b64,
+            "precision": "mixed_bf16", "graph": {"instantiations": int(st["graph_instantiations"]),
+                                                 "in_place_updates": int(st["graph_updates"])},
+            "batched_mixed_s_per_step": t_mix, "batched_mixed_stream_launch_s_per_step": t_mix_stream,
+            "batched_fp64_cap64_s_per_step": t_b64,
             "sequential_plain_s_per_step": t_seq,
             "body_interactions_per_s": inter / t_mix,
             "sequential_body_interactions_per_s": inter / t_seq,

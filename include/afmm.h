@@ -76,6 +76,9 @@ typedef struct {
     int32_t max_leaf_occ;                    /* AdaptiveOctree::maxLeafOcc (:191-195) */
     double ms_tree, ms_lists, ms_upward, ms_m2l, ms_downward, ms_leaf, ms_total; /* device time */
     double ms_p2p; /* the P2P near-field kernel alone (part of ms_leaf) */
+    /* graph mode (afmm_set_graph): executable graphs built and updated in
+     * place so far; in graph mode only ms_tree and ms_total are timed */
+    int64_t graph_instantiations, graph_updates;
 } afmm_stats;
 
 /* Tree export, in the reference's box numbering and list orders
@@ -253,6 +256,14 @@ int afmm_eval_device_potential(afmm_ctx* ctx, int64_t n, const double* d_pos_xyz
 
 /* Block until the context's stream is idle. */
 int afmm_synchronize(afmm_ctx* ctx);
+
+/* Graph mode (no counterpart in the reference, which launches from the host
+ * per pass): when enabled, each evaluation captures its FMM passes (P2M
+ * through P2P, after the tree build, which needs host round trips) into a HIP
+ * graph and launches that; an evaluation whose passes have the same shape as
+ * the previous one updates the executable graph in place. Results are the
+ * same as without it. */
+int afmm_set_graph(afmm_ctx* ctx, int enable);
 
 /* ---- introspection ------------------------------------------------------ */
 int afmm_get_stats(const afmm_ctx* ctx, afmm_stats* out);

@@ -45,7 +45,7 @@ class Stats(C.Structure):
                 ("max_leaf_occ", C.c_int32), ("ms_tree", C.c_double), ("ms_lists", C.c_double),
                 ("ms_upward", C.c_double), ("ms_m2l", C.c_double), ("ms_downward", C.c_double),
                 ("ms_leaf", C.c_double), ("ms_total", C.c_double),
-                ("ms_p2p", C.c_double)]
+                ("ms_p2p", C.c_double), ("graph_instantiations", C.c_int64), ("graph_updates", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -126,6 +126,7 @@ def library():
         L.afmm_server_destroy.argtypes = [C.c_void_p]
         L.afmm_direct.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64, C.POINTER(C.c_int64), _dp, _dp]
         L.afmm_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+        L.afmm_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.afmm_tree_export.argtypes = [C.c_void_p, C.POINTER(Tree)]
         _lib = L
     return _lib
@@ -242,6 +243,11 @@ class AdaptiveFMM3D:
 
     def synchronize(self):
         self._check(self.lib.afmm_synchronize(self.ctx))
+
+    def set_graph(self, enable=True):
+        """Graph mode: the FMM passes of each evaluation are captured into a
+        HIP graph and launched from it (results unchanged)."""
+        self._check(self.lib.afmm_set_graph(self.ctx, 1 if enable else 0))
 
     def stats(self):
         st = Stats()

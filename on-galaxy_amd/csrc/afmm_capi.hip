@@ -70,7 +70,44 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     tv.sys_eps2 = d_ge + nsys;
     AFMM_CHECK(hipEventRecord(c->ev[1], s));
     FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw, c->d_m2l_hl, c->d_m2l_bf};
-    run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2);
+    if (c->use_graph) {
+        // size every buffer first, so that the capture allocates nothing
+        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr, true);
+        hipGraph_t g = nullptr;
+        AFMM_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr);
+        } catch (...) {
+            if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        AFMM_CHECK(hipStreamEndCapture(s, &g));
+        bool updated = false;
+        if (c->gexec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(c->gexec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(c->gexec);
+                c->gexec = nullptr;
+            }
+        }
+        if (!c->gexec) {
+            const hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                c->gexec = nullptr;
+                throw HipError(e, "hipGraphInstantiate");
+            }
+        }
+        AFMM_CHECK(hipGraphDestroy(g));
+        AFMM_CHECK(hipGraphLaunch(c->gexec, s));
+        AFMM_CHECK(hipEventRecord(c->ev[6], s));
+        (updated ? c->graph_updates : c->graph_instantiations) += 1;
+    } else {
+        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2);
+    }
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
     c->meta = meta;
@@ -93,13 +130,17 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     st.max_leaf_occ = (int32_t)tot[2];
     st.p2p_pairs = (int64_t)tot[3] - n;
     st.ms_tree = elapsed(c->ev[0], c->ev[1]);
-    st.ms_upward = elapsed(c->ev[2], c->ev[3]);
-    st.ms_lists = elapsed(c->ev[3], c->ev[7]);  // M2L ordering / source table
-    st.ms_m2l = elapsed(c->ev[7], c->ev[4]);
-    st.ms_downward = elapsed(c->ev[4], c->ev[5]);
-    st.ms_leaf = elapsed(c->ev[5], c->ev[6]);
     st.ms_total = elapsed(c->ev[0], c->ev[6]);
-    st.ms_p2p = elapsed(c->ev[8], c->ev[6]);
+    if (!c->use_graph) {  // the graph has no per-phase events
+        st.ms_upward = elapsed(c->ev[2], c->ev[3]);
+        st.ms_lists = elapsed(c->ev[3], c->ev[7]);  // M2L ordering / source table
+        st.ms_m2l = elapsed(c->ev[7], c->ev[4]);
+        st.ms_downward = elapsed(c->ev[4], c->ev[5]);
+        st.ms_leaf = elapsed(c->ev[5], c->ev[6]);
+        st.ms_p2p = elapsed(c->ev[8], c->ev[6]);
+    }
+    st.graph_instantiations = c->graph_instantiations;
+    st.graph_updates = c->graph_updates;
     return AFMM_OK;
 }
 
@@ -194,9 +235,16 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
     return AFMM_OK;
 }
 
+int afmm_set_graph(afmm_ctx* c, int enable) {
+    if (!c) return AFMM_ERR_ARG;
+    c->use_graph = enable != 0;
+    return AFMM_OK;
+}
+
 void afmm_destroy(afmm_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     c->pool.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);

@@ -27,6 +27,12 @@ struct afmm_ctx {
     int nsys = 1;
     afmm_stats stats{};
     hipEvent_t ev[9] = {};
+    // FMM passes replayed as a HIP graph (afmm_set_graph): the executable
+    // graph of the last evaluation, updated in place when the next capture has
+    // the same shape, re-instantiated otherwise
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    int64_t graph_instantiations = 0, graph_updates = 0;
 };
 
 namespace afmm {

@@ -196,9 +196,10 @@ constexpr int kPrecFP64 = 0, kPrecMixed = 1, kPrecMixedBf16 = 2;
 // tv.sys_eps2). prec kPrecMixed: M2L as split-FP32 MFMA; kPrecMixedBf16: M2L
 // as three-piece bf16 MFMA; both with P2P in FP32 and FP64 accumulation;
 // everything else FP64.
+// ev: 7 events or null. reserve_only: take every buffer from the pool, launch nothing.
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
              const FmmTables& tab, int prec, double* d_acc, double* d_phi,
-             hipEvent_t* ev /* 6 or null */);
+             hipEvent_t* ev, bool reserve_only = false);
 
 // ---- integrate.hip -----------------------------------------------------------
 // One system of a leapfrog step: its resident arrays and step settings.

@@ -897,15 +897,18 @@ __global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict
 // else 1) of a lane group of T lanes (T = ceil(n_leaf / R), 8 <= T <= 64);
 // the wave's G = floor(64/T) groups take the tile's sources g, g+G, ... and
 // their partial sums are reduced in a fixed order (deterministic). The kernel
-// is (|d|^2 + eps^2)^{-3/2} from v_rsq_f64 refined by two Newton steps; the
+// is (|d|^2 + eps^2)^{-3/2} from v_rsq_f64 refined by one Newton step; the
 // self pair is skipped by index, as in the reference (branch-free: its
 // distance is replaced by 1 and its mass by 0).
+//
+// v_rsq_f64 is accurate to 2^-24.2 relative on gfx950 (measured over 2^20
+// arguments spanning 2^-20 .. 2^20); one Newton step brings that to 4.1e-15
+// (a second one to 2.4e-16). The per-pair error after one step is far below
+// the 1e-12 agreement asked of the FP64 forces, so the second step (4 FP64
+// operations per pair) is not spent.
 __device__ __forceinline__ double rsqrt_f64(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double hx = 0.5 * x;
-    y = y * (1.5 - hx * y * y);
-    y = y * (1.5 - hx * y * y);
-    return y;
+    const double y = __builtin_amdgcn_rsq(x);
+    return y * (1.5 - (0.5 * x) * y * y);
 }
 
 template <bool POT>
@@ -1064,9 +1067,12 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict
     else p2p_leaf<MIX, POT, 1>(t, b, li, far, acc, fpot, phi, sl, sx, sy, sz, sm, sj, red);
 }
 
+// All device buffers are taken from the pool before the first launch, so that
+// a call with reserve_only (no launches) sizes them and a following call under
+// stream capture allocates nothing.
 template <int P>
 void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, int prec,
-         double* acc, double* phi, hipEvent_t* ev) {
+         double* acc, double* phi, hipEvent_t* ev, bool reserve_only) {
     const bool mixed = prec != kPrecFP64;
     constexpr int S = nterms(P);
     const int nb = t.nb;
@@ -1074,6 +1080,18 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     double* L = pool.get<double>("Lexp", (size_t)nb * S);
     double* far = pool.get<double>("far", 3 * (size_t)t.n);
     double* fpot = phi ? pool.get<double>("farpot", t.n) : nullptr;
+    // M2L targets in (octant, level) order, tiles cut inside octant groups
+    const int TILE = prec == kPrecMixedBf16 ? kM2LBfTile : m2l_tile<P>();
+    if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
+    const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
+    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
+    const int np = nb;
+    int* rank = pool.get<int>("m2l_rank", nb);
+    int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
+    unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
+    uint4* Mbf = prec == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
+                                        : nullptr;
+    if (reserve_only) return;
     if (ev) AFMM_CHECK(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL((k_accumulate<P, 0>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M);
     for (int l = t.max_level - 1; l >= 0; --l) {
@@ -1086,25 +1104,14 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (ev) AFMM_CHECK(hipEventRecord(ev[1], s));
     {
         // M2L targets in (level, octant) order; transposed source table; tile plan
-        // M2L targets in (octant, level) order, tiles cut inside octant groups
-        const int TILE = prec == kPrecMixedBf16 ? kM2LBfTile : m2l_tile<P>();
-        const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
-        const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
-        if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
-        const int np = nb;
-        int* rank = pool.get<int>("m2l_rank", nb);
-        int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
-        unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
         hipLaunchKernelGGL(k_rank_of, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.m2l_order, rank);
         hipLaunchKernelGGL(k_fill_int, dim3(blocks((long long)kNumOffsets * np)), dim3(kT), 0, s, srcT,
                            (size_t)kNumOffsets * np, -1);
         hipLaunchKernelGGL(k_m2l_src, dim3(blocks((long long)nb * kVSlots)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv,
                            t.ic, srcT);
         hipLaunchKernelGGL(k_m2l_plan, dim3(ntiles * kNumOffsets), dim3(64), 0, s, tstart, np, srcT, present);
-        uint4* Mbf = nullptr;
         if (prec == kPrecMixedBf16) {
             constexpr int KB = m2l_bf_kb(S);
-            Mbf = pool.get<uint4>("m2l_Mbf", (size_t)nb * KB * kBfPieces * 4);
             hipLaunchKernelGGL(k_split_bf16<P>, dim3(blocks((long long)nb * KB * 4)), dim3(kT), 0, s, nb, M, Mbf);
         }
         if (ev) AFMM_CHECK(hipEventRecord(ev[5], s));
@@ -1150,12 +1157,12 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
 }  // namespace
 
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
-             int prec, double* d_acc, double* d_phi, hipEvent_t* ev) {
+             int prec, double* d_acc, double* d_phi, hipEvent_t* ev, bool reserve_only) {
     if (prec == kPrecMixed && !tab.m2l_hl) throw std::runtime_error("split FP32 M2L operators not loaded");
     if (prec == kPrecMixedBf16 && !tab.m2l_bf) throw std::runtime_error("bf16 M2L operators not loaded");
     switch (p) {
 #define AFMM_CASE(PP) \
-    case PP: run<PP>(pool, s, tv, meta, tab, prec, d_acc, d_phi, ev); return;
+    case PP: run<PP>(pool, s, tv, meta, tab, prec, d_acc, d_phi, ev, reserve_only); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
 #undef AFMM_CASE

@@ -55,7 +55,7 @@ __device__ __forceinline__ void take_hi(Ext& a, Ext b) {
     if (b.v > a.v || (b.v == a.v && b.i < a.i)) a = b;
 }
 
-constexpr int kBBoxChunk = 8192;
+constexpr int kBBoxChunk = 2048;  // about 500 blocks at 1M bodies
 
 // One block per chunk [cbeg[c], cend[c]) of one system's bodies.
 __global__ void __launch_bounds__(kT) k_bbox_partial(const double* pos, const double* mass,
@@ -411,11 +411,6 @@ __global__ void k_fill(int* a, int n, int v) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i < n) a[i] = v;
 }
-__global__ void k_fill_u64(unsigned long long* a, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = 0ull;
-}
-
 __global__ void __launch_bounds__(kT) k_count_starts(int nb, const int* seg_first, const int* level,
                                                     int* cnt, int* minlev) {
     const int b = blockIdx.x * kT + threadIdx.x;
@@ -487,8 +482,11 @@ __global__ void __launch_bounds__(kOctCountThreads) k_octant_count(int nb, const
         }
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-        if (my[q]) atomicAdd(&c[q], my[q]);
+    for (int q = 0; q < 8; ++q) {
+        int v = my[q];
+        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&c[q], v);
+    }
     __syncthreads();
     if (threadIdx.x < 8) cnt[threadIdx.x] = c[threadIdx.x];
 }
@@ -496,19 +494,30 @@ __global__ void __launch_bounds__(kOctCountThreads) k_octant_count(int nb, const
 // split into the fewest near-equal tiles of at most T targets. The host
 // derives the tile count from the same counts with m2l_tile_count().
 __host__ __device__ inline int m2l_tiles_of(int cnt, int T) { return (cnt + T - 1) / T; }
-__global__ void k_m2l_tile_plan(const int* cnt, int T, int* tstart) {
-    if (threadIdx.x != 0) return;
-    int pos = 0, k0 = 0;
-    for (int q = 0; q < 8; ++q) {
-        const int c = cnt[q];
-        if (c <= 0) continue;
-        const int k = m2l_tiles_of(c, T), base = c / k, rem = c % k;
-        for (int i = 0; i < k; ++i) {
-            tstart[k0++] = pos;
-            pos += base + (i < rem ? 1 : 0);
+__global__ void __launch_bounds__(kT) k_m2l_tile_plan(const int* cnt, int T, int ntiles, int* tstart) {
+    __shared__ int c[8], first_tile[9], first_box[9];
+    if (threadIdx.x == 0) {
+        int k = 0, pos = 0;
+        for (int q = 0; q < 8; ++q) {
+            c[q] = cnt[q];
+            first_tile[q] = k;
+            first_box[q] = pos;
+            if (c[q] > 0) k += m2l_tiles_of(c[q], T), pos += c[q];
         }
+        first_tile[8] = k;
+        first_box[8] = pos;
     }
-    tstart[k0] = pos;
+    __syncthreads();
+    for (int t = threadIdx.x; t <= ntiles; t += kT) {
+        if (t == ntiles) {
+            tstart[t] = first_box[8];
+            continue;
+        }
+        int q = 0;
+        while (t >= first_tile[q + 1]) ++q;
+        const int k = m2l_tiles_of(c[q], T), base = c[q] / k, rem = c[q] % k, i = t - first_tile[q];
+        tstart[t] = first_box[q] + i * base + (i < rem ? i : rem);
+    }
 }
 __global__ void __launch_bounds__(kT) k_nonleaf_ids(int nb, const int* nchild, const int* leaf_scan, int* ids) {
     const int b = blockIdx.x * kT + threadIdx.x;
@@ -530,14 +539,15 @@ __global__ void __launch_bounds__(kT) k_leaf_ids(int nb, const int* nchild, cons
     if (b < nb && nchild[b] == 0) ids[scan[b]] = b;
 }
 // key2[input index] = first sorted slot of its leaf; leaf_of[sorted slot] = leaf box.
+// One wave per leaf (lanes over its bodies).
 __global__ void __launch_bounds__(kT) k_leaf_keys(int nleaves, const int* leaf_ids, const int* seg_first,
                                                  const int* seg_count, const int* perm1, uint64_t* key2,
                                                  uint32_t* klo2, int* val2, int* leaf_of) {
-    const int li = blockIdx.x * kT + threadIdx.x;
+    const int li = blockIdx.x * (kT / 64) + (int)(threadIdx.x >> 6);
     if (li >= nleaves) return;
     const int b = leaf_ids[li];
     const int f = seg_first[b], c = seg_count[b];
-    for (int k = f; k < f + c; ++k) {
+    for (int k = f + (int)(threadIdx.x & 63); k < f + c; k += 64) {
         const int j = perm1[k];
         key2[j] = (uint64_t)f;
         klo2[j] = 0;
@@ -701,20 +711,23 @@ __device__ __forceinline__ int wave_total(int v) {
 constexpr int kUWMaxAnc = kMaxDepth + 1;
 constexpr int kUWSortCap = 1024;
 
+// MODE 0 counts, MODE 1 fills (only the fill pass needs the sort buffers in
+// LDS; the counting pass keeps its footprint small for occupancy).
+template <int mode>
 __global__ void __launch_bounds__(64) k_uwlist(int nleaves, const int* __restrict__ leaf_ids,
                                               const RootInfo* __restrict__ roots, const int* __restrict__ bsys,
                                               const int* __restrict__ parent, const int* __restrict__ child_first,
                                               const int* __restrict__ nchild, const int* __restrict__ seg_first,
                                               const int* __restrict__ seg_count,
                                               const double* __restrict__ cfp, const double* __restrict__ half,
-                                              const int* __restrict__ coll, const int* __restrict__ ncoll, int mode,
+                                              const int* __restrict__ coll, const int* __restrict__ ncoll,
                                               int* __restrict__ ucnt, int* __restrict__ wcnt,
                                               const int* __restrict__ uoff, const int* __restrict__ woff,
                                               int* __restrict__ uidx, int* __restrict__ widx,
                                               int* __restrict__ usrc) {
     __shared__ int anc[kUWMaxAnc], cum[kUWMaxAnc + 1];
     __shared__ int nanc;
-    __shared__ int ubuf[kUWSortCap];
+    __shared__ int ubuf[mode ? kUWSortCap : 1];
     const int li = blockIdx.x;
     const int b = leaf_ids[li];
     const int lane = threadIdx.x;
@@ -825,7 +838,9 @@ __global__ void __launch_bounds__(64) k_uwlist(int nleaves, const int* __restric
 }
 
 // The three list sizes and the slot-overflow flag, for one readback.
-__global__ void k_list_sizes(const int* nu, const int* nw, const int* nx, const int* overflow, int* out) {
+__global__ void k_list_sizes(const int* nu, const int* nw, const int* nx, const int* overflow, int* out,
+                             unsigned long long* totals) {
+    if (threadIdx.x < 4) totals[threadIdx.x] = 0ull;  // zeroed for k_list_totals
     if (threadIdx.x == 0) {
         out[0] = *nu;
         out[1] = *nw;
@@ -835,21 +850,25 @@ __global__ void k_list_sizes(const int* nu, const int* nw, const int* nx, const 
 }
 
 // Totals for the statistics: colleague and V entries, the largest leaf and
-// sum_leaf n_leaf * usrc_leaf (integer atomics, so the order does not matter).
-__global__ void __launch_bounds__(kT) k_list_totals(int nb, int nleaves, const int* __restrict__ ncoll,
-                                                   const int* __restrict__ nv, const int* __restrict__ leaf_ids,
-                                                   const int* __restrict__ seg_count, const int* __restrict__ usrc,
-                                                   unsigned long long* __restrict__ tot) {
-    const int i = blockIdx.x * kT + threadIdx.x;
+// sum_leaf n_leaf * usrc_leaf: block sums in LDS, then four integer atomics
+// per block (order-free).
+constexpr int kTotThreads = 1024;
+__global__ void __launch_bounds__(kTotThreads) k_list_totals(int nb, int nleaves, const int* __restrict__ ncoll,
+                                                            const int* __restrict__ nv,
+                                                            const int* __restrict__ leaf_ids,
+                                                            const int* __restrict__ seg_count,
+                                                            const int* __restrict__ usrc,
+                                                            unsigned long long* __restrict__ tot) {
+    __shared__ unsigned long long red[4][kTotThreads / 64];
     unsigned long long c = 0, v = 0, occ = 0, pairs = 0;
+    const int i = blockIdx.x * kTotThreads + threadIdx.x;
     if (i < nb) {
         c = (unsigned long long)ncoll[i];
         v = (unsigned long long)nv[i];
     }
     if (i < nleaves) {
-        const int nt = seg_count[leaf_ids[i]];
-        occ = (unsigned long long)nt;
-        pairs = (unsigned long long)nt * (unsigned long long)usrc[i];
+        occ = (unsigned long long)seg_count[leaf_ids[i]];
+        pairs = occ * (unsigned long long)usrc[i];
     }
     for (int m = 32; m >= 1; m >>= 1) {
         c += __shfl_xor(c, m, 64);
@@ -858,11 +877,22 @@ __global__ void __launch_bounds__(kT) k_list_totals(int nb, int nleaves, const i
         const unsigned long long o = __shfl_xor(occ, m, 64);
         occ = o > occ ? o : occ;
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&tot[0], c);
-        atomicAdd(&tot[1], v);
-        atomicMax(&tot[2], occ);
-        atomicAdd(&tot[3], pairs);
+        red[0][w] = c;
+        red[1][w] = v;
+        red[2][w] = occ;
+        red[3][w] = pairs;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kTotThreads / 64; ++k) {
+            const unsigned long long x = red[threadIdx.x][k];
+            t = threadIdx.x == 2 ? (x > t ? x : t) : t + x;
+        }
+        if (threadIdx.x == 2) atomicMax(&tot[2], t);
+        else atomicAdd(&tot[threadIdx.x], t);
     }
 }
 
@@ -875,15 +905,16 @@ __global__ void __launch_bounds__(kT) k_list_totals(int nb, int nleaves, const i
 // reference's order (X filled from `leaves` in ascending id). mode 0 counts,
 // mode 1 writes.
 constexpr int kXSortCap = 1024;
+template <int mode>
 __global__ void __launch_bounds__(64) k_xlist(int nb, const RootInfo* __restrict__ roots, const int* __restrict__ bsys,
                                              const int* __restrict__ parent, const int* __restrict__ nchild,
                                              const int* __restrict__ refid, const double* __restrict__ cfp,
                                              const double* __restrict__ half, const int* __restrict__ coll,
-                                             const int* __restrict__ ncoll, int mode, int* __restrict__ xcnt,
+                                             const int* __restrict__ ncoll, int* __restrict__ xcnt,
                                              const int* __restrict__ xoff, int* __restrict__ xidx) {
     __shared__ int anc[kUWMaxAnc], cum[kUWMaxAnc + 1];
     __shared__ int nanc;
-    __shared__ int xbuf[kXSortCap];
+    __shared__ int xbuf[mode ? kXSortCap : 1];
     const int w = blockIdx.x;
     const int lane = threadIdx.x;
     const int pw = parent[w];
@@ -1167,7 +1198,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     tv.leaf_ids = pool.get<int>("leaf_ids", tv.nleaves);
     hipLaunchKernelGGL(k_leaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.leaf_ids);
     tv.leaf_of = pool.get<int>("leaf_of", n);
-    hipLaunchKernelGGL(k_leaf_keys, dim3(blocks(tv.nleaves)), dim3(kT), 0, s, tv.nleaves, tv.leaf_ids,
+    hipLaunchKernelGGL(k_leaf_keys, dim3(blocks(64LL * tv.nleaves)), dim3(kT), 0, s, tv.nleaves, tv.leaf_ids,
                        tv.seg_first, tv.seg_count, perm1, sb.khi[0], sb.klo[0], sb.val[0], tv.leaf_of);
     int bits = 1;
     while ((1LL << bits) <= n) ++bits;
@@ -1237,8 +1268,10 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         if (tot != nb) throw std::runtime_error("octant counts do not cover the boxes");
         tv.m2l_tiles256 = pool.get<int>("m2l_tiles256", tv.m2l_ntiles256 + 1);
         tv.m2l_tiles128 = pool.get<int>("m2l_tiles128", tv.m2l_ntiles128 + 1);
-        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(64), 0, s, d_out + nlev, 256, tv.m2l_tiles256);
-        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(64), 0, s, d_out + nlev, 128, tv.m2l_tiles128);
+        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_out + nlev, 256, tv.m2l_ntiles256,
+                           tv.m2l_tiles256);
+        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_out + nlev, 128, tv.m2l_ntiles128,
+                           tv.m2l_tiles128);
     }
 
     // ---- colleagues, V, M2L source table
@@ -1262,23 +1295,24 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     int* wcnt = pool.get<int>("wcnt", nl + 1);
     tv.u_off = pool.get<int>("u_off", nl + 1);
     tv.w_off = pool.get<int>("w_off", nl + 1);
-    hipLaunchKernelGGL(k_uwlist, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
+    hipLaunchKernelGGL(k_uwlist<0>, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
                        tv.child_first,
-                       tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, 0, ucnt, wcnt, nullptr, nullptr,
+                       tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, ucnt, wcnt, nullptr, nullptr,
                        nullptr, nullptr, nullptr);
     scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
     scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp + scan_tmp_ints(scan_cap), s);
     // X counts too, so that one round trip returns all three list sizes
     int* xcnt = pool.get<int>("xcnt", nb + 1);
     tv.x_off = pool.get<int>("x_off", nb + 1);
-    hipLaunchKernelGGL(k_xlist, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid, tv.cfp,
-                       tv.half, tv.coll, tv.ncoll, 0, xcnt, nullptr, nullptr);
+    hipLaunchKernelGGL(k_xlist<0>, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid,
+                       tv.cfp, tv.half, tv.coll, tv.ncoll, xcnt, nullptr, nullptr);
     scan_exclusive(xcnt, tv.x_off, nb, tv.x_off + nb, scan_tmp, s);
     {
         int tot[4];
         int* d_tot = pool.get<int>("list_sizes", 4);
+        tv.list_totals = pool.get<unsigned long long>("list_totals", 4);
         hipLaunchKernelGGL(k_list_sizes, dim3(1), dim3(64), 0, s, tv.u_off + nl, tv.w_off + nl, tv.x_off + nb, overflow,
-                           d_tot);
+                           d_tot, tv.list_totals);
         AFMM_CHECK(hipMemcpyAsync(tot, d_tot, sizeof(tot), hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipStreamSynchronize(s));
         if (tot[3] != 0) throw std::runtime_error("interaction-list slot overflow");
@@ -1289,16 +1323,15 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     tv.u_idx = pool.get<int>("u_idx", tv.n_u);
     tv.w_idx = pool.get<int>("w_idx", tv.n_w);
     tv.x_idx = pool.get<int>("x_idx", tv.n_x);
-    hipLaunchKernelGGL(k_uwlist, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
+    hipLaunchKernelGGL(k_uwlist<1>, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
                        tv.child_first,
-                       tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, 1, nullptr, nullptr, tv.u_off,
+                       tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, nullptr, nullptr, tv.u_off,
                        tv.w_off, tv.u_idx, tv.w_idx, ucnt);
-    tv.list_totals = pool.get<unsigned long long>("list_totals", 4);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, tv.list_totals, 4);
-    hipLaunchKernelGGL(k_list_totals, dim3(blocks(std::max(nb, nl))), dim3(kT), 0, s, nb, nl, tv.ncoll, tv.nv,
+    hipLaunchKernelGGL(k_list_totals, dim3(blocks(std::max(nb, nl), kTotThreads)), dim3(kTotThreads), 0, s, nb, nl,
+                       tv.ncoll, tv.nv,
                        tv.leaf_ids, tv.seg_count, ucnt, tv.list_totals);
-    hipLaunchKernelGGL(k_xlist, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid, tv.cfp,
-                       tv.half, tv.coll, tv.ncoll, 1, nullptr, tv.x_off, tv.x_idx);
+    hipLaunchKernelGGL(k_xlist<1>, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid,
+                       tv.cfp, tv.half, tv.coll, tv.ncoll, nullptr, tv.x_off, tv.x_idx);
     AFMM_CHECK(hipGetLastError());
 }
 
