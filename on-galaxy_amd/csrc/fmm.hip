@@ -789,6 +789,23 @@ __global__ void __launch_bounds__(kM2LBfWaves * 64) k_m2l_bf16(int np, int ntile
 //     src(o, t) = parent if octant(t) == o (one operator per target); adds
 //     into Lhat[t].
 constexpr int kLevelWaves = 8;
+
+// Degree of coefficient row r (rows n^2 .. (n+1)^2 - 1 hold degree n).
+constexpr int degree_of(int r) {
+    int n = 0;
+    while ((n + 1) * (n + 1) <= r) ++n;
+    return n;
+}
+// Whether the 16 x 4 operator block (row block rb, k step ks) can be nonzero:
+// M2M takes sources of degree <= the target's, L2L of degree >= the target's.
+template <int S, int MODE>
+constexpr bool level_block_nonzero(int rb, int ks) {
+    const int rlo = 16 * rb, clo = 4 * ks;
+    if (rlo >= S || clo >= S) return false;
+    const int rhi = rlo + 15 < S ? rlo + 15 : S - 1, chi = clo + 3 < S ? clo + 3 : S - 1;
+    return MODE == 0 ? degree_of(clo) <= degree_of(rhi) : degree_of(chi) >= degree_of(rlo);
+}
+
 template <int P, int MODE>
 __global__ void __launch_bounds__(kLevelWaves * 64) k_level_mfma(int n, const int* __restrict__ tgt,
                                                                 const int* __restrict__ child_first,
@@ -821,11 +838,14 @@ __global__ void __launch_bounds__(kLevelWaves * 64) k_level_mfma(int n, const in
     const bool active = __ballot(src >= 0) != 0ull;
     if (active) {
         const double* __restrict__ ms = in + (size_t)(src >= 0 ? src : 0) * S;
+        // structurally zero blocks (a third of them at p = 8) are skipped
+#pragma unroll
         for (int c = 0; c < NKC; ++c) {
             double bv[KC], av[RB * KC];
             const double* __restrict__ A = Tsw + ((size_t)o * NKC + c) * CH + lane;
 #pragma unroll
-            for (int e = 0; e < RB * KC; ++e) av[e] = A[e * 64];
+            for (int e = 0; e < RB * KC; ++e)
+                av[e] = level_block_nonzero<S, MODE>(e / KC, c * KC + e % KC) ? A[e * 64] : 0.0;
 #pragma unroll
             for (int kk = 0; kk < KC; ++kk) {
                 const int k = 4 * (c * KC + kk) + kq;
@@ -835,7 +855,8 @@ __global__ void __launch_bounds__(kLevelWaves * 64) k_level_mfma(int n, const in
             for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb * KC + kk], bv[kk], acc[rb], 0, 0, 0);
+                    if (level_block_nonzero<S, MODE>(rb, c * KC + kk))
+                        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[rb * KC + kk], bv[kk], acc[rb], 0, 0, 0);
         }
     }
     // octant-ordered sum of the partial tiles

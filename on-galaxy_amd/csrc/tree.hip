@@ -724,7 +724,7 @@ __global__ void __launch_bounds__(64) k_uwlist(int nleaves, const int* __restric
                                               int* __restrict__ ucnt, int* __restrict__ wcnt,
                                               const int* __restrict__ uoff, const int* __restrict__ woff,
                                               int* __restrict__ uidx, int* __restrict__ widx,
-                                              int* __restrict__ usrc) {
+                                              int* __restrict__ usrc, int* __restrict__ lane_cnt) {
     __shared__ int anc[kUWMaxAnc], cum[kUWMaxAnc + 1];
     __shared__ int nanc;
     __shared__ int ubuf[mode ? kUWSortCap : 1];
@@ -773,7 +773,14 @@ __global__ void __launch_bounds__(64) k_uwlist(int nleaves, const int* __restric
         c = coll[kCollSlots * (size_t)b + lane];
         if (c == b || nchild[c] == 0) c = -1;
     }
-    for (int pass = 0; pass < (mode ? 2 : 1); ++pass) {
+    // the counting launch keeps each lane's counts, so the filling launch
+    // descends once
+    int* lc = lane_cnt + (size_t)li * 2 * kCollSlots;
+    if (mode && lane < kCollSlots) {
+        myu = lc[lane];
+        myw = lc[kCollSlots + lane];
+    }
+    for (int pass = mode ? 1 : 0; pass < (mode ? 2 : 1); ++pass) {
         int uo = 0, wo = 0;
         if (pass == 1) {
             uo = ku + wave_excl_scan(myu, lane);
@@ -801,6 +808,10 @@ __global__ void __launch_bounds__(64) k_uwlist(int nleaves, const int* __restric
     }
     const int totu = ku + wave_total(myu), totw = wave_total(myw);
     if (!mode) {
+        if (lane < kCollSlots) {
+            lc[lane] = myu;
+            lc[kCollSlots + lane] = myw;
+        }
         if (lane == 0) {
             ucnt[li] = totu;
             wcnt[li] = totw;
@@ -1295,10 +1306,11 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     int* wcnt = pool.get<int>("wcnt", nl + 1);
     tv.u_off = pool.get<int>("u_off", nl + 1);
     tv.w_off = pool.get<int>("w_off", nl + 1);
+    int* uw_lane_cnt = pool.get<int>("uw_lane_cnt", 2 * kCollSlots * (size_t)nl);
     hipLaunchKernelGGL(k_uwlist<0>, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
                        tv.child_first,
                        tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, ucnt, wcnt, nullptr, nullptr,
-                       nullptr, nullptr, nullptr);
+                       nullptr, nullptr, nullptr, uw_lane_cnt);
     scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
     scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp + scan_tmp_ints(scan_cap), s);
     // X counts too, so that one round trip returns all three list sizes
@@ -1326,7 +1338,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     hipLaunchKernelGGL(k_uwlist<1>, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
                        tv.child_first,
                        tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, nullptr, nullptr, tv.u_off,
-                       tv.w_off, tv.u_idx, tv.w_idx, ucnt);
+                       tv.w_off, tv.u_idx, tv.w_idx, ucnt, uw_lane_cnt);
     hipLaunchKernelGGL(k_list_totals, dim3(blocks(std::max(nb, nl), kTotThreads)), dim3(kTotThreads), 0, s, nb, nl,
                        tv.ncoll, tv.nv,
                        tv.leaf_ids, tv.seg_count, ucnt, tv.list_totals);
