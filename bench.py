@@ -318,6 +318,7 @@ def main():
     ap.add_argument("--large-n", type=int, default=4_000_000)
     ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-orders", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (scale for HBM counters) and exit")
     args = ap.parse_args()
@@ -343,9 +344,11 @@ def main():
     d_pos = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
     d_mass = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
     d_acc = torch.empty_like(d_pos)
-    def timed(precision):
+    def timed(precision, p=None, steps=None):
         """W warmup + K timed evaluations; returns (s/eval, averaged phase ms, last stats, solver)."""
-        solver = afmm.AdaptiveFMM3D(G=1.0, eps=args.eps, p=args.p, threshold=args.cap, precision=precision)
+        p = args.p if p is None else p
+        nsteps = args.steps if steps is None else steps
+        solver = afmm.AdaptiveFMM3D(G=1.0, eps=args.eps, p=p, threshold=args.cap, precision=precision)
 
         def step():
             solver.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
@@ -356,14 +359,14 @@ def main():
         torch.cuda.synchronize()
         phases = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(nsteps):
             step()
             phases.append(solver.stats())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         keys = ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_leaf", "ms_total", "ms_p2p")
         avg = {k[3:]: float(np.mean([ph[k] for ph in phases])) for k in keys}
-        return dt / args.steps, avg, phases[-1], solver
+        return dt / nsteps, avg, phases[-1], solver
 
     progress(f"FP64 timing, n={n}")
     s_per_eval, avg, st, solver = timed(afmm.PRECISION_FP64)
@@ -423,6 +426,26 @@ def main():
             ref0 = s0.direct(pos, mass, idx)
         accuracy["eps0"][name + "_vs_direct"] = errs(ref0, a0[idx])
         s0.close()
+    # the other orders of the same workload: FP64 and bf16-split timing (softened,
+    # as the headline) and accuracy against FP64 direct at eps = 0
+    orders = {}
+    if not args.no_orders:
+        progress("orders 4 and 12")
+        for po in (4, 12):
+            if po == args.p:
+                continue
+            row = {}
+            for name, prec in (("fp64", afmm.PRECISION_FP64), ("mixed_bf16", afmm.PRECISION_MIXED_BF16)):
+                t_o, ph_o, _, so = timed(prec, p=po, steps=3)
+                so.close()
+                s0 = afmm.AdaptiveFMM3D(G=1.0, eps=0.0, p=po, threshold=args.cap, precision=prec)
+                s0.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
+                torch.cuda.synchronize()
+                a0 = d_acc.cpu().numpy()
+                s0.close()
+                row[name] = {"s_per_eval": t_o, "m2l_ms": ph_o["m2l"], "eps0_vs_direct": errs(ref0, a0[idx])}
+            row["mixed_bf16_speedup_vs_fp64"] = row["fp64"]["s_per_eval"] / row["mixed_bf16"]["s_per_eval"]
+            orders[f"p{po}"] = row
     best = min((mix_s, "fp32_split"), (bf_s, "bf16_split"))
     mixed = {"s_per_eval": best[0], "speedup_vs_fp64": s_per_eval / best[0], "form": best[1],
              "fp32_split": {"s_per_eval": mix_s, "speedup_vs_fp64": s_per_eval / mix_s, "phases_ms": mix_avg,
@@ -451,6 +474,7 @@ def main():
            "phases_ms": avg,
            "mixed": mixed,
            "accuracy": accuracy,
+           "orders": orders,
            "tree": {"boxes": st["n_boxes"], "leaves": st["n_leaves"], "max_level": st["max_level"],
                     "max_leaf_occ": st["max_leaf_occ"], "v_pairs": st["n_v"], "p2p_pairs": st["p2p_pairs"],
                     "u_entries": st["n_u"], "w_entries": st["n_w"]},
