@@ -21,21 +21,30 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
 
+// Exclusive scan of one int per thread over the block: wave inclusive scans
+// (shuffles), then the wave totals through LDS (lds: at least kScanThreads / 64
+// ints). Integer sums: the result does not depend on the order.
 __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total) {
-    // lds: kScanThreads ints
-    const int tid = threadIdx.x;
-    lds[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        const int t = (tid >= off) ? lds[tid - off] : 0;
-        __syncthreads();
-        lds[tid] += t;
-        __syncthreads();
+    constexpr int kW = kScanThreads / kWave;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += t;
     }
-    total = lds[kScanThreads - 1];
-    const int incl = lds[tid];
+    __syncthreads();  // lds free from a previous call
+    if (lane == kWave - 1) lds[w] = incl;
     __syncthreads();
-    return incl - v;
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+        const int t = lds[k];
+        before += k < w ? t : 0;
+        total += t;
+    }
+    return before + incl - v;
 }
 
 // rng (optional, device): the item count is rng[1] - rng[0] instead of n, and

@@ -530,9 +530,18 @@ __global__ void k_gather_at(const int* a, const int* idx, int n, int total, int*
 }
 
 // ---- leaves and the in-leaf input order ------------------------------------------
-__global__ void __launch_bounds__(kT) k_leaf_flags(int nb, const int* nchild, int* flag) {
+// Leaf flags, and the largest leaf's body count into *maxleaf (zeroed before).
+__global__ void __launch_bounds__(kT) k_leaf_flags(int nb, const int* nchild, const int* seg_count, int* flag,
+                                                  int* maxleaf) {
     const int b = blockIdx.x * kT + threadIdx.x;
-    if (b < nb) flag[b] = nchild[b] == 0;
+    int c = 0;
+    if (b < nb) {
+        const bool leaf = nchild[b] == 0;
+        flag[b] = leaf;
+        if (leaf) c = seg_count[b];
+    }
+    for (int m = 32; m >= 1; m >>= 1) c = max(c, __shfl_xor(c, m, 64));
+    if ((threadIdx.x & 63) == 0 && c > 0) atomicMax(maxleaf, c);
 }
 __global__ void __launch_bounds__(kT) k_leaf_ids(int nb, const int* nchild, const int* scan, int* ids) {
     const int b = blockIdx.x * kT + threadIdx.x;
@@ -553,6 +562,27 @@ __global__ void __launch_bounds__(kT) k_leaf_keys(int nleaves, const int* leaf_i
         klo2[j] = 0;
         val2[j] = j;
         leaf_of[k] = b;
+    }
+}
+constexpr int kLeafRankMax = 2048;  // per-leaf ranking is O(c^2 / 64) per lane
+
+// Input order inside every leaf: the final order of leaf b's slots
+// [f, f + c) is its input indices perm1[f .. f + c) ascending (what a stable
+// sort of all bodies by leaf start would give), here ranked inside the leaf by
+// one wave (indices are distinct). Also leaf_of[slot] = b.
+__global__ void __launch_bounds__(kT) k_leaf_order(int nleaves, const int* leaf_ids, const int* seg_first,
+                                                  const int* seg_count, const int* __restrict__ perm1,
+                                                  int* __restrict__ out, int* __restrict__ leaf_of) {
+    const int li = blockIdx.x * (kT / 64) + (int)(threadIdx.x >> 6);
+    if (li >= nleaves) return;
+    const int b = leaf_ids[li];
+    const int f = seg_first[b], c = seg_count[b];
+    for (int k = (int)(threadIdx.x & 63); k < c; k += 64) {
+        const int v = perm1[f + k];
+        int r = 0;
+        for (int j = 0; j < c; ++j) r += perm1[f + j] < v;
+        out[f + r] = v;
+        leaf_of[f + k] = b;
     }
 }
 
@@ -1203,23 +1233,39 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     // ---- leaves, pass 2 (input order inside each leaf), gather
     int* flags = pool.get<int>("leaf_flag", nb);
     int* fscan = pool.get<int>("leaf_scan", nb);
-    hipLaunchKernelGGL(k_leaf_flags, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, flags);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, d_total + 1, 1, 0);
+    hipLaunchKernelGGL(k_leaf_flags, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.seg_count, flags,
+                       d_total + 1);
     scan_exclusive(flags, fscan, nb, d_total, scan_tmp, s);
-    tv.nleaves = read_int(d_total, s);
+    int lt[2];
+    AFMM_CHECK(hipMemcpyAsync(lt, d_total, sizeof(lt), hipMemcpyDeviceToHost, s));
+    AFMM_CHECK(hipStreamSynchronize(s));
+    tv.nleaves = lt[0];
+    const int max_leaf = lt[1];
     tv.leaf_ids = pool.get<int>("leaf_ids", tv.nleaves);
     hipLaunchKernelGGL(k_leaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.leaf_ids);
     tv.leaf_of = pool.get<int>("leaf_of", n);
-    hipLaunchKernelGGL(k_leaf_keys, dim3(blocks(64LL * tv.nleaves)), dim3(kT), 0, s, tv.nleaves, tv.leaf_ids,
-                       tv.seg_first, tv.seg_count, perm1, sb.khi[0], sb.klo[0], sb.val[0], tv.leaf_of);
-    int bits = 1;
-    while ((1LL << bits) <= n) ++bits;
-    const int r2 = radix_sort(sb, n, 1, (bits + 8) / 9, s);
+    // input order inside the leaves: ranked per leaf when leaves are small (one
+    // kernel), else a stable radix sort of all bodies by leaf start
+    const int* leaf_perm;
+    if (max_leaf <= kLeafRankMax) {
+        int* lp = pool.get<int>("leaf_perm", n);
+        hipLaunchKernelGGL(k_leaf_order, dim3(blocks(64LL * tv.nleaves)), dim3(kT), 0, s, tv.nleaves, tv.leaf_ids,
+                           tv.seg_first, tv.seg_count, perm1, lp, tv.leaf_of);
+        leaf_perm = lp;
+    } else {
+        hipLaunchKernelGGL(k_leaf_keys, dim3(blocks(64LL * tv.nleaves)), dim3(kT), 0, s, tv.nleaves, tv.leaf_ids,
+                           tv.seg_first, tv.seg_count, perm1, sb.khi[0], sb.klo[0], sb.val[0], tv.leaf_of);
+        int bits = 1;
+        while ((1LL << bits) <= n) ++bits;
+        leaf_perm = sb.val[radix_sort(sb, n, 1, (bits + 8) / 9, s)];
+    }
     tv.x = pool.get<double>("bx", n);
     tv.y = pool.get<double>("by", n);
     tv.z = pool.get<double>("bz", n);
     tv.m = pool.get<double>("bm", n);
     tv.orig = pool.get<int>("orig", n);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, n, sb.val[r2], d_pos, d_mass, tv.x, tv.y, tv.z,
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, n, leaf_perm, d_pos, d_mass, tv.x, tv.y, tv.z,
                        tv.m, tv.orig);
 
     // ---- geometry

@@ -160,3 +160,28 @@ def test_many_more_boxes_than_bodies(afmm):
     ref, _ = orc.fmm(pos, m, 1.0, 0.0, 2, 1)
     mx, _ = cabi.rel_err(ref, acc)
     assert mx <= TOL
+
+
+@pytest.mark.parametrize("ndup", [1500, 3000])
+def test_large_duplicate_leaf(afmm, ndup):
+    """A cluster of identical bodies stops splitting at the depth cap and stays
+    one leaf of ndup bodies, whose slots must hold the bodies in input order.
+    1500 takes the per-leaf ranking, 3000 the radix sort by leaf start (the
+    ranking is used up to 2048 bodies per leaf). Tree bit-exact and forces vs
+    the oracle."""
+    rng = np.random.default_rng(ndup)
+    pos = rng.uniform(-1, 1, (4000, 3))
+    dup = rng.choice(len(pos), ndup, replace=False)
+    pos[dup] = pos[dup[0]]
+    m = rng.uniform(0.5, 1.5, len(pos))
+    print(f"duplicate leaf of {ndup} in 4000 bodies", flush=True)
+    orc = cabi.oracle()
+    s = solver(afmm, 4, 32, 0.01)
+    acc = s.computeAccelerations(pos, m)
+    assert s.stats()["max_leaf_occ"] >= ndup
+    t, tr = s.tree(), orc.tree(pos, m, 32, 0.01)
+    for k in TREE_KEYS:
+        assert np.array_equal(t[k], tr[k]), k
+    ref, _ = orc.fmm(pos, m, 1.0, 0.01, 4, 32)
+    mx, _ = cabi.rel_err(ref, acc)
+    assert mx <= TOL
