@@ -209,29 +209,39 @@ __global__ void __launch_bounds__(64) k_accumulate(TreeView t, double* __restric
 }
 
 // ---- M2P over W (AdaptiveFMM3D.hpp:95, FmmExpansions3D.hpp:180-200) --------------
-// One wave per target leaf (an 8-wave workgroup per leaf measured 15% slower:
-// most W lists are short and leave its groups idle; prefetching the next
-// multipoles into registers gained nothing). The wave is split into
-// G = floor(64/T) lane groups of T = max(n_leaf, 8) lanes (T = 64 above 64 bodies); group g handles W
-// entries g, g+G, ... with their multipoles staged in LDS. Partial fields are
-// reduced across groups in a fixed order and added to far[] (and, with POT,
-// the potential to fpot[]).
+// One workgroup of kM2PWaves waves per target leaf. Its L = 64 * kM2PWaves lanes
+// form G = L / T lane groups of T = max(n_leaf, L / m2p_groups) lanes (T = L
+// above L bodies); group g handles W entries g, g + G, ... with their
+// multipoles staged in LDS. Partial fields are reduced across groups in a
+// fixed order and added to far[] (and, with POT, the potential to fpot[]).
+// Four waves per leaf rather than one: the W lists are heavy-tailed (a few
+// clustered leaves hold 200-300 entries), and one wave per leaf left those
+// waves running long after the rest of the grid had drained.
+// At p <= 8 the kernel is held to 128 VGPRs (four waves per SIMD; a few bytes
+// spill), measured 5% faster than 143 VGPRs at three waves; above p = 8 that
+// limit spills hundreds of bytes and is not applied.
+constexpr int kM2PWaves = 4;
+template <int P>
+constexpr int m2p_groups() { return P <= 8 ? 32 : 16; }
+template <int P>
+constexpr int m2p_min_waves() { return P <= 8 ? 4 : 1; }
+
 template <int P, bool POT>
-__global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict__ M, double* __restrict__ far,
-                                           double* __restrict__ fpot) {
-    constexpr int S = nterms(P);
-    __shared__ double mw[8][S];
-    __shared__ double red[4][64];
+__global__ void __launch_bounds__(kM2PWaves * 64, m2p_min_waves<P>()) k_m2p(TreeView t, const double* __restrict__ M,
+                                                       double* __restrict__ far, double* __restrict__ fpot) {
+    constexpr int S = nterms(P), L = kM2PWaves * 64, GM = m2p_groups<P>();
+    __shared__ double mw[GM][S];
+    __shared__ double red[4][L];
     const int li = xcd_chunk_item();
     if (li >= t.nleaves) return;
     const int wa = t.w_off[li], nw = t.w_off[li + 1] - wa;
     if (nw == 0) return;
     const int b = t.leaf_ids[li];
     const int f = t.seg_first[b], nt = t.seg_count[b];
-    const int lane = threadIdx.x;
-    const int T = nt > 64 ? 64 : (nt < 8 ? 8 : nt);
-    const int Gs = 64 / T;
-    const int g = lane / T, tl = lane % T;
+    const int tid = threadIdx.x;
+    const int T = nt > L ? L : max(nt, L / GM);
+    const int Gs = L / T;  // <= GM
+    const int g = tid / T, tl = tid % T;
     for (int tb = 0; tb < nt; tb += T) {
         const int i = f + tb + tl;
         const bool act = g < Gs && tb + tl < nt;
@@ -240,7 +250,7 @@ __global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict
         for (int r = 0; r < nw; r += Gs) {
             __syncthreads();
             const int nq = min(Gs, nw - r);
-            for (int e = lane; e < nq * S; e += 64) {
+            for (int e = tid; e < nq * S; e += L) {
                 const int q = e / S, c = e - q * S;
                 mw[q][c] = M[(size_t)t.w_idx[wa + r + q] * S + c];
             }
@@ -264,10 +274,10 @@ __global__ void __launch_bounds__(64) k_m2p(TreeView t, const double* __restrict
         }
         if (Gs > 1) {
             __syncthreads();
-            red[0][lane] = fx;
-            red[1][lane] = fy;
-            red[2][lane] = fz;
-            red[3][lane] = fp;
+            red[0][tid] = fx;
+            red[1][tid] = fy;
+            red[2][tid] = fz;
+            red[3][tid] = fp;
             __syncthreads();
             if (g == 0)
                 for (int q = 1; q < Gs; ++q) {
@@ -1158,8 +1168,9 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (ev) AFMM_CHECK(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far, fpot);
     if (t.n_w > 0) {
-        if (fpot) hipLaunchKernelGGL((k_m2p<P, true>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M, far, fpot);
-        else hipLaunchKernelGGL((k_m2p<P, false>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M, far, fpot);
+        const dim3 mg(xcd_chunk_grid(t.nleaves)), mb(kM2PWaves * 64);
+        if (fpot) hipLaunchKernelGGL((k_m2p<P, true>), mg, mb, 0, s, t, M, far, fpot);
+        else hipLaunchKernelGGL((k_m2p<P, false>), mg, mb, 0, s, t, M, far, fpot);
     }
     const dim3 pg(xcd_chunk_grid(t.nleaves)), pb(64);
     if (ev) AFMM_CHECK(hipEventRecord(ev[6], s));
