@@ -139,30 +139,46 @@ __device__ __forceinline__ int stream_body(const StreamLds& sl, int m, int q) {
 
 // Coefficients C0 .. C0 + CW - 1 of one target box: lanes take the staged
 // bodies 64 at a time, then the wave sums its windows by recursive halving.
+// P2M (KIND 0) reads the leaf's own contiguous bodies f0 .. f0 + n0 - 1
+// directly; P2L streams the bodies of the X list through LDS.
 template <int P, int KIND, int C0, int CW>
-__device__ __forceinline__ void accumulate_window(const TreeView& t, const int* list, int nlist, const double* ch,
-                                                  double ih, int lane, StreamLds& sl, double* __restrict__ out,
-                                                  size_t out_base) {
+__device__ __forceinline__ void accumulate_window(const TreeView& t, const int* list, int nlist, int f0, int n0,
+                                                  const double* ch, double ih, int lane, StreamLds& sl,
+                                                  double* __restrict__ out, size_t out_base) {
     constexpr int S = nterms(P);
     double acc[CW];
 #pragma unroll
     for (int k = 0; k < CW; ++k) acc[k] = 0.0;
     for (int e0 = 0; e0 < nlist; e0 += kStreamChunk) {
         const int m = min(kStreamChunk, nlist - e0);
-        __syncthreads();  // previous chunk's readers are done
-        const int total = stage_chunk(sl, list, e0, m, t.seg_first, t.seg_count, lane);
-        for (int q0 = 0; q0 < total; q0 += 64) {
-            const int q = q0 + lane;
+        int total = n0;
+        if constexpr (KIND != 0) {
+            __syncthreads();  // previous chunk's readers are done
+            total = stage_chunk(sl, list, e0, m, t.seg_first, t.seg_count, lane);
+        }
+        // The next slot's body is located and loaded before the current slot's
+        // harmonics, so its search and load latency overlap the arithmetic.
+        double nx = 0.0, ny = 0.0, nz = 0.0, nm = 0.0;
+        auto fetch = [&](int q) {
             if (q < total) {
-                const int i = stream_body(sl, m, q);
-                const double dx = offset_from(t.x[i], Centre{ch[0], ch[1]}) * ih;
-                const double dy = offset_from(t.y[i], Centre{ch[2], ch[3]}) * ih;
-                const double dz = offset_from(t.z[i], Centre{ch[4], ch[5]}) * ih;
+                const int i = KIND == 0 ? f0 + q : stream_body(sl, m, q);
+                nx = t.x[i], ny = t.y[i], nz = t.z[i], nm = t.m[i];
+            }
+        };
+        fetch(lane);
+        for (int q0 = 0; q0 < total; q0 += 64) {
+            const bool valid = q0 + lane < total;
+            const double bx = nx, by = ny, bz = nz, bm = nm;
+            fetch(q0 + 64 + lane);
+            if (valid) {
+                const double dx = offset_from(bx, Centre{ch[0], ch[1]}) * ih;
+                const double dy = offset_from(by, Centre{ch[2], ch[3]}) * ih;
+                const double dz = offset_from(bz, Centre{ch[4], ch[5]}) * ih;
                 auto add = [&](int k, double v) {
                     if (k >= C0 && k < C0 + CW) acc[k - C0] += v;
                 };
-                if (KIND == 0) p2m_point<P>(t.m[i], dx, dy, dz, add);
-                else p2l_point<P>(t.m[i], dx, dy, dz, add);
+                if (KIND == 0) p2m_point<P>(bm, dx, dy, dz, add);
+                else p2l_point<P>(bm, dx, dy, dz, add);
             }
         }
     }
@@ -204,8 +220,10 @@ __global__ void __launch_bounds__(64) k_accumulate(TreeView t, double* __restric
     const double ch[6] = {t.cdd[6 * (size_t)b], t.cdd[6 * (size_t)b + 1], t.cdd[6 * (size_t)b + 2],
                           t.cdd[6 * (size_t)b + 3], t.cdd[6 * (size_t)b + 4], t.cdd[6 * (size_t)b + 5]};
     const double ih = t.invh[b];
-    accumulate_window<P, KIND, 0, CW>(t, list, nlist, ch, ih, lane, sl, out, (size_t)b * S);
-    if constexpr (CW < S) accumulate_window<P, KIND, CW, S - CW>(t, list, nlist, ch, ih, lane, sl, out, (size_t)b * S);
+    const int f0 = KIND == 0 ? t.seg_first[b] : 0, n0 = KIND == 0 ? t.seg_count[b] : 0;
+    accumulate_window<P, KIND, 0, CW>(t, list, nlist, f0, n0, ch, ih, lane, sl, out, (size_t)b * S);
+    if constexpr (CW < S)
+        accumulate_window<P, KIND, CW, S - CW>(t, list, nlist, f0, n0, ch, ih, lane, sl, out, (size_t)b * S);
 }
 
 // ---- M2P over W (AdaptiveFMM3D.hpp:95, FmmExpansions3D.hpp:180-200) --------------
