@@ -66,29 +66,20 @@ __global__ void __launch_bounds__(kScanThreads) k_tile_sums(const int* in, int n
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// One block scans `m` tile sums in chunks with a running carry. With rng, m is
-// the tile count of rng[1] - rng[0] items and the total is written as
-// rng[1] + total (the end offset of the next range).
-__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int* sums, int m, int* total_out, const int* rng) {
-    __shared__ int lds[kScanThreads];
-    if (rng) m = min(m, (max(0, rng[1] - rng[0]) + kScanTile - 1) / kScanTile);
-    int carry = 0;
-    for (int base = 0; base < m; base += kScanThreads) {
-        const int i = base + threadIdx.x;
-        const int v = (i < m) ? sums[i] : 0;
-        int total;
-        const int ex = block_exclusive_scan(v, lds, total);
-        if (i < m) sums[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0 && total_out) *total_out = carry + (rng ? rng[1] : 0);
-}
-
+// Each block finds its tile's offset by summing the raw tile sums before it
+// (a few hundred ints at most for the sizes scanned here), then scans its own
+// tile; the block of the last tile writes the total (with rng: rng[1] + total,
+// the end offset of the next range). Two launches per scan instead of three.
 __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int* in, int n, const int* sums,
-                                                             int* out, const int* rng) {
+                                                             int* out, const int* rng, int* total_out) {
     __shared__ int lds[kScanThreads];
     n = scan_count(n, rng);
-    if (blockIdx.x * kScanTile >= n) return;
+    const int last = n > 0 ? (n - 1) / kScanTile : 0;
+    if ((int)blockIdx.x > last) return;
+    int part = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanThreads) part += sums[k];
+    int tile_base;
+    block_exclusive_scan(part, lds, tile_base);
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
     int s = 0;
@@ -98,12 +89,14 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int* in, int 
         s += v[k];
     }
     int total;
-    int run = block_exclusive_scan(s, lds, total) + sums[blockIdx.x];
+    int run = block_exclusive_scan(s, lds, total) + tile_base;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k < n) out[base + k] = run;
         run += v[k];
     }
+    if ((int)blockIdx.x == last && threadIdx.x == 0 && total_out)
+        *total_out = tile_base + total + (rng ? rng[1] : 0);
 }
 
 // ---- radix sort -----------------------------------------------------------
@@ -197,23 +190,17 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* k
 int scan_tmp_ints(int n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
 void scan_exclusive(const int* d_in, int* d_out, int n, int* d_total, int* d_tmp, hipStream_t s) {
-    const int tiles = (n + kScanTile - 1) / kScanTile;
-    if (n <= 0) {
-        if (d_total)
-            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, 0, d_total, nullptr);
-        return;
-    }
-    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp, nullptr);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, tiles, d_total, nullptr);
-    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp, d_out, nullptr);
+    n = std::max(n, 0);
+    const int tiles = std::max(1, (n + kScanTile - 1) / kScanTile);
+    if (n > 0) hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp, nullptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, d_in, n, d_tmp, d_out, nullptr, d_total);
 }
 
 void scan_exclusive_range(const int* d_in, int* d_out, const int* d_rng, int nmax, int* d_end, int* d_tmp,
                           hipStream_t s) {
     const int tiles = std::max(1, (nmax + kScanTile - 1) / kScanTile);
     hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, d_in, nmax, d_tmp, d_rng);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_tmp, tiles, d_end, d_rng);
-    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, d_in, nmax, d_tmp, d_out, d_rng);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanThreads), 0, s, d_in, nmax, d_tmp, d_out, d_rng, d_end);
 }
 
 int sort_blocks(int n) { return (n + kSortTile - 1) / kSortTile; }
