@@ -75,7 +75,8 @@ typedef struct {
     int32_t max_level;
     int32_t max_leaf_occ;                    /* AdaptiveOctree::maxLeafOcc (:191-195) */
     double ms_tree, ms_lists, ms_upward, ms_m2l, ms_downward, ms_leaf, ms_total; /* device time */
-    double ms_p2p; /* the P2P near-field kernel alone (part of ms_leaf) */
+    double ms_p2p; /* the P2P near-field kernel alone; outside graph mode it runs on a second
+                      stream beside the upward pass through M2P */
     /* graph mode (afmm_set_graph): executable graphs built and updated in
      * place so far; in graph mode only ms_tree and ms_total are timed */
     int64_t graph_instantiations, graph_updates;

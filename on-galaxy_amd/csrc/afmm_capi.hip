@@ -106,7 +106,8 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         AFMM_CHECK(hipEventRecord(c->ev[6], s));
         (updated ? c->graph_updates : c->graph_instantiations) += 1;
     } else {
-        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2);
+        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2, false,
+                SideStream{c->side, c->side_fork, c->side_join});
     }
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
@@ -137,7 +138,7 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         st.ms_m2l = elapsed(c->ev[7], c->ev[4]);
         st.ms_downward = elapsed(c->ev[4], c->ev[5]);
         st.ms_leaf = elapsed(c->ev[5], c->ev[6]);
-        st.ms_p2p = elapsed(c->ev[8], c->ev[6]);
+        st.ms_p2p = elapsed(c->ev[8], c->ev[9]);
     }
     st.graph_instantiations = c->graph_instantiations;
     st.graph_updates = c->graph_updates;
@@ -200,6 +201,9 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
     const int r = guarded(c, [&]() -> int {
         AFMM_CHECK(hipSetDevice(prm->device));
         AFMM_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        AFMM_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->side_fork, hipEventDisableTiming));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->side_join, hipEventDisableTiming));
         for (auto& e : c->ev) AFMM_CHECK(hipEventCreate(&e));
         const Tables& T = tables_for_order(prm->p);
         const std::vector<double> m2m = mfma_a_layout(T.m2m, T.S, 8);
@@ -244,10 +248,14 @@ int afmm_set_graph(afmm_ctx* c, int enable) {
 void afmm_destroy(afmm_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     c->pool.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->side_fork) (void)hipEventDestroy(c->side_fork);
+    if (c->side_join) (void)hipEventDestroy(c->side_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -26,7 +26,10 @@ struct afmm_ctx {
     afmm::RootInfo root{};  // of system 0
     int nsys = 1;
     afmm_stats stats{};
-    hipEvent_t ev[9] = {};
+    hipEvent_t ev[10] = {};
+    // second stream for the P2P near field (run beside the far-field passes)
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
     // FMM passes replayed as a HIP graph (afmm_set_graph): the executable
     // graph of the last evaluation, updated in place when the next capture has
     // the same shape, re-instantiated otherwise

@@ -196,10 +196,19 @@ constexpr int kPrecFP64 = 0, kPrecMixed = 1, kPrecMixedBf16 = 2;
 // tv.sys_eps2). prec kPrecMixed: M2L as split-FP32 MFMA; kPrecMixedBf16: M2L
 // as three-piece bf16 MFMA; both with P2P in FP32 and FP64 accumulation;
 // everything else FP64.
-// ev: 7 events or null. reserve_only: take every buffer from the pool, launch nothing.
+// ev: 8 events or null. reserve_only: take every buffer from the pool, launch nothing.
+// side (optional): a second stream with two events; the P2P near field then
+// runs on it concurrently with the upward pass, M2L, downward pass, L2P and
+// M2P on s, and s joins it before the final combine. Without it (e.g. under
+// stream capture) P2P runs on s and combines in the same kernel; both give
+// the same bits.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
              const FmmTables& tab, int prec, double* d_acc, double* d_phi,
-             hipEvent_t* ev, bool reserve_only = false);
+             hipEvent_t* ev, bool reserve_only = false, SideStream side = SideStream{});
 
 // ---- integrate.hip -----------------------------------------------------------
 // One system of a leapfrog step: its resident arrays and step settings.

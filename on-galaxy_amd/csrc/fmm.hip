@@ -1025,10 +1025,13 @@ __device__ __forceinline__ void p2p_tile(const double* sx, const double* sy, con
     }
 }
 
-template <bool MIX, bool POT, int R>
+// NEAR: write the near field (sorted order) to nr/npot for a later combine
+// instead of combining with far here.
+template <bool MIX, bool POT, int R, bool NEAR>
 __device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const double* __restrict__ far,
                                          double* __restrict__ acc, const double* __restrict__ fpot,
-                                         double* __restrict__ phi, StreamLds& sl, double* sx, double* sy,
+                                         double* __restrict__ phi, double* __restrict__ nr,
+                                         double* __restrict__ npot, StreamLds& sl, double* sx, double* sy,
                                          double* sz, double* sm, int* sj, double (*red)[64]) {
     const int f = t.seg_first[b], nt = t.seg_count[b];
     const int lane = threadIdx.x;
@@ -1090,7 +1093,13 @@ __device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const
                 }
                 __syncthreads();
             }
-            if (act[r] && g == 0) {
+            if (NEAR && act[r] && g == 0) {
+                const int i = ii[r];
+                nr[3 * (size_t)i] = a[r].x;
+                nr[3 * (size_t)i + 1] = a[r].y;
+                nr[3 * (size_t)i + 2] = a[r].z;
+                if constexpr (POT) npot[i] = a[r].p;
+            } else if (act[r] && g == 0) {
                 const int i = ii[r];
                 const int o = t.orig[i];
                 acc[3 * (size_t)o] = G * (far[3 * (size_t)i] + a[r].x);
@@ -1102,9 +1111,10 @@ __device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const
     }
 }
 
-template <bool MIX, bool POT>
+template <bool MIX, bool POT, bool NEAR>
 __global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict__ far, double* __restrict__ acc,
-                                           const double* __restrict__ fpot, double* __restrict__ phi) {
+                                           const double* __restrict__ fpot, double* __restrict__ phi,
+                                           double* __restrict__ nr, double* __restrict__ npot) {
     __shared__ double sx[64], sy[64], sz[64], sm[64];
     __shared__ int sj[64];
     __shared__ StreamLds sl;
@@ -1112,8 +1122,47 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict
     const int li = xcd_chunk_item();
     if (li >= t.nleaves) return;
     const int b = t.leaf_ids[li];
-    if (t.seg_count[b] > 32) p2p_leaf<MIX, POT, 2>(t, b, li, far, acc, fpot, phi, sl, sx, sy, sz, sm, sj, red);
-    else p2p_leaf<MIX, POT, 1>(t, b, li, far, acc, fpot, phi, sl, sx, sy, sz, sm, sj, red);
+    if (t.seg_count[b] > 32)
+        p2p_leaf<MIX, POT, 2, NEAR>(t, b, li, far, acc, fpot, phi, nr, npot, sl, sx, sy, sz, sm, sj, red);
+    else
+        p2p_leaf<MIX, POT, 1, NEAR>(t, b, li, far, acc, fpot, phi, nr, npot, sl, sx, sy, sz, sm, sj, red);
+}
+
+// acc = G (far + near) and phi = -G (fpot + npot) per body in input order:
+// the expressions of the combining P2P, so that both paths give the same
+// bits. One wave per leaf (G is per system, systems are per box).
+template <bool POT>
+__global__ void __launch_bounds__(64) k_combine(TreeView t, const double* __restrict__ far,
+                                               const double* __restrict__ nr, const double* __restrict__ fpot,
+                                               const double* __restrict__ npot, double* __restrict__ acc,
+                                               double* __restrict__ phi) {
+    const int li = xcd_chunk_item();
+    if (li >= t.nleaves) return;
+    const int b = t.leaf_ids[li];
+    const int f = t.seg_first[b], nt = t.seg_count[b];
+    const double G = t.sys_G[t.bsys[b]];
+    for (int k = threadIdx.x; k < nt; k += 64) {
+        const size_t i = (size_t)(f + k);
+        const size_t o = (size_t)t.orig[i];
+        acc[3 * o] = G * (far[3 * i] + nr[3 * i]);
+        acc[3 * o + 1] = G * (far[3 * i + 1] + nr[3 * i + 1]);
+        acc[3 * o + 2] = G * (far[3 * i + 2] + nr[3 * i + 2]);
+        if constexpr (POT) phi[o] = -G * (fpot[i] + npot[i]);
+    }
+}
+
+template <bool NEAR>
+void launch_p2p(hipStream_t s, const TreeView& t, bool mixed, const double* far, double* acc, const double* fpot,
+                double* phi, double* nr, double* npot) {
+    const dim3 pg(xcd_chunk_grid(t.nleaves)), pb(64);
+    if (mixed && phi)
+        hipLaunchKernelGGL((k_p2p<true, true, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
+    else if (mixed)
+        hipLaunchKernelGGL((k_p2p<true, false, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
+    else if (phi)
+        hipLaunchKernelGGL((k_p2p<false, true, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
+    else
+        hipLaunchKernelGGL((k_p2p<false, false, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
 }
 
 // All device buffers are taken from the pool before the first launch, so that
@@ -1121,7 +1170,7 @@ __global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict
 // stream capture allocates nothing.
 template <int P>
 void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, int prec,
-         double* acc, double* phi, hipEvent_t* ev, bool reserve_only) {
+         double* acc, double* phi, hipEvent_t* ev, bool reserve_only, const SideStream& side) {
     const bool mixed = prec != kPrecFP64;
     constexpr int S = nterms(P);
     const int nb = t.nb;
@@ -1140,7 +1189,20 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
     uint4* Mbf = prec == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
                                         : nullptr;
+    const bool split = side.s != nullptr;
+    double* nr = split ? pool.get<double>("near", 3 * (size_t)t.n) : nullptr;
+    double* npot = split && phi ? pool.get<double>("nearpot", t.n) : nullptr;
     if (reserve_only) return;
+    if (split) {
+        // P2P needs only the tree and the U lists: it runs beside everything
+        // up to M2P on the second stream.
+        AFMM_CHECK(hipEventRecord(side.fork, s));
+        AFMM_CHECK(hipStreamWaitEvent(side.s, side.fork, 0));
+        if (ev) AFMM_CHECK(hipEventRecord(ev[6], side.s));
+        launch_p2p<true>(side.s, t, mixed, far, acc, fpot, phi, nr, npot);
+        if (ev) AFMM_CHECK(hipEventRecord(ev[7], side.s));
+        AFMM_CHECK(hipEventRecord(side.join, side.s));
+    }
     if (ev) AFMM_CHECK(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL((k_accumulate<P, 0>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M);
     for (int l = t.max_level - 1; l >= 0; --l) {
@@ -1190,16 +1252,16 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         if (fpot) hipLaunchKernelGGL((k_m2p<P, true>), mg, mb, 0, s, t, M, far, fpot);
         else hipLaunchKernelGGL((k_m2p<P, false>), mg, mb, 0, s, t, M, far, fpot);
     }
-    const dim3 pg(xcd_chunk_grid(t.nleaves)), pb(64);
-    if (ev) AFMM_CHECK(hipEventRecord(ev[6], s));
-    if (mixed && phi)
-        hipLaunchKernelGGL((k_p2p<true, true>), pg, pb, 0, s, t, far, acc, fpot, phi);
-    else if (mixed)
-        hipLaunchKernelGGL((k_p2p<true, false>), pg, pb, 0, s, t, far, acc, fpot, phi);
-    else if (phi)
-        hipLaunchKernelGGL((k_p2p<false, true>), pg, pb, 0, s, t, far, acc, fpot, phi);
-    else
-        hipLaunchKernelGGL((k_p2p<false, false>), pg, pb, 0, s, t, far, acc, fpot, phi);
+    if (split) {
+        AFMM_CHECK(hipStreamWaitEvent(s, side.join, 0));
+        const dim3 cg(xcd_chunk_grid(t.nleaves)), cb(64);
+        if (phi) hipLaunchKernelGGL(k_combine<true>, cg, cb, 0, s, t, far, nr, fpot, npot, acc, phi);
+        else hipLaunchKernelGGL(k_combine<false>, cg, cb, 0, s, t, far, nr, fpot, npot, acc, phi);
+    } else {
+        if (ev) AFMM_CHECK(hipEventRecord(ev[6], s));
+        launch_p2p<false>(s, t, mixed, far, acc, fpot, phi, nullptr, nullptr);
+        if (ev) AFMM_CHECK(hipEventRecord(ev[7], s));
+    }
     if (ev) AFMM_CHECK(hipEventRecord(ev[4], s));
     AFMM_CHECK(hipGetLastError());
 }
@@ -1207,12 +1269,12 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
 }  // namespace
 
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
-             int prec, double* d_acc, double* d_phi, hipEvent_t* ev, bool reserve_only) {
+             int prec, double* d_acc, double* d_phi, hipEvent_t* ev, bool reserve_only, SideStream side) {
     if (prec == kPrecMixed && !tab.m2l_hl) throw std::runtime_error("split FP32 M2L operators not loaded");
     if (prec == kPrecMixedBf16 && !tab.m2l_bf) throw std::runtime_error("bf16 M2L operators not loaded");
     switch (p) {
 #define AFMM_CASE(PP) \
-    case PP: run<PP>(pool, s, tv, meta, tab, prec, d_acc, d_phi, ev, reserve_only); return;
+    case PP: run<PP>(pool, s, tv, meta, tab, prec, d_acc, d_phi, ev, reserve_only, side); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
 #undef AFMM_CASE
