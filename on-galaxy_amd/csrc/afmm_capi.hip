@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -35,6 +37,21 @@ int check_params(const afmm_params* p) {
         p->precision != AFMM_PRECISION_MIXED_BF16)
         return AFMM_ERR_PRECISION;
     return AFMM_OK;
+}
+
+// The second stream for P2P, one per device for the whole process (created on
+// first use, never destroyed): contexts share it through their own fork and
+// join events, so extra contexts add no streams (each stream beyond the few
+// hardware queues shares a queue with another and can serialise behind it).
+hipStream_t shared_side_stream() {
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    int dev = 0;
+    AFMM_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    hipStream_t& st = streams[dev];
+    if (!st) AFMM_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    return st;
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -106,8 +123,13 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         AFMM_CHECK(hipEventRecord(c->ev[6], s));
         (updated ? c->graph_updates : c->graph_instantiations) += 1;
     } else {
+        if (c->split_p2p && !c->side) {
+            AFMM_CHECK(hipEventCreateWithFlags(&c->side_fork, hipEventDisableTiming));
+            AFMM_CHECK(hipEventCreateWithFlags(&c->side_join, hipEventDisableTiming));
+            c->side = shared_side_stream();
+        }
         run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2, false,
-                SideStream{c->side, c->side_fork, c->side_join});
+                c->split_p2p ? SideStream{c->side, c->side_fork, c->side_join} : SideStream{});
     }
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
@@ -201,9 +223,6 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
     const int r = guarded(c, [&]() -> int {
         AFMM_CHECK(hipSetDevice(prm->device));
         AFMM_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        AFMM_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-        AFMM_CHECK(hipEventCreateWithFlags(&c->side_fork, hipEventDisableTiming));
-        AFMM_CHECK(hipEventCreateWithFlags(&c->side_join, hipEventDisableTiming));
         for (auto& e : c->ev) AFMM_CHECK(hipEventCreate(&e));
         const Tables& T = tables_for_order(prm->p);
         const std::vector<double> m2m = mfma_a_layout(T.m2m, T.S, 8);
@@ -248,14 +267,13 @@ int afmm_set_graph(afmm_ctx* c, int enable) {
 void afmm_destroy(afmm_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->side) (void)hipEventSynchronize(c->side_join);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     c->pool.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->side_fork) (void)hipEventDestroy(c->side_fork);
     if (c->side_join) (void)hipEventDestroy(c->side_join);
-    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

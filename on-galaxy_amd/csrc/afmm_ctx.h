@@ -27,7 +27,10 @@ struct afmm_ctx {
     int nsys = 1;
     afmm_stats stats{};
     hipEvent_t ev[10] = {};
-    // second stream for the P2P near field (run beside the far-field passes)
+    // second stream for the P2P near field (run beside the far-field passes):
+    // the process's shared one, taken on first use; off for the request
+    // server's workers, which already run several evaluations side by side
+    bool split_p2p = true;
     hipStream_t side = nullptr;
     hipEvent_t side_fork = nullptr, side_join = nullptr;
     // FMM passes replayed as a HIP graph (afmm_set_graph): the executable

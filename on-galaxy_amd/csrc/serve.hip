@@ -155,6 +155,7 @@ Worker* worker_for(afmm_server* sv, int p, int* rc) {
     afmm_params prm{1.0, 0.0, p, sv->prm.leaf_cap, sv->prm.precision, sv->prm.device};
     *rc = afmm_create(&prm, &w->ctx);
     if (*rc) return nullptr;
+    w->ctx->split_p2p = false;  // one stream per worker: the workers share the GPU
     Worker* raw = w.get();
     sv->workers.emplace(p, std::move(w));
     raw->th = std::thread(worker_loop, sv, raw);
