@@ -80,13 +80,33 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     TreeView tv{};
     TreeMeta meta;
     std::vector<RootInfo> roots;
-    build_forest(c->pool, s, pos, mass, nsys, sys_off, eps, c->prm.leaf_cap, tv, meta, roots);
+    // Outside graph mode the upward pass runs on the second stream while the
+    // interaction lists are built (both leave most of the GPU idle).
+    const bool split = c->split_p2p && !c->use_graph;
+    if (split && !c->side) {
+        AFMM_CHECK(hipEventCreateWithFlags(&c->side_fork, hipEventDisableTiming));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->side_join, hipEventDisableTiming));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->up_fork, hipEventDisableTiming));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->up_done, hipEventDisableTiming));
+        c->side = shared_side_stream();
+    }
+    FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw, c->d_m2l_hl, c->d_m2l_bf};
+    bool up_launched = false;
+    auto on_core = [&](const TreeView& t, const TreeMeta& m) {
+        AFMM_CHECK(hipEventRecord(c->up_fork, s));
+        AFMM_CHECK(hipStreamWaitEvent(c->side, c->up_fork, 0));
+        run_upward(c->pool, c->side, t, m, c->prm.p, tab, c->ev[2], c->ev[3]);
+        AFMM_CHECK(hipEventRecord(c->up_done, c->side));
+        up_launched = true;
+    };
+    build_forest(c->pool, s, pos, mass, nsys, sys_off, eps, c->prm.leaf_cap, tv, meta, roots,
+                 split ? std::function<void(const TreeView&, const TreeMeta&)>(on_core)
+                       : std::function<void(const TreeView&, const TreeMeta&)>());
     for (const RootInfo& r : roots)
         if (r.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
     tv.sys_G = d_ge;
     tv.sys_eps2 = d_ge + nsys;
     AFMM_CHECK(hipEventRecord(c->ev[1], s));
-    FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw, c->d_m2l_hl, c->d_m2l_bf};
     if (c->use_graph) {
         // size every buffer first, so that the capture allocates nothing
         run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr, true);
@@ -123,13 +143,9 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         AFMM_CHECK(hipEventRecord(c->ev[6], s));
         (updated ? c->graph_updates : c->graph_instantiations) += 1;
     } else {
-        if (c->split_p2p && !c->side) {
-            AFMM_CHECK(hipEventCreateWithFlags(&c->side_fork, hipEventDisableTiming));
-            AFMM_CHECK(hipEventCreateWithFlags(&c->side_join, hipEventDisableTiming));
-            c->side = shared_side_stream();
-        }
-        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2, false,
-                c->split_p2p ? SideStream{c->side, c->side_fork, c->side_join} : SideStream{});
+        SideStream side{};
+        if (split) side = SideStream{c->side, c->side_fork, c->side_join, up_launched, c->up_done};
+        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2, false, side);
     }
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
@@ -272,8 +288,8 @@ void afmm_destroy(afmm_ctx* c) {
     c->pool.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    if (c->side_fork) (void)hipEventDestroy(c->side_fork);
-    if (c->side_join) (void)hipEventDestroy(c->side_join);
+    for (hipEvent_t e : {c->side_fork, c->side_join, c->up_fork, c->up_done})
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

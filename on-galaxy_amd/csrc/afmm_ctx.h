@@ -27,12 +27,13 @@ struct afmm_ctx {
     int nsys = 1;
     afmm_stats stats{};
     hipEvent_t ev[10] = {};
-    // second stream for the P2P near field (run beside the far-field passes):
-    // the process's shared one, taken on first use; off for the request
-    // server's workers, which already run several evaluations side by side
+    // second stream (the process's shared one, taken on first use) for the
+    // upward pass beside the list build and the P2P near field beside the
+    // far-field passes; off in graph mode and for the request server's
+    // workers, which already run several evaluations side by side
     bool split_p2p = true;
     hipStream_t side = nullptr;
-    hipEvent_t side_fork = nullptr, side_join = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr, up_fork = nullptr, up_done = nullptr;
     // FMM passes replayed as a HIP graph (afmm_set_graph): the executable
     // graph of the last evaluation, updated in place when the next capture has
     // the same shape, re-instantiated otherwise

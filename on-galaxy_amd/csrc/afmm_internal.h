@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -174,9 +175,13 @@ int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, 
 // tolerance and leaf-edge floor h_eps_floor[q]; box q of level 0 is its root.
 // Each tree, its box numbering (refid) and its lists are those the reference
 // builds for that system alone. roots[q].nonfinite > 0 aborts the build.
+// on_core (optional) is called once the boxes, bodies, geometry and level
+// orders are enqueued and before the interaction lists are: everything the
+// upward pass reads is then in place on s.
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
-                  std::vector<RootInfo>& roots);
+                  std::vector<RootInfo>& roots,
+                  const std::function<void(const TreeView&, const TreeMeta&)>& on_core = {});
 
 // ---- fmm.hip -----------------------------------------------------------------
 struct FmmTimes {
@@ -205,7 +210,15 @@ constexpr int kPrecFP64 = 0, kPrecMixed = 1, kPrecMixedBf16 = 2;
 struct SideStream {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // set when run_upward already enqueued P2M/M2M on s (during the list
+    // build); up_done is recorded after them and s is waited on before M2L
+    bool upward_launched = false;
+    hipEvent_t up_done = nullptr;
 };
+// P2M and M2M alone on stream s (events e0/e1 around them when non-null); a
+// run_fmm with side.upward_launched then skips them.
+void run_upward(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
+                hipEvent_t e0, hipEvent_t e1);
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
              const FmmTables& tab, int prec, double* d_acc, double* d_phi,
              hipEvent_t* ev, bool reserve_only = false, SideStream side = SideStream{});

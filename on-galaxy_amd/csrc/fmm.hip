@@ -1165,6 +1165,22 @@ void launch_p2p(hipStream_t s, const TreeView& t, bool mixed, const double* far,
         hipLaunchKernelGGL((k_p2p<false, false, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
 }
 
+// P2M into M, then M2M level by level (upward pass).
+template <int P>
+void upward(hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, double* M, hipEvent_t e0,
+            hipEvent_t e1) {
+    if (e0) AFMM_CHECK(hipEventRecord(e0, s));
+    hipLaunchKernelGGL((k_accumulate<P, 0>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M);
+    for (int l = t.max_level - 1; l >= 0; --l) {
+        const int off = meta.nonleaf_off[l], cnt = meta.nonleaf_off[l + 1] - off;
+        if (cnt > 0)
+            hipLaunchKernelGGL((k_level_mfma<P, 0>), dim3((cnt + 15) / 16), dim3(kLevelWaves * 64), 0, s, cnt,
+                               t.nonleaf_ids + off,
+                               t.child_first, t.child_mask, t.parent, t.octant, tab.m2m_sw, M, M);
+    }
+    if (e1) AFMM_CHECK(hipEventRecord(e1, s));
+}
+
 // All device buffers are taken from the pool before the first launch, so that
 // a call with reserve_only (no launches) sizes them and a following call under
 // stream capture allocates nothing.
@@ -1203,16 +1219,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         if (ev) AFMM_CHECK(hipEventRecord(ev[7], side.s));
         AFMM_CHECK(hipEventRecord(side.join, side.s));
     }
-    if (ev) AFMM_CHECK(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL((k_accumulate<P, 0>), dim3(xcd_chunk_grid(t.nleaves)), dim3(64), 0, s, t, M);
-    for (int l = t.max_level - 1; l >= 0; --l) {
-        const int off = meta.nonleaf_off[l], cnt = meta.nonleaf_off[l + 1] - off;
-        if (cnt > 0)
-            hipLaunchKernelGGL((k_level_mfma<P, 0>), dim3((cnt + 15) / 16), dim3(kLevelWaves * 64), 0, s, cnt,
-                               t.nonleaf_ids + off,
-                               t.child_first, t.child_mask, t.parent, t.octant, tab.m2m_sw, M, M);
-    }
-    if (ev) AFMM_CHECK(hipEventRecord(ev[1], s));
+    if (!side.upward_launched) upward<P>(s, t, meta, tab, M, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     {
         // M2L targets in (level, octant) order; transposed source table; tile plan
         hipLaunchKernelGGL(k_rank_of, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.m2l_order, rank);
@@ -1221,6 +1228,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         hipLaunchKernelGGL(k_m2l_src, dim3(blocks((long long)nb * kVSlots)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv,
                            t.ic, srcT);
         hipLaunchKernelGGL(k_m2l_plan, dim3(ntiles * kNumOffsets), dim3(64), 0, s, tstart, np, srcT, present);
+        if (side.upward_launched) AFMM_CHECK(hipStreamWaitEvent(s, side.up_done, 0));  // M is complete
         if (prec == kPrecMixedBf16) {
             constexpr int KB = m2l_bf_kb(S);
             hipLaunchKernelGGL(k_split_bf16<P>, dim3(blocks((long long)nb * KB * 4)), dim3(kT), 0, s, nb, M, Mbf);
@@ -1267,6 +1275,19 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
 }
 
 }  // namespace
+
+void run_upward(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
+                hipEvent_t e0, hipEvent_t e1) {
+    double* M = pool.get<double>("Mexp", (size_t)tv.nb * nterms(p));
+    switch (p) {
+#define AFMM_CASE(PP) \
+    case PP: upward<PP>(s, tv, meta, tab, M, e0, e1); return;
+        AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
+        AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+#undef AFMM_CASE
+    }
+    throw std::runtime_error("unsupported expansion order");
+}
 
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
              int prec, double* d_acc, double* d_phi, hipEvent_t* ev, bool reserve_only, SideStream side) {

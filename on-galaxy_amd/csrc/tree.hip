@@ -1035,7 +1035,8 @@ int read_int(const int* d, hipStream_t s) {
 
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
-                  std::vector<RootInfo>& roots) {
+                  std::vector<RootInfo>& roots,
+                  const std::function<void(const TreeView&, const TreeMeta&)>& on_core) {
     const int n = h_sys_off[nsys];
     // ---- per-system root cubes: chunks of at most kBBoxChunk bodies of one system
     std::vector<int> hb;  // [cbeg | cend | cfirst(nsys+1) | sys_off(nsys+1)] as one upload
@@ -1330,6 +1331,8 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_out + nlev, 128, tv.m2l_ntiles128,
                            tv.m2l_tiles128);
     }
+
+    if (on_core) on_core(tv, meta);
 
     // ---- colleagues, V, M2L source table
     tv.coll = pool.get<int>("coll", kCollSlots * (size_t)nb);
