@@ -13,39 +13,10 @@
 #include <algorithm>
 
 #include "afmm_internal.h"
+#include "block_scan.h"
 
 namespace afmm {
 namespace {
-
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
-
-// Exclusive scan of one int per thread over the block: wave inclusive scans
-// (shuffles), then the wave totals through LDS (lds: at least kScanThreads / 64
-// ints). Integer sums: the result does not depend on the order.
-__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total) {
-    constexpr int kW = kScanThreads / kWave;
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
-    int incl = v;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const int t = __shfl_up(incl, d, kWave);
-        if (lane >= d) incl += t;
-    }
-    __syncthreads();  // lds free from a previous call
-    if (lane == kWave - 1) lds[w] = incl;
-    __syncthreads();
-    int before = 0;
-    total = 0;
-#pragma unroll
-    for (int k = 0; k < kW; ++k) {
-        const int t = lds[k];
-        before += k < w ? t : 0;
-        total += t;
-    }
-    return before + incl - v;
-}
 
 // rng (optional, device): the item count is rng[1] - rng[0] instead of n, and
 // blocks past it only write zero tile sums.
@@ -76,10 +47,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int* in, int 
     n = scan_count(n, rng);
     const int last = n > 0 ? (n - 1) / kScanTile : 0;
     if ((int)blockIdx.x > last) return;
-    int part = 0;
-    for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanThreads) part += sums[k];
-    int tile_base;
-    block_exclusive_scan(part, lds, tile_base);
+    const int tile_base = block_prefix_of_tiles(sums, blockIdx.x, lds);
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
     int s = 0;

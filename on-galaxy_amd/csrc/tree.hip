@@ -31,6 +31,7 @@
 #include <cmath>
 
 #include "afmm_internal.h"
+#include "block_scan.h"
 #include "afmm_math.h"
 
 #pragma clang fp contract(off)
@@ -287,6 +288,8 @@ __global__ void __launch_bounds__(kT) k_emit(int off, int cnt, int l, int next_o
 // slots beyond the box arrays' capacity set *overflow and are not written.
 // Level range of lev[l] clamped to the buffers (maxcnt entries per level,
 // `capacity` boxes); a clamp can only follow an overflow, which the host sees.
+constexpr int kEmitTile = kScanThreads;  // boxes per emit block, one per thread
+
 __device__ __forceinline__ int level_count(const int* lev, int l, int maxcnt, int capacity) {
     const int off = lev[l];
     return max(0, min(min(lev[l + 1] - off, maxcnt), capacity - off));
@@ -295,7 +298,8 @@ __device__ __forceinline__ int level_count(const int* lev, int l, int maxcnt, in
 __global__ void __launch_bounds__(kT) k_split_dev(const int* __restrict__ lev, int l, int maxcnt, int capacity,
                                                  int cap, const double* eps_floor, const RootInfo* roots,
                                                  const int* bsys, const int* seg_first, const int* seg_count,
-                                                 const uint64_t* khi, const uint32_t* klo, int* bnd, int* nch) {
+                                                 const uint64_t* khi, const uint32_t* klo, int* bnd, int* nch,
+                                                 int* tsum) {
     const int off = lev[l], cnt = level_count(lev, l, maxcnt, capacity);
     for (long long gt = (long long)blockIdx.x * kT + threadIdx.x; gt < 8LL * ((cnt + 7) / 8 * 8);
          gt += (long long)gridDim.x * kT) {
@@ -325,6 +329,12 @@ __global__ void __launch_bounds__(kT) k_split_dev(const int* __restrict__ lev, i
         int nonempty = (split && end > lo) ? 1 : 0;
 #pragma unroll
         for (int m = 1; m < 8; m <<= 1) nonempty += __shfl_xor(nonempty, m, 8);
+        // children of the wave's 8 boxes (one tile: tiles are multiples of 8
+        // boxes) into that tile's total; integer sums in any order
+        int wsum = (o == 0 && live && split) ? nonempty : 0;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) wsum += __shfl_xor(wsum, m, 64);
+        if ((threadIdx.x & 63) == 0 && wsum) atomicAdd(&tsum[(int)((gt >> 3) / kEmitTile)], wsum);
         if (!live) continue;
         int* bd = bnd + 9 * (size_t)i;
         if (split) {
@@ -335,24 +345,39 @@ __global__ void __launch_bounds__(kT) k_split_dev(const int* __restrict__ lev, i
     }
 }
 
-__global__ void __launch_bounds__(kT) k_emit_dev(const int* __restrict__ lev, int l, int maxcnt, int capacity,
-                                                const int* bnd, const int* nch, const int* cbase, int* seg_first,
-                                                int* seg_count, int* level, int* parent, int* octant,
-                                                int* child_first, int* nchild, unsigned char* mask, int* bsys,
-                                                int* overflow) {
+// Children of level l, one box per thread: each block scans its tile of
+// kEmitTile child counts (offset: the tile totals before it, summed by the
+// split), writes the children of its boxes and zeroes the next level's tile
+// totals; the block of the last tile writes lev[l + 2].
+__global__ void __launch_bounds__(kScanThreads) k_emit_dev(int* __restrict__ lev, int l, int maxcnt, int capacity,
+                                                          const int* bnd, const int* nch, const int* tsum,
+                                                          int* tsum_next, int ntiles, int* seg_first,
+                                                          int* seg_count, int* level, int* parent, int* octant,
+                                                          int* child_first, int* nchild, unsigned char* mask,
+                                                          int* bsys, int* overflow) {
+    __shared__ int lds[kScanThreads / kWave];
     const int off = lev[l], cnt = level_count(lev, l, maxcnt, capacity), next_off = lev[l + 1];
     if (blockIdx.x == 0 && threadIdx.x == 0 && cnt != lev[l + 1] - off) atomicOr(overflow, 2);
-    for (int i = blockIdx.x * kT + threadIdx.x; i < cnt; i += gridDim.x * kT) {
+    for (int k = blockIdx.x * kScanThreads + threadIdx.x; k < ntiles; k += gridDim.x * kScanThreads)
+        tsum_next[k] = 0;
+    const int last = cnt > 0 ? (cnt - 1) / kEmitTile : 0;
+    if ((int)blockIdx.x > last) return;
+    const int tile_base = block_prefix_of_tiles(tsum, blockIdx.x, lds);
+    const int i = blockIdx.x * kEmitTile + threadIdx.x;
+    const int kc = i < cnt ? nch[i] : 0;
+    int total;
+    const int ex = block_exclusive_scan(kc, lds, total) + tile_base;
+    if ((int)blockIdx.x == last && threadIdx.x == 0) lev[l + 2] = next_off + tile_base + total;
+    if (i < cnt) {
         const int b = off + i;
-        const int k = nch[i];
-        nchild[b] = k;
-        if (k == 0) {
+        nchild[b] = kc;
+        if (kc == 0) {
             child_first[b] = -1;
             mask[b] = 0;
-            continue;
+            return;
         }
         const int* bd = bnd + 9 * (size_t)i;
-        const int first = next_off + cbase[i];
+        const int first = next_off + ex;
         child_first[b] = first;
         unsigned char mk = 0;
         int r = 0;
@@ -385,12 +410,13 @@ __global__ void __launch_bounds__(kT) k_fill_leaf_range(int off, int cnt, int* n
     mask[off + i] = 0;
 }
 
-__global__ void k_level_init(int* lev, int nsys, int* overflow) {
+__global__ void k_level_init(int* lev, int nsys, int* overflow, int* tsum0, int ntiles) {
     if (threadIdx.x == 0) {
         lev[0] = 0;
         lev[1] = nsys;
         *overflow = 0;
     }
+    for (int k = threadIdx.x; k < ntiles; k += blockDim.x) tsum0[k] = 0;
 }
 
 // Box q of level 0 is the root of system q (bodies [sys_off[q], sys_off[q+1])).
@@ -1144,19 +1170,21 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
             int* lev = pool.get<int>("level_dev", L + 3);
             int* bnd = pool.get<int>("split_bnd", 9 * (size_t)maxcnt);
             int* nch = pool.get<int>("split_nch", maxcnt);
-            int* cbase = pool.get<int>("split_base", maxcnt);
             int* ovf = lev + L + 2;
-            int* stmp = pool.get<int>("level_scan_tmp", scan_tmp_ints(maxcnt) + 16);
-            hipLaunchKernelGGL(k_level_init, dim3(1), dim3(64), 0, s, lev, nsys, ovf);
+            // child-count totals per kEmitTile boxes, two levels' worth alternating
+            const int ntiles = (maxcnt + kEmitTile - 1) / kEmitTile;
+            int* tsum = pool.get<int>("level_tile_sums", 2 * (size_t)ntiles);
+            hipLaunchKernelGGL(k_level_init, dim3(1), dim3(256), 0, s, lev, nsys, ovf, tsum, ntiles);
             const int g8 = std::max(1, std::min(4096, (int)((8LL * maxcnt + kT - 1) / kT)));
-            const int g1 = std::max(1, std::min(4096, (int)((maxcnt + kT - 1) / kT)));
             for (int lv = 0; lv < L; ++lv) {
+                int* ts = tsum + (size_t)(lv & 1) * ntiles;
+                int* ts_next = tsum + (size_t)((lv + 1) & 1) * ntiles;
                 hipLaunchKernelGGL(k_split_dev, dim3(g8), dim3(kT), 0, s, lev, lv, maxcnt, (int)cap_boxes, cap,
-                                   d_eps_floor, d_roots, tv.bsys, tv.seg_first, tv.seg_count, khi, klo, bnd, nch);
-                scan_exclusive_range(nch, cbase, lev + lv, maxcnt, lev + lv + 2, stmp, s);
-                hipLaunchKernelGGL(k_emit_dev, dim3(g1), dim3(kT), 0, s, lev, lv, maxcnt, (int)cap_boxes, bnd, nch, cbase,
-                                   tv.seg_first, tv.seg_count, tv.level, tv.parent, tv.octant, tv.child_first,
-                                   tv.nchild, tv.child_mask, tv.bsys, ovf);
+                                   d_eps_floor, d_roots, tv.bsys, tv.seg_first, tv.seg_count, khi, klo, bnd, nch, ts);
+                hipLaunchKernelGGL(k_emit_dev, dim3(ntiles), dim3(kScanThreads), 0, s, lev, lv, maxcnt,
+                                   (int)cap_boxes, bnd, nch, ts, ts_next, ntiles, tv.seg_first, tv.seg_count,
+                                   tv.level, tv.parent, tv.octant, tv.child_first, tv.nchild, tv.child_mask,
+                                   tv.bsys, ovf);
             }
             std::vector<int> hl(L + 3);
             AFMM_CHECK(hipMemcpyAsync(hl.data(), lev, sizeof(int) * (L + 3), hipMemcpyDeviceToHost, s));
