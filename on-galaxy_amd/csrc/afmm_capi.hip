@@ -99,9 +99,15 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         AFMM_CHECK(hipEventRecord(c->up_done, c->side));
         up_launched = true;
     };
+    // ... and the M2L preparation is enqueued right after the V lists, ahead of
+    // the U, W and X lists that M2L does not need.
+    bool prep_done = false;
+    auto on_vlists = [&](const TreeView& t, const TreeMeta&) {
+        run_m2l_prep(c->pool, s, t, c->prm.p, c->prm.precision);
+        prep_done = true;
+    };
     build_forest(c->pool, s, pos, mass, nsys, sys_off, eps, c->prm.leaf_cap, tv, meta, roots,
-                 split ? std::function<void(const TreeView&, const TreeMeta&)>(on_core)
-                       : std::function<void(const TreeView&, const TreeMeta&)>());
+                 split ? TreeHook(on_core) : TreeHook(), split ? TreeHook(on_vlists) : TreeHook());
     for (const RootInfo& r : roots)
         if (r.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
     tv.sys_G = d_ge;
@@ -144,7 +150,7 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         (updated ? c->graph_updates : c->graph_instantiations) += 1;
     } else {
         SideStream side{};
-        if (split) side = SideStream{c->side, c->side_fork, c->side_join, up_launched, c->up_done};
+        if (split) side = SideStream{c->side, c->side_fork, c->side_join, up_launched, c->up_done, prep_done};
         run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2, false, side);
     }
     AFMM_CHECK(hipStreamSynchronize(s));

@@ -177,11 +177,12 @@ int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, 
 // builds for that system alone. roots[q].nonfinite > 0 aborts the build.
 // on_core (optional) is called once the boxes, bodies, geometry and level
 // orders are enqueued and before the interaction lists are: everything the
-// upward pass reads is then in place on s.
+// upward pass reads is then in place on s. on_vlists (optional) is called
+// once the colleague and V lists are enqueued, before U, W and X.
+using TreeHook = std::function<void(const TreeView&, const TreeMeta&)>;
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
-                  std::vector<RootInfo>& roots,
-                  const std::function<void(const TreeView&, const TreeMeta&)>& on_core = {});
+                  std::vector<RootInfo>& roots, const TreeHook& on_core = {}, const TreeHook& on_vlists = {});
 
 // ---- fmm.hip -----------------------------------------------------------------
 struct FmmTimes {
@@ -214,7 +215,13 @@ struct SideStream {
     // build); up_done is recorded after them and s is waited on before M2L
     bool upward_launched = false;
     hipEvent_t up_done = nullptr;
+    // set when run_m2l_prep already enqueued the M2L preparation on the
+    // evaluation's own stream (after the V lists)
+    bool prep_done = false;
 };
+// The M2L preparation alone on stream s (needs the V lists and the M2L order);
+// a run_fmm with side.prep_done then skips it.
+void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec);
 // P2M and M2M alone on stream s (events e0/e1 around them when non-null); a
 // run_fmm with side.upward_launched then skips them.
 void run_upward(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,

@@ -1165,6 +1165,35 @@ void launch_p2p(hipStream_t s, const TreeView& t, bool mixed, const double* far,
         hipLaunchKernelGGL((k_p2p<false, false, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
 }
 
+// M2L preparation: target ranks in the M2L order, the transposed source table
+// srcT[offset * np + rank] (-1 where a target has no source at that offset)
+// and the per-tile offset presence table.
+void m2l_prep(hipStream_t s, const TreeView& t, int np, int ntiles, const int* tstart, int* rank, int* srcT,
+              unsigned char* present) {
+    const int nb = t.nb;
+    hipLaunchKernelGGL(k_rank_of, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.m2l_order, rank);
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks((long long)kNumOffsets * np)), dim3(kT), 0, s, srcT,
+                       (size_t)kNumOffsets * np, -1);
+    hipLaunchKernelGGL(k_m2l_src, dim3(blocks((long long)nb * kVSlots)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv,
+                       t.ic, srcT);
+    hipLaunchKernelGGL(k_m2l_plan, dim3(ntiles * kNumOffsets), dim3(64), 0, s, tstart, np, srcT, present);
+}
+
+// The M2L buffers of order P and precision prec, taken from the pool under
+// the names run<P> uses.
+template <int P>
+void m2l_prep_for(Pool& pool, hipStream_t s, const TreeView& t, int prec) {
+    const int nb = t.nb, np = nb;
+    const int TILE = prec == kPrecMixedBf16 ? kM2LBfTile : m2l_tile<P>();
+    if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
+    const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
+    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
+    int* rank = pool.get<int>("m2l_rank", nb);
+    int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
+    unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
+    m2l_prep(s, t, np, ntiles, tstart, rank, srcT, present);
+}
+
 // P2M into M, then M2M level by level (upward pass).
 template <int P>
 void upward(hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, double* M, hipEvent_t e0,
@@ -1221,13 +1250,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     }
     if (!side.upward_launched) upward<P>(s, t, meta, tab, M, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     {
-        // M2L targets in (level, octant) order; transposed source table; tile plan
-        hipLaunchKernelGGL(k_rank_of, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.m2l_order, rank);
-        hipLaunchKernelGGL(k_fill_int, dim3(blocks((long long)kNumOffsets * np)), dim3(kT), 0, s, srcT,
-                           (size_t)kNumOffsets * np, -1);
-        hipLaunchKernelGGL(k_m2l_src, dim3(blocks((long long)nb * kVSlots)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv,
-                           t.ic, srcT);
-        hipLaunchKernelGGL(k_m2l_plan, dim3(ntiles * kNumOffsets), dim3(64), 0, s, tstart, np, srcT, present);
+        if (!side.prep_done) m2l_prep(s, t, np, ntiles, tstart, rank, srcT, present);
         if (side.upward_launched) AFMM_CHECK(hipStreamWaitEvent(s, side.up_done, 0));  // M is complete
         if (prec == kPrecMixedBf16) {
             constexpr int KB = m2l_bf_kb(S);
@@ -1282,6 +1305,17 @@ void run_upward(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& m
     switch (p) {
 #define AFMM_CASE(PP) \
     case PP: upward<PP>(s, tv, meta, tab, M, e0, e1); return;
+        AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
+        AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+#undef AFMM_CASE
+    }
+    throw std::runtime_error("unsupported expansion order");
+}
+
+void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec) {
+    switch (p) {
+#define AFMM_CASE(PP) \
+    case PP: m2l_prep_for<PP>(pool, s, tv, prec); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
 #undef AFMM_CASE

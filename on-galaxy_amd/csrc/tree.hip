@@ -1061,8 +1061,7 @@ int read_int(const int* d, hipStream_t s) {
 
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
-                  std::vector<RootInfo>& roots,
-                  const std::function<void(const TreeView&, const TreeMeta&)>& on_core) {
+                  std::vector<RootInfo>& roots, const TreeHook& on_core, const TreeHook& on_vlists) {
     const int n = h_sys_off[nsys];
     // ---- per-system root cubes: chunks of at most kBBoxChunk bodies of one system
     std::vector<int> hb;  // [cbeg | cend | cfirst(nsys+1) | sys_off(nsys+1)] as one upload
@@ -1376,6 +1375,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     }
     // (the overflow flag is read back with the list sizes below; list counts
     // are capped at their slots, so the kernels in between stay in bounds)
+    if (on_vlists) on_vlists(tv, meta);
 
     // ---- U and W per leaf, X per box (count, scan, one size readback, fill)
     const int nl = tv.nleaves;
