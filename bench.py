@@ -120,9 +120,18 @@ def drift_leg(afmm, ics, args):
     return out
 
 
+def rel_errors(ref, a):
+    """RMS and max-rel (floor 1e-30 + 1e-6 max|ref|, adaptive_gpu_harness.cu:36-46)."""
+    rn = np.linalg.norm(ref, axis=1)
+    floor = 1e-30 + 1e-6 * rn.max()
+    d = np.linalg.norm(a - ref, axis=1)
+    return {"rms": float(np.sqrt((d ** 2).sum() / (rn ** 2).sum())), "max_rel": float((d / np.maximum(rn, floor)).max())}
+
+
 def large_leg(afmm, ics, torch, args):
-    """C2 at N = 4M (2 x 2M merger): s/eval of FP64 and both mixed forms, and
-    the eps = 0 truncation error of each against FP64 direct on 4096 targets."""
+    """C2 at N = 4M (2 x 2M merger): s/eval of FP64 and both mixed forms at
+    p = 8, FP64 and bf16 mixed at p = 4 and 12, and the eps = 0 truncation error
+    of each against FP64 direct on 4096 targets."""
     pos, _, mass, _ = ics.merger(args.large_n // 2, seed=args.seed + 9)
     n = len(mass)
     dev = torch.device("cuda:0")
@@ -132,9 +141,13 @@ def large_leg(afmm, ics, torch, args):
     idx = np.random.default_rng(args.seed + 10).choice(n, 4096, replace=False)
     out = {"n": n}
     ref0 = None
-    for name, prec in (("fp64", afmm.PRECISION_FP64), ("mixed_fp32_split", afmm.PRECISION_MIXED),
-                       ("mixed_bf16_split", afmm.PRECISION_MIXED_BF16)):
-        s = afmm.AdaptiveFMM3D(eps=args.eps, p=args.p, threshold=args.cap, precision=prec)
+    forms = [(name, prec, args.p) for name, prec in (("fp64", afmm.PRECISION_FP64),
+                                                       ("mixed_fp32_split", afmm.PRECISION_MIXED),
+                                                       ("mixed_bf16_split", afmm.PRECISION_MIXED_BF16))]
+    for p in (4, 12):
+        forms += [(f"p{p}_fp64", afmm.PRECISION_FP64, p), (f"p{p}_mixed_bf16_split", afmm.PRECISION_MIXED_BF16, p)]
+    for name, prec, p in forms:
+        s = afmm.AdaptiveFMM3D(eps=args.eps, p=p, threshold=args.cap, precision=prec)
         s.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -142,19 +155,19 @@ def large_leg(afmm, ics, torch, args):
             s.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / 3
-        s0 = afmm.AdaptiveFMM3D(eps=0.0, p=args.p, threshold=args.cap, precision=prec)
+        s0 = afmm.AdaptiveFMM3D(eps=0.0, p=p, threshold=args.cap, precision=prec)
         s0.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
         torch.cuda.synchronize()
         a0 = d_acc.cpu().numpy()[idx]
         if ref0 is None:
             ref0 = s0.direct(pos, mass, idx)
-        d = np.linalg.norm(a0 - ref0, axis=1)
-        rn = np.linalg.norm(ref0, axis=1)
-        out[name] = {"s_per_eval": t, "eps0_rms_vs_direct": float(np.sqrt((d ** 2).sum() / (rn ** 2).sum()))}
+        e = rel_errors(ref0, a0)
+        out[name] = {"p": p, "s_per_eval": t, "eps0_rms_vs_direct": e["rms"], "eps0_max_rel_vs_direct": e["max_rel"]}
         s.close()
         s0.close()
-    for k in ("mixed_fp32_split", "mixed_bf16_split"):
-        out[k]["speedup_vs_fp64"] = out["fp64"]["s_per_eval"] / out[k]["s_per_eval"]
+    for k, ref in (("mixed_fp32_split", "fp64"), ("mixed_bf16_split", "fp64"),
+                   ("p4_mixed_bf16_split", "p4_fp64"), ("p12_mixed_bf16_split", "p12_fp64")):
+        out[k]["speedup_vs_fp64"] = out[ref]["s_per_eval"] / out[k]["s_per_eval"]
     return out
 
 
