@@ -948,7 +948,8 @@ __global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict
 // their partial sums are reduced in a fixed order (deterministic). The kernel
 // is (|d|^2 + eps^2)^{-3/2} from v_rsq_f64 refined by one Newton step; the
 // self pair is skipped by index, as in the reference (branch-free: its
-// distance is replaced by 1 and its mass by 0).
+// distance is replaced by 1 and its mass by 0), where its term is not already
+// an exact zero (the potential, and eps = 0).
 //
 // v_rsq_f64 is accurate to 2^-24.2 relative on gfx950 (measured over 2^20
 // arguments spanning 2^-20 .. 2^20); one Newton step brings that to 4.1e-15
@@ -968,7 +969,7 @@ struct P2PAcc {
 // MIX: the pair difference stays FP64 (it carries the cancellation), then
 // r^2, v_rsq_f32 and the pair term run in FP32; each lane sums its terms of
 // one 64-body source tile in FP32 and adds that partial to its FP64 accumulator.
-template <bool MIX, bool POT, int R>
+template <bool MIX, bool POT, int R, bool MASK>
 __device__ __forceinline__ void p2p_tile(const double* sx, const double* sy, const double* sz, const double* sm,
                                          const int* sj, int kfirst, int cnt, int Gs, const double (&xi)[R][3],
                                          const int (&ii)[R], double eps2, P2PAcc<POT> (&a)[R]) {
@@ -981,11 +982,11 @@ __device__ __forceinline__ void p2p_tile(const double* sx, const double* sy, con
         for (int k = kfirst; k < cnt; k += Gs) {
             const double xk = sx[k], yk = sy[k], zk = sz[k];
             const float mk = (float)sm[k];
-            const int j = sj[k];
+            const int j = MASK ? sj[k] : 0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const float dx = (float)(xk - xi[r][0]), dy = (float)(yk - xi[r][1]), dz = (float)(zk - xi[r][2]);
-                const bool self = j == ii[r];
+                const bool self = MASK && j == ii[r];
                 const float r2 = self ? 1.f : dx * dx + dy * dy + dz * dz + e2;
                 const float m = self ? 0.f : mk;
                 const float rin = __builtin_amdgcn_rsqf(r2);
@@ -1007,11 +1008,11 @@ __device__ __forceinline__ void p2p_tile(const double* sx, const double* sy, con
 #pragma unroll 1
         for (int k = kfirst; k < cnt; k += Gs) {
             const double xk = sx[k], yk = sy[k], zk = sz[k], mk = sm[k];
-            const int j = sj[k];
+            const int j = MASK ? sj[k] : 0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const double dx = xk - xi[r][0], dy = yk - xi[r][1], dz = zk - xi[r][2];
-                const bool self = j == ii[r];
+                const bool self = MASK && j == ii[r];
                 const double r2 = self ? 1.0 : dx * dx + dy * dy + dz * dz + eps2;
                 const double m = self ? 0.0 : mk;
                 const double rin = rsqrt_f64(r2);
@@ -1037,6 +1038,12 @@ __device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const
     const int lane = threadIdx.x;
     const int q = t.bsys[b];
     const double G = t.sys_G[q], eps2 = t.sys_eps2[q];
+    // With eps^2 >= 1e-20 the self pair (and any coincident pair) has d = +0
+    // and a finite kernel (also in FP32), so its acceleration term is exactly
+    // +0 and adding it leaves every sum's bits unchanged: the index test and
+    // masking are then skipped. They stay for the potential (the self term
+    // m/eps is not zero) and for eps = 0.
+    const bool mask = POT || !(eps2 >= 1e-20);
     const int ua = t.u_off[li], nu = t.u_off[li + 1] - ua;
     const int T = min(64, max(8, (nt + R - 1) / R));
     const int Gs = 64 / T;
@@ -1071,7 +1078,10 @@ __device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const
                     sj[lane] = j;
                 }
                 __syncthreads();
-                p2p_tile<MIX, POT, R>(sx, sy, sz, sm, sj, kfirst, min(64, total - q0), Gs, xi, ii, eps2, a);
+                if (mask)
+                    p2p_tile<MIX, POT, R, true>(sx, sy, sz, sm, sj, kfirst, min(64, total - q0), Gs, xi, ii, eps2, a);
+                else
+                    p2p_tile<MIX, POT, R, false>(sx, sy, sz, sm, sj, kfirst, min(64, total - q0), Gs, xi, ii, eps2, a);
                 __syncthreads();
             }
         }
