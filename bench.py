@@ -357,11 +357,13 @@ def main():
     d_pos = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
     d_mass = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
     d_acc = torch.empty_like(d_pos)
-    def timed(precision, p=None, steps=None):
-        """W warmup + K timed evaluations; returns (s/eval, averaged phase ms, last stats, solver)."""
+    def timed(precision, p=None, steps=None, overlap=True):
+        """W warmup + K timed evaluations; returns (s/eval, averaged phase ms, last stats, solver).
+        overlap=False runs every pass in order on one stream."""
         p = args.p if p is None else p
         nsteps = args.steps if steps is None else steps
         solver = afmm.AdaptiveFMM3D(G=1.0, eps=args.eps, p=p, threshold=args.cap, precision=precision)
+        solver.set_overlap(overlap)
 
         def step():
             solver.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
@@ -392,6 +394,12 @@ def main():
                 "traffic": measured_traffic("k_m2l_mfma_fp64"),
                 "traffic_source": "profiles/m2l_traffic_merger1M_p8.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
                 "algorithmic_per_launch": f"2*S^2*|V| = 2*{S}^2*{st['n_v']} = {m2l_flops:.4e} flop"}
+
+    # the same evaluation with every pass in order on one stream: P2P timed
+    # alone (on two streams it shares the GPU with M2L), and the overlap's gain
+    progress("FP64 timing, one stream")
+    seq_s, seq_avg, _, seq_solver = timed(afmm.PRECISION_FP64, steps=max(3, args.steps // 2), overlap=False)
+    seq_solver.close()
 
     # PCIe-inclusive drop-in semantics (host arrays in and out), reported aside
     t1 = time.perf_counter()
@@ -480,11 +488,14 @@ def main():
            "roofline": roofline,
            # the near field on the VALU, at the conventional 20 flop per softened
            # pair interaction, against the FP64 peak
-           "p2p_valu": {"kernel": "k_p2p (FP64 VALU)", "pairs": st["p2p_pairs"], "ms": avg["p2p"],
-                        "achieved_tflops": 20.0 * st["p2p_pairs"] / (avg["p2p"] * 1e-3) / 1e12,
+           "p2p_valu": {"kernel": "k_p2p (FP64 VALU), timed alone in the one-stream run",
+                        "pairs": st["p2p_pairs"], "ms": seq_avg["p2p"],
+                        "achieved_tflops": 20.0 * st["p2p_pairs"] / (seq_avg["p2p"] * 1e-3) / 1e12,
                         "peak_tflops": PEAK_FP64_TFLOPS,
-                        "frac": 20.0 * st["p2p_pairs"] / (avg["p2p"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
+                        "frac": 20.0 * st["p2p_pairs"] / (seq_avg["p2p"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
            "phases_ms": avg,
+           "one_stream": {"s_per_eval": seq_s, "phases_ms": seq_avg,
+                          "note": "every pass in order on one stream (afmm_set_overlap 0)"},
            "mixed": mixed,
            "accuracy": accuracy,
            "orders": orders,

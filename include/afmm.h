@@ -266,6 +266,15 @@ int afmm_synchronize(afmm_ctx* ctx);
  * same as without it. */
 int afmm_set_graph(afmm_ctx* ctx, int enable);
 
+/* Two-stream overlap (no counterpart in the reference, which has one host
+ * thread and one stream): when enabled (the default), an evaluation outside
+ * graph mode runs the upward pass beside the interaction-list build and the
+ * P2P near field beside the far-field passes on a second stream shared by the
+ * process's contexts on that device. Disabled, everything runs in order on the
+ * context's stream (afmm_stats.ms_p2p then times P2P alone). Results are the
+ * same bits either way. */
+int afmm_set_overlap(afmm_ctx* ctx, int enable);
+
 /* ---- introspection ------------------------------------------------------ */
 int afmm_get_stats(const afmm_ctx* ctx, afmm_stats* out);
 int afmm_tree_export(afmm_ctx* ctx, afmm_tree* out);

@@ -127,6 +127,7 @@ def library():
         L.afmm_direct.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64, C.POINTER(C.c_int64), _dp, _dp]
         L.afmm_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.afmm_set_graph.argtypes = [C.c_void_p, C.c_int]
+        L.afmm_set_overlap.argtypes = [C.c_void_p, C.c_int]
         L.afmm_tree_export.argtypes = [C.c_void_p, C.POINTER(Tree)]
         _lib = L
     return _lib
@@ -248,6 +249,12 @@ class AdaptiveFMM3D:
         """Graph mode: the FMM passes of each evaluation are captured into a
         HIP graph and launched from it (results unchanged)."""
         self._check(self.lib.afmm_set_graph(self.ctx, 1 if enable else 0))
+
+    def set_overlap(self, enable=True):
+        """Two-stream overlap (default on): the upward pass beside the list
+        build and P2P beside the far-field passes. Off: one stream, in order.
+        Results are the same bits either way."""
+        self._check(self.lib.afmm_set_overlap(self.ctx, 1 if enable else 0))
 
     def stats(self):
         st = Stats()

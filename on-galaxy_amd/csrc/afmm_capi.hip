@@ -286,6 +286,12 @@ int afmm_set_graph(afmm_ctx* c, int enable) {
     return AFMM_OK;
 }
 
+int afmm_set_overlap(afmm_ctx* c, int enable) {
+    if (!c) return AFMM_ERR_ARG;
+    c->split_p2p = enable != 0;
+    return AFMM_OK;
+}
+
 void afmm_destroy(afmm_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);

@@ -1248,16 +1248,20 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     double* nr = split ? pool.get<double>("near", 3 * (size_t)t.n) : nullptr;
     double* npot = split && phi ? pool.get<double>("nearpot", t.n) : nullptr;
     if (reserve_only) return;
-    if (split) {
-        // P2P needs only the tree and the U lists: it runs beside everything
-        // up to M2P on the second stream.
+    // P2P needs only the tree and the U lists: on the second stream it runs
+    // beside the far-field passes. It is forked before M2L in FP64 (it shares
+    // the GPU with M2L's start) and after the bf16 M2L, whose short MFMA phase
+    // it slowed more than it gained (measured both ways in both forms).
+    auto fork_p2p = [&]() {
         AFMM_CHECK(hipEventRecord(side.fork, s));
         AFMM_CHECK(hipStreamWaitEvent(side.s, side.fork, 0));
         if (ev) AFMM_CHECK(hipEventRecord(ev[6], side.s));
         launch_p2p<true>(side.s, t, mixed, far, acc, fpot, phi, nr, npot);
         if (ev) AFMM_CHECK(hipEventRecord(ev[7], side.s));
         AFMM_CHECK(hipEventRecord(side.join, side.s));
-    }
+    };
+    const bool p2p_after_m2l = prec == kPrecMixedBf16;
+    if (split && !p2p_after_m2l) fork_p2p();
     if (!side.upward_launched) upward<P>(s, t, meta, tab, M, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     {
         if (!side.prep_done) m2l_prep(s, t, np, ntiles, tstart, rank, srcT, present);
@@ -1277,6 +1281,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
             hipLaunchKernelGGL((k_m2l_mfma<P, false>), dim3(xcd_grid(ntiles)), dim3(m2l_waves<P>() * 64), 0, s, np, ntiles,
                                tstart, t.m2l_order, srcT, present, tab.m2l_sw, M, L);
     }
+    if (split && p2p_after_m2l) fork_p2p();
     if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
     if (t.n_x > 0)
         hipLaunchKernelGGL((k_accumulate<P, 1>), dim3(xcd_chunk_grid(nb)), dim3(64), 0, s, t, L);

@@ -139,3 +139,43 @@ def test_graph_mode_replays_equal_stream_launches(afmm, prec):
     a_e, f_e = eager.computeAccelerationsAndPotential(p1, m1)
     assert np.array_equal(a_g, a_e) and np.array_equal(f_g, f_e)
     assert graph.stats()["graph_instantiations"] + graph.stats()["graph_updates"] == 5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "mixed_bf16"])
+def test_two_stream_overlap_equals_one_stream(afmm, prec):
+    """The two-stream schedule (upward pass beside the list build, P2P near
+    field beside the far-field passes, then a separate far + near combine)
+    returns the same bits as every pass in order on one stream, for a batch
+    with eps = 0 and eps > 0 systems (masked and unmasked self pairs) and a
+    single system with and without potentials."""
+    precision = {"fp64": afmm.PRECISION_FP64, "mixed_bf16": afmm.PRECISION_MIXED_BF16}[prec]
+    sys_ = systems(21, 9)
+    rng = np.random.default_rng(8)
+    G = rng.uniform(0.5, 2.0, len(sys_))
+    eps = rng.choice([0.0, 0.01], len(sys_))
+    off = np.concatenate([[0], np.cumsum([len(m) for _, m in sys_])])
+    pos = np.concatenate([p for p, _ in sys_])
+    mass = np.concatenate([m for _, m in sys_])
+    print(f"overlap {prec}: {len(sys_)} systems, {off[-1]} bodies", flush=True)
+    two = afmm.AdaptiveFMM3D(p=6, threshold=32, precision=precision)
+    one = afmm.AdaptiveFMM3D(p=6, threshold=32, precision=precision)
+    one.set_overlap(False)
+    for pot in (False, True):
+        r2 = two.computeBatch(pos, mass, off, G, eps, potential=pot)
+        r1 = one.computeBatch(pos, mass, off, G, eps, potential=pot)
+        if pot:
+            assert np.array_equal(r2[0], r1[0]) and np.array_equal(r2[1], r1[1])
+        else:
+            assert np.array_equal(r2, r1)
+    p1, m1 = sys_[0]
+    for s_ in (two, one):
+        s_.close()
+    two = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=64, precision=precision)
+    one = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=64, precision=precision)
+    one.set_overlap(False)
+    assert np.array_equal(two.computeAccelerations(p1, m1), one.computeAccelerations(p1, m1))
+    a2, f2 = two.computeAccelerationsAndPotential(p1, m1)
+    a1, f1 = one.computeAccelerationsAndPotential(p1, m1)
+    assert np.array_equal(a2, a1) and np.array_equal(f2, f1)
+    assert two.stats()["ms_p2p"] > 0 and one.stats()["ms_p2p"] > 0
