@@ -1265,8 +1265,19 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     hipLaunchKernelGGL(k_leaf_flags, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.seg_count, flags,
                        d_total + 1);
     scan_exclusive(flags, fscan, nb, d_total, scan_tmp, s);
+    // leaves before each level boundary (for the non-leaf ranges per level) and
+    // boxes per octant (for the M2L tiles) come back with the leaf count in
+    // one round trip
+    const int nlev = (int)meta.level_off.size();
+    int* d_lvl_idx = pool.get<int>("lvl_idx", nlev);
+    int* d_lvl_out = pool.get<int>("lvl_out", nlev + 8);
+    AFMM_CHECK(hipMemcpyAsync(d_lvl_idx, meta.level_off.data(), sizeof(int) * nlev, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_lvl_idx, nlev, nb, d_lvl_out);
+    hipLaunchKernelGGL(k_octant_count, dim3(1), dim3(kOctCountThreads), 0, s, nb, tv.octant, d_lvl_out + nlev);
+    std::vector<int> got(nlev + 8);
     int lt[2];
     AFMM_CHECK(hipMemcpyAsync(lt, d_total, sizeof(lt), hipMemcpyDeviceToHost, s));
+    AFMM_CHECK(hipMemcpyAsync(got.data(), d_lvl_out, sizeof(int) * (nlev + 8), hipMemcpyDeviceToHost, s));
     AFMM_CHECK(hipStreamSynchronize(s));
     tv.nleaves = lt[0];
     const int max_leaf = lt[1];
@@ -1327,16 +1338,8 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         AFMM_CHECK(hipMemcpyAsync(tv.m2l_order, lb.val[r2], sizeof(int) * nb, hipMemcpyDeviceToDevice, s));
         tv.nonleaf_ids = pool.get<int>("nonleaf_ids", nb);
         hipLaunchKernelGGL(k_nonleaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.nonleaf_ids);
-        const int nlev = (int)meta.level_off.size();
-        std::vector<int> bounds(meta.level_off), got(nlev);
-        int* d_idx = pool.get<int>("lvl_idx", nlev);
-        int* d_out = pool.get<int>("lvl_out", nlev + 8);
-        AFMM_CHECK(hipMemcpyAsync(d_idx, bounds.data(), sizeof(int) * nlev, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_idx, nlev, nb, d_out);
-        hipLaunchKernelGGL(k_octant_count, dim3(1), dim3(kOctCountThreads), 0, s, nb, tv.octant, d_out + nlev);
-        got.resize(nlev + 8);
-        AFMM_CHECK(hipMemcpyAsync(got.data(), d_out, sizeof(int) * (nlev + 8), hipMemcpyDeviceToHost, s));
-        AFMM_CHECK(hipStreamSynchronize(s));
+        const std::vector<int>& bounds = meta.level_off;
+        int* d_out = d_lvl_out;
         meta.nonleaf_off.resize(nlev);
         for (int l = 0; l < nlev; ++l)
             meta.nonleaf_off[l] = bounds[l] - (bounds[l] < nb ? got[l] : tv.nleaves);
