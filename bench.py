@@ -345,8 +345,9 @@ def main():
         print(json.dumps({"copy_bytes_per_launch": a.numel() * 8, "launches": 5}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards as described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: "
+              "collectives and sharding are not built (independent systems would be bin-packed "
+              "across GPUs, one large system partitioned by Morton key range)", file=sys.stderr)
         sys.exit(2)
 
     import torch
