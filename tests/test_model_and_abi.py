@@ -59,3 +59,13 @@ def test_parameter_validation_without_device():
         with pytest.raises(mod.AfmmError) as e:
             mod.AdaptiveFMM3D(**kw)
         assert e.value.code == code
+
+
+def test_context_switches_reject_null_without_device():
+    """The mode switches check their context before touching the device."""
+    lib = C.CDLL(os.path.join(cabi.ROOT, "on-galaxy_amd", "libafmm.so"))
+    for name in ("afmm_set_overlap", "afmm_set_graph"):
+        f = getattr(lib, name)
+        f.argtypes = [C.c_void_p, C.c_int]
+        f.restype = C.c_int
+        assert f(None, 1) == -1 and f(None, 0) == -1, name
