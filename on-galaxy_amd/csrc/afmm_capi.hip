@@ -82,7 +82,8 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     std::vector<RootInfo> roots;
     // Outside graph mode the upward pass runs on the second stream while the
     // interaction lists are built (both leave most of the GPU idle).
-    const bool split = c->split_p2p && !c->use_graph;
+    // (also in graph mode: the graph then holds the passes from M2L on)
+    const bool split = c->split_p2p;
     if (split && !c->side) {
         AFMM_CHECK(hipEventCreateWithFlags(&c->side_fork, hipEventDisableTiming));
         AFMM_CHECK(hipEventCreateWithFlags(&c->side_join, hipEventDisableTiming));
@@ -114,12 +115,19 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     tv.sys_eps2 = d_ge + nsys;
     AFMM_CHECK(hipEventRecord(c->ev[1], s));
     if (c->use_graph) {
+        // the upward pass and the M2L preparation already ran outside the graph
+        // (beside and within the list build); the stream waits for them before
+        // the capture, so the graph holds no cross-stream dependency
+        if (up_launched) AFMM_CHECK(hipStreamWaitEvent(s, c->up_done, 0));
+        SideStream gside{};
+        gside.upward_launched = up_launched;
+        gside.prep_done = prep_done;
         // size every buffer first, so that the capture allocates nothing
-        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr, true);
+        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr, true, gside);
         hipGraph_t g = nullptr;
         AFMM_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr);
+            run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr, false, gside);
         } catch (...) {
             if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
             throw;

@@ -1265,7 +1265,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (!side.upward_launched) upward<P>(s, t, meta, tab, M, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     {
         if (!side.prep_done) m2l_prep(s, t, np, ntiles, tstart, rank, srcT, present);
-        if (side.upward_launched) AFMM_CHECK(hipStreamWaitEvent(s, side.up_done, 0));  // M is complete
+        if (side.upward_launched && side.up_done) AFMM_CHECK(hipStreamWaitEvent(s, side.up_done, 0));  // M done
         if (prec == kPrecMixedBf16) {
             constexpr int KB = m2l_bf_kb(S);
             hipLaunchKernelGGL(k_split_bf16<P>, dim3(blocks((long long)nb * KB * 4)), dim3(kT), 0, s, nb, M, Mbf);
