@@ -259,9 +259,9 @@ int afmm_eval_device_potential(afmm_ctx* ctx, int64_t n, const double* d_pos_xyz
 int afmm_synchronize(afmm_ctx* ctx);
 
 /* Graph mode (no counterpart in the reference, which launches from the host
- * per pass): when enabled, each evaluation captures its FMM passes (P2M
- * through P2P, after the tree build, which needs host round trips) into a HIP
- * graph and launches that; an evaluation whose passes have the same shape as
+ * per pass): when enabled, each evaluation captures its FMM passes (M2L
+ * through P2P, or P2M through P2P with afmm_set_overlap off; after the tree
+ * build, which needs host round trips) into a HIP graph and launches that; an evaluation whose passes have the same shape as
  * the previous one updates the executable graph in place. Results are the
  * same as without it. */
 int afmm_set_graph(afmm_ctx* ctx, int enable);
