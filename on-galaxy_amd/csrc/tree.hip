@@ -550,9 +550,12 @@ __global__ void __launch_bounds__(kT) k_nonleaf_ids(int nb, const int* nchild, c
     if (b < nb && nchild[b] > 0) ids[b - leaf_scan[b]] = b;
 }
 // leaf_scan at each level boundary (levels given by host offsets)
-__global__ void k_gather_at(const int* a, const int* idx, int n, int total, int* out) {
+// out[i] = a[idx[i]] (-1 past total) for i < n, and out[n + 8 + k] = extra[k]
+// for k < nextra (the 8 entries between are the octant counts).
+__global__ void k_gather_at(const int* a, const int* idx, int n, int total, int* out, const int* extra, int nextra) {
     const int i = threadIdx.x;
     if (i < n) out[i] = idx[i] < total ? a[idx[i]] : -1;
+    if (i < nextra) out[n + 8 + i] = extra[i];
 }
 
 // ---- leaves and the in-leaf input order ------------------------------------------
@@ -1163,6 +1166,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     int* d_total = pool.get<int>("total", 4);
     int l = 0;
     bool done = false;
+    const int* d_level_dev = nullptr;  // the level offsets on the device (device-driven loop)
     {
         if (device_levels) {
             const int maxcnt = std::min<size_t>(bound, (size_t)n + nsys);
@@ -1189,6 +1193,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
             AFMM_CHECK(hipMemcpyAsync(hl.data(), lev, sizeof(int) * (L + 3), hipMemcpyDeviceToHost, s));
             AFMM_CHECK(hipStreamSynchronize(s));
             if (hl[L + 2] == 0) {
+                d_level_dev = lev;  // (not on the fallback below, which rebuilds the levels)
                 // level L boxes (if any) are leaves
                 hipLaunchKernelGGL(k_fill_leaf_range, dim3(blocks(hl[L + 1] - hl[L])), dim3(kT), 0, s, hl[L],
                                    hl[L + 1] - hl[L], tv.nchild, tv.child_first, tv.child_mask);
@@ -1268,19 +1273,23 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     // leaves before each level boundary (for the non-leaf ranges per level) and
     // boxes per octant (for the M2L tiles) come back with the leaf count in
     // one round trip
+    // (level offsets: the device copy of the device-driven loop, whose leading
+    // entries are meta.level_off, else uploaded)
     const int nlev = (int)meta.level_off.size();
-    int* d_lvl_idx = pool.get<int>("lvl_idx", nlev);
-    int* d_lvl_out = pool.get<int>("lvl_out", nlev + 8);
-    AFMM_CHECK(hipMemcpyAsync(d_lvl_idx, meta.level_off.data(), sizeof(int) * nlev, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_lvl_idx, nlev, nb, d_lvl_out);
+    const int* d_lvl_idx = d_level_dev;
+    if (!d_lvl_idx) {
+        int* up = pool.get<int>("lvl_idx", nlev);
+        AFMM_CHECK(hipMemcpyAsync(up, meta.level_off.data(), sizeof(int) * nlev, hipMemcpyHostToDevice, s));
+        d_lvl_idx = up;
+    }
+    int* d_lvl_out = pool.get<int>("lvl_out", nlev + 8 + 2);
+    hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_lvl_idx, nlev, nb, d_lvl_out, d_total, 2);
     hipLaunchKernelGGL(k_octant_count, dim3(1), dim3(kOctCountThreads), 0, s, nb, tv.octant, d_lvl_out + nlev);
-    std::vector<int> got(nlev + 8);
-    int lt[2];
-    AFMM_CHECK(hipMemcpyAsync(lt, d_total, sizeof(lt), hipMemcpyDeviceToHost, s));
-    AFMM_CHECK(hipMemcpyAsync(got.data(), d_lvl_out, sizeof(int) * (nlev + 8), hipMemcpyDeviceToHost, s));
+    std::vector<int> got(nlev + 8 + 2);
+    AFMM_CHECK(hipMemcpyAsync(got.data(), d_lvl_out, sizeof(int) * (nlev + 8 + 2), hipMemcpyDeviceToHost, s));
     AFMM_CHECK(hipStreamSynchronize(s));
-    tv.nleaves = lt[0];
-    const int max_leaf = lt[1];
+    tv.nleaves = got[nlev + 8];
+    const int max_leaf = got[nlev + 9];
     tv.leaf_ids = pool.get<int>("leaf_ids", tv.nleaves);
     hipLaunchKernelGGL(k_leaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.leaf_ids);
     tv.leaf_of = pool.get<int>("leaf_of", n);
