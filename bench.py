@@ -91,17 +91,22 @@ def cpu_baseline(pos, mass, args):
 
 
 def drift_leg(afmm, ics, args):
-    """C2 energy drift: the merger (with velocities from the solver's forces)
+    """C1/C2 energy drift: the merger (with velocities from the solver's forces)
     integrated for --drift-steps leapfrog steps of dt = 0.01 on the resident
-    state, in FP64 and both mixed forms; |E - E0|/|E0| every 10 steps with the
-    FMM potential (afmm_state_diagnostics)."""
+    state, in FP64 and both mixed forms at p = 8, and in FP64 and bf16 mixed at
+    p = 4 and 12; |E - E0|/|E0| every 10 steps with the FMM potential
+    (afmm_state_diagnostics)."""
     ic = afmm.AdaptiveFMM3D(eps=args.eps, p=4, threshold=args.cap)
     pos, vel, mass, _ = ics.merger(args.drift_n // 2, seed=args.seed, solver=ic)
     ic.close()
     out = {"n": int(len(mass)), "steps": args.drift_steps, "dt": 0.01, "p": args.p, "eps": args.eps}
-    for name, prec in (("fp64", afmm.PRECISION_FP64), ("mixed_fp32_split", afmm.PRECISION_MIXED),
-                       ("mixed_bf16_split", afmm.PRECISION_MIXED_BF16)):
-        s = afmm.AdaptiveFMM3D(eps=args.eps, p=args.p, threshold=args.cap, precision=prec)
+    forms = [(name, prec, args.p) for name, prec in (("fp64", afmm.PRECISION_FP64),
+                                                       ("mixed_fp32_split", afmm.PRECISION_MIXED),
+                                                       ("mixed_bf16_split", afmm.PRECISION_MIXED_BF16))]
+    for p in (4, 12):
+        forms += [(f"p{p}_fp64", afmm.PRECISION_FP64, p), (f"p{p}_mixed_bf16_split", afmm.PRECISION_MIXED_BF16, p)]
+    for name, prec, p in forms:
+        s = afmm.AdaptiveFMM3D(eps=args.eps, p=p, threshold=args.cap, precision=prec)
         st = afmm.LeapfrogState(s, pos, vel, mass)
         e0 = st.diagnostics()[0]["total"]
         drift = []
@@ -111,7 +116,7 @@ def drift_leg(afmm, ics, args):
             st.step(0.01, 10)
             t_step += time.perf_counter() - t0
             drift.append(abs(st.diagnostics()[0]["total"] - e0) / abs(e0))
-        out[name] = {"E0": e0, "max_rel_drift": float(max(drift)), "final_rel_drift": float(drift[-1]),
+        out[name] = {"p": p, "E0": e0, "max_rel_drift": float(max(drift)), "final_rel_drift": float(drift[-1]),
                      "s_per_step": t_step / (10 * len(drift))}
         st.close()
         s.close()
