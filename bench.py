@@ -135,9 +135,12 @@ def rel_errors(ref, a):
 
 def large_leg(afmm, ics, torch, args):
     """C2 at N = 4M (2 x 2M merger): s/eval of FP64 and both mixed forms at
-    p = 8, FP64 and bf16 mixed at p = 4 and 12, and the eps = 0 truncation error
-    of each against FP64 direct on 4096 targets."""
-    pos, _, mass, _ = ics.merger(args.large_n // 2, seed=args.seed + 9)
+    p = 8, FP64 and bf16 mixed at p = 4 and 12, the eps = 0 truncation error
+    of each against FP64 direct on 4096 targets, and the 100-step energy drift
+    of the FP64 and bf16 forms at p = 8."""
+    ic = afmm.AdaptiveFMM3D(eps=args.eps, p=4, threshold=args.cap)  # disk velocities from its forces
+    pos, vel, mass, _ = ics.merger(args.large_n // 2, seed=args.seed + 9, solver=ic)
+    ic.close()
     n = len(mass)
     dev = torch.device("cuda:0")
     d_pos = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
@@ -173,6 +176,22 @@ def large_leg(afmm, ics, torch, args):
     for k, ref in (("mixed_fp32_split", "fp64"), ("mixed_bf16_split", "fp64"),
                    ("p4_mixed_bf16_split", "p4_fp64"), ("p12_mixed_bf16_split", "p12_fp64")):
         out[k]["speedup_vs_fp64"] = out[ref]["s_per_eval"] / out[k]["s_per_eval"]
+    # energy drift over --drift-steps leapfrog steps (dt = 0.01) at 4M, p = 8,
+    # from the generator's velocities, with the FMM potential every 10 steps
+    for name, prec in (("fp64", afmm.PRECISION_FP64), ("mixed_bf16_split", afmm.PRECISION_MIXED_BF16)):
+        s = afmm.AdaptiveFMM3D(eps=args.eps, p=args.p, threshold=args.cap, precision=prec)
+        st = afmm.LeapfrogState(s, pos, vel, mass)
+        e0 = st.diagnostics()[0]["total"]
+        drift, t_step = [], 0.0
+        for _ in range(max(args.drift_steps // 10, 1)):
+            t0 = time.perf_counter()
+            st.step(0.01, 10)
+            t_step += time.perf_counter() - t0
+            drift.append(abs(st.diagnostics()[0]["total"] - e0) / abs(e0))
+        out[name]["drift_100_steps"] = {"steps": 10 * len(drift), "max_rel_drift": float(max(drift)),
+                                        "final_rel_drift": float(drift[-1]), "s_per_step": t_step / (10 * len(drift))}
+        st.close()
+        s.close()
     return out
 
 
