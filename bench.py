@@ -203,15 +203,8 @@ def batched_leg(afmm, ics, tune, torch, args):
     evaluation (one launch per FMM phase for all systems) against 256
     sequential plain (FP64, leaf capacity 64) solves of the same systems. The
     same step with stream launches is reported beside it."""
-    rng = np.random.default_rng(args.seed + 3)
-    ns = np.exp(rng.uniform(np.log(2000), np.log(200000), args.batch_systems)).astype(np.int64)
-    parts = []
-    for q, nq in enumerate(ns):
-        if q % 2 == 0:
-            pq, _, mq = ics.plummer(int(nq), seed=1000 + q)
-        else:
-            pq, _, mq = ics.hernquist(int(nq), a=0.5, seed=1000 + q, velocities=False)
-        parts.append((pq, mq))
+    parts = ics.batch_systems(args.batch_systems, seed=args.seed + 3)
+    ns = np.array([len(mq) for _, mq in parts], np.int64)
     off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
     dev = torch.device("cuda:0")
     d_pos = torch.from_numpy(np.ascontiguousarray(np.concatenate([p for p, _ in parts]))).to(dev)
@@ -266,21 +259,8 @@ def serving_leg(afmm, ics, args):
     through the continuous-batching server, and the same jobs one at a time
     (each on its own resident leapfrog, in arrival order)."""
     import threading
-    rng = np.random.default_rng(args.seed + 4)
-    arrivals = np.cumsum(rng.exponential(1.0 / 20.0, args.serve_jobs))
-    jobs = []
     ic_solver = afmm.AdaptiveFMM3D(eps=0.005, p=4, threshold=64)
-    for k in range(args.serve_jobs):
-        n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
-        p = int(rng.choice([4, 8, 12]))
-        steps = int(rng.integers(1, 51))
-        if rng.random() < 0.5:
-            pos, vel, m = ics.plummer(n, seed=2000 + k)
-            eps = 0.01
-        else:
-            pos, vel, m, _ = ics.merger(max(n // 2, 1), seed=2000 + k, solver=ic_solver)
-            eps = 0.005
-        jobs.append(dict(pos=pos, vel=vel, mass=m, eps=eps, p=p, steps=steps, t=float(arrivals[k])))
+    jobs = ics.serving_jobs(args.serve_jobs, seed=args.seed + 4, rate=20.0, solver=ic_solver)
     ic_solver.close()
     work = float(sum(len(j["mass"]) * j["steps"] for j in jobs))
 

@@ -33,7 +33,7 @@ typedef enum {
     AFMM_ERR_PRECISION = -9   /* precision mode not supported */
 } afmm_status;
 
-#define AFMM_MAX_ORDER 16
+#define AFMM_MAX_ORDER 12
 /* FP64: every phase in FP64 (the reference CPU path's arithmetic).
  * MIXED: M2L as split-FP32 MFMA products (each operand carried as an FP32
  * hi/lo pair, three products per block) and P2P in FP32 with FP64
@@ -196,7 +196,7 @@ typedef struct {
     const double* vel_xyz;  /* may be NULL when steps == 0 */
     const double* mass;
     double G, eps, dt;
-    int32_t p;              /* expansion order 1..12 */
+    int32_t p;              /* expansion order 1..AFMM_MAX_ORDER */
     int32_t steps;          /* 0: force evaluation; k > 0: k leapfrog steps */
 } afmm_job_desc;
 #define AFMM_JOB_QUEUED 0
@@ -213,7 +213,8 @@ typedef struct {
 } afmm_job_info;
 
 /* Footprint estimate used for admission: resident arrays, the job's share of
- * the batch, tree and sort buffers, its expansions and lists. */
+ * the batch, tree and sort buffers, its expansions and lists, at leaf
+ * capacity 64 (a server scales the box count by its own leaf_cap). */
 int64_t afmm_job_bytes(int64_t n, int32_t p);
 int afmm_server_create(const afmm_server_params* prm, afmm_server** out);
 /* Prepares the worker of order p (context, translation tables) ahead of jobs. */
@@ -229,7 +230,10 @@ int afmm_server_info(afmm_server* sv, int64_t id, afmm_job_info* info);
 int afmm_server_result(afmm_server* sv, int64_t id, double* pos_xyz, double* vel_xyz, double* acc_xyz);
 /* Sum of the footprint estimates of the admitted, unfinished jobs. */
 int64_t afmm_server_hbm_in_use(afmm_server* sv);
-/* Finishes running jobs, fails queued ones, joins the workers. */
+/* Finishes running jobs, fails queued ones (AFMM_ERR_STATE; their waiters
+ * return), joins the workers. A thread blocked in afmm_server_wait or
+ * afmm_server_drain must return before destroy frees the server: callers do
+ * not let those calls race destroy. */
 void afmm_server_destroy(afmm_server* sv);
 
 /* FP64 direct summation over all n bodies onto the targets tgt[0..ntgt)
@@ -274,6 +278,16 @@ int afmm_set_graph(afmm_ctx* ctx, int enable);
  * context's stream (afmm_stats.ms_p2p then times P2P alone). Results are the
  * same bits either way. */
 int afmm_set_overlap(afmm_ctx* ctx, int enable);
+
+/* ---- initial conditions (host) -------------------------------------------
+ * makePlummerSphere (InitialConditions3D.hpp:100-152): n equal-mass bodies of
+ * a Plummer sphere (scale a, total mass total_mass, constant G) drawn from
+ * libstdc++'s std::mt19937(seed) through uniform_real_distribution<double>,
+ * centre of mass and net momentum removed; the reference's bodies bit for bit
+ * for the same seed. Host arrays (pos_xyz, vel_xyz n x 3; mass n); no device
+ * work. AFMM_ERR_ARG for n < 1, a <= 0 or non-finite parameters. */
+int afmm_make_plummer(int64_t n, double a, double total_mass, double G, uint32_t seed, double* pos_xyz,
+                      double* vel_xyz, double* mass);
 
 /* ---- introspection ------------------------------------------------------ */
 int afmm_get_stats(const afmm_ctx* ctx, afmm_stats* out);

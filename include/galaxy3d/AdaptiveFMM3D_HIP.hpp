@@ -54,8 +54,11 @@ public:
     }
 
     const char* name() const override {
-        return precision_ == AFMM_PRECISION_MIXED ? "AdaptiveFMM3D(CGR, MI355X HIP mixed)"
-                                                  : "AdaptiveFMM3D(CGR, MI355X HIP FP64)";
+        switch (precision_) {
+            case AFMM_PRECISION_MIXED: return "AdaptiveFMM3D(CGR, MI355X HIP mixed split-FP32)";
+            case AFMM_PRECISION_MIXED_BF16: return "AdaptiveFMM3D(CGR, MI355X HIP mixed bf16-split)";
+            default: return "AdaptiveFMM3D(CGR, MI355X HIP FP64)";
+        }
     }
 
     afmm_ctx* context() const { return ctx_.get(); }

@@ -129,6 +129,7 @@ def library():
         L.afmm_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.afmm_set_overlap.argtypes = [C.c_void_p, C.c_int]
         L.afmm_tree_export.argtypes = [C.c_void_p, C.POINTER(Tree)]
+        L.afmm_make_plummer.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint32, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -136,6 +137,35 @@ def library():
 def _p(a):
     t = {np.float64: _dp, np.int32: _ip, np.int8: _bp}[a.dtype.type]
     return a.ctypes.data_as(t)
+
+
+def _offsets(sys_off, n):
+    """System offsets as int64, checked against the n bodies they index: the C
+    side reads and writes 3 * sys_off[-1] doubles of the position and result
+    arrays, so offsets that do not start at 0, are not strictly increasing or
+    do not end at n are rejected here rather than overrunning those buffers."""
+    off = np.ascontiguousarray(sys_off, np.int64).reshape(-1)
+    if off.size < 2:
+        raise ValueError("sys_off needs at least two entries (one system)")
+    if off[0] != 0:
+        raise ValueError("sys_off[0] must be 0")
+    if np.any(np.diff(off) <= 0):
+        raise ValueError("sys_off must be strictly increasing (every system needs a body)")
+    if off[-1] != n:
+        raise ValueError(f"sys_off[-1] = {int(off[-1])} does not match the {n} bodies")
+    return off
+
+
+def make_plummer(n, a=1.0, total_mass=1.0, G=1.0, seed=7):
+    """makePlummerSphere (InitialConditions3D.hpp:100-152) on libstdc++'s
+    mt19937 stream, generated on the host by the library: (pos (n,3), vel (n,3),
+    mass (n,)) identical to the reference's bodies for the same seed."""
+    n = int(n)
+    pos, vel, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    rc = library().afmm_make_plummer(n, a, total_mass, G, int(seed) & 0xFFFFFFFF, _p(pos), _p(vel), _p(mass))
+    if rc:
+        raise AfmmError(rc, library().afmm_strerror(rc).decode())
+    return pos, vel, mass
 
 
 class AdaptiveFMM3D:
@@ -207,7 +237,9 @@ class AdaptiveFMM3D:
         (None: the solver's). Returns acc (and phi when potential=True)."""
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         mass = np.ascontiguousarray(mass, np.float64)
-        off = np.ascontiguousarray(sys_off, np.int64)
+        if len(mass) != len(pos):
+            raise ValueError("pos and mass lengths differ")
+        off = _offsets(sys_off, len(mass))
         nsys = len(off) - 1
         g = None if G is None else np.ascontiguousarray(np.broadcast_to(G, (nsys,)), np.float64)
         e = None if eps is None else np.ascontiguousarray(np.broadcast_to(eps, (nsys,)), np.float64)
@@ -219,8 +251,10 @@ class AdaptiveFMM3D:
         return (acc, phi) if potential else acc
 
     def compute_batch_device(self, sys_off, pos_ptr, mass_ptr, acc_ptr, G=None, eps=None, phi_ptr=None):
-        """Batched evaluation on device pointers (sys_off, G, eps on the host)."""
-        off = np.ascontiguousarray(sys_off, np.int64)
+        """Batched evaluation on device pointers (sys_off, G, eps on the host);
+        the device arrays must hold sys_off[-1] bodies."""
+        off = np.ascontiguousarray(sys_off, np.int64).reshape(-1)
+        off = _offsets(off, int(off[-1]) if off.size else 0)
         nsys = len(off) - 1
         g = None if G is None else np.ascontiguousarray(np.broadcast_to(G, (nsys,)), np.float64)
         e = None if eps is None else np.ascontiguousarray(np.broadcast_to(eps, (nsys,)), np.float64)
@@ -301,7 +335,9 @@ class LeapfrogState:
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         vel = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
         mass = np.ascontiguousarray(mass, np.float64)
-        off = np.ascontiguousarray([0, len(mass)] if sys_off is None else sys_off, np.int64)
+        if not (len(mass) == len(pos) == len(vel)):
+            raise ValueError("pos, vel and mass lengths differ")
+        off = _offsets([0, len(mass)] if sys_off is None else sys_off, len(mass))
         self.nsys = len(off) - 1
         self.n = len(mass)
         g = None if G is None else np.ascontiguousarray(np.broadcast_to(G, (self.nsys,)), np.float64)

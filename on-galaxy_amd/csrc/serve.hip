@@ -145,6 +145,16 @@ namespace {
 
 void worker_loop(afmm_server* sv, Worker* w);
 
+// Footprint estimate at leaf capacity cap: resident arrays 80 B/body; batch
+// copies, keys, sort and sorted bodies ~420 B/body; per box the two
+// expansions (2 S doubles) and the lists (~1.3 KB). Boxes number about
+// 4 n / cap + 64 (n / 16 at capacity 64; measured n / 22 on the merger).
+int64_t job_bytes_cap(int64_t n, int32_t p, int32_t cap) {
+    const int64_t S = (int64_t)(p + 1) * (p + 1);
+    const int64_t boxes = 4 * n / std::max<int32_t>(cap, 1) + 64;
+    return n * 500 + boxes * (16 * S + 1300);
+}
+
 // The worker of order p, created with its context on first use (caller holds
 // sv->mu); null with *rc set when the context cannot be created.
 Worker* worker_for(afmm_server* sv, int p, int* rc) {
@@ -318,14 +328,7 @@ void worker_loop(afmm_server* sv, Worker* w) {
 
 extern "C" {
 
-int64_t afmm_job_bytes(int64_t n, int32_t p) {
-    // resident arrays 80 B/body; batch copies, keys, sort and sorted bodies
-    // ~420 B/body; per box (about n/16 + nsys at leaf capacity 64) the two
-    // expansions (2 S doubles) and the lists (~1.3 KB).
-    const int64_t S = (int64_t)(p + 1) * (p + 1);
-    const int64_t boxes = n / 16 + 64;
-    return n * 500 + boxes * (16 * S + 1300);
-}
+int64_t afmm_job_bytes(int64_t n, int32_t p) { return job_bytes_cap(n, p, 64); }
 
 int afmm_server_create(const afmm_server_params* prm, afmm_server** out) {
     if (!prm || !out) return AFMM_ERR_ARG;
@@ -357,7 +360,7 @@ int afmm_server_submit(afmm_server* sv, const afmm_job_desc* d, int64_t* id) {
     if (d->vel_xyz) j->vel.assign(d->vel_xyz, d->vel_xyz + 3 * n);
     j->mass.assign(d->mass, d->mass + n);
     j->d.pos_xyz = j->d.vel_xyz = j->d.mass = nullptr;
-    j->bytes = afmm_job_bytes(d->n, d->p);
+    j->bytes = job_bytes_cap(d->n, d->p, sv->prm.leaf_cap);
     std::lock_guard<std::mutex> lk(sv->mu);
     if (sv->stop) return AFMM_ERR_STATE;
     j->id = sv->next_id++;
@@ -441,21 +444,27 @@ int64_t afmm_server_hbm_in_use(afmm_server* sv) {
 void afmm_server_destroy(afmm_server* sv) {
     if (!sv) return;
     {
+        // queued jobs fail; each is counted finished and its waiters are woken
+        // (a wait or drain must still return before the server is deleted:
+        // callers must not let them race destroy)
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->stop = true;
-        for (auto& j : sv->pending) {
+        auto fail_queued = [&](const std::shared_ptr<Job>& j, bool admitted) {
             j->state = AFMM_JOB_FAILED;
             j->error = AFMM_ERR_STATE;
-        }
+            j->msg = "server destroyed before the job started";
+            j->t_done = sv->now();
+            if (admitted) sv->hbm_used -= j->bytes;
+            --sv->unfinished;
+        };
+        for (auto& j : sv->pending) fail_queued(j, false);
         sv->pending.clear();
         for (auto& kv : sv->workers) {
-            for (auto& j : kv.second->inbox) {
-                j->state = AFMM_JOB_FAILED;
-                j->error = AFMM_ERR_STATE;
-            }
+            for (auto& j : kv.second->inbox) fail_queued(j, true);
             kv.second->inbox.clear();
         }
     }
+    sv->done_cv.notify_all();
     sv->work_cv.notify_all();
     for (auto& kv : sv->workers)
         if (kv.second->th.joinable()) kv.second->th.join();

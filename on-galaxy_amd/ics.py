@@ -2,9 +2,9 @@
 
 plummer()   the reference's Plummer recipe (InitialConditions3D.hpp:100-152):
             inverse-CDF radii with u >= 1e-12, Aarseth-Henon-Wielen speed
-            rejection bounded by 0.1, COM position/momentum removed. numpy's
-            generator replaces libstdc++'s mt19937 stream (the oracle keeps
-            the libstdc++ stream for bit-level fixtures).
+            rejection bounded by 0.1, COM position/momentum removed, on
+            libstdc++'s mt19937 stream (afmm_make_plummer, generated on the
+            host by the library): the reference's bodies bit for bit.
 hernquist() isotropic Hernquist sphere truncated at r_t (positions by inverse
             CDF; velocities from the isotropic Jeans equation).
 merger()    the two-galaxy recipe: per galaxy mass 1 in equal-mass bodies,
@@ -19,6 +19,12 @@ merger()    the two-galaxy recipe: per galaxy mass 1 in equal-mass bodies,
             with kappa^2 kept within [Omega^2, 4 Omega^2].
             The reference's nearest recipe (gpu/GalaxyIC.hpp:19-125) uses a
             Plummer bulge and Gaussian thickness instead.
+batch_systems()  the batched workload: independent systems with N log-uniform
+            over [nmin, nmax], even ones Plummer spheres (a = 1), odd ones
+            Hernquist spheres (a = 0.5, truncated at 20a), mass 1 each.
+serving_jobs()   the request stream: Poisson arrivals, each job a Plummer
+            sphere or a two-galaxy merger with N log-uniform over [nmin, nmax],
+            an order from `orders` and 1..max_steps leapfrog steps.
 """
 import numpy as np
 
@@ -30,25 +36,22 @@ def _isotropic(rng, n):
     return np.stack([st * np.cos(ph), st * np.sin(ph), ct], axis=1)
 
 
+def _afmm():
+    import importlib.util
+    import os
+    import sys
+    name = "on_galaxy_amd_afmm"
+    if name not in sys.modules:
+        spec = importlib.util.spec_from_file_location(name, os.path.join(os.path.dirname(__file__), "afmm.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        sys.modules[name] = mod
+    return sys.modules[name]
+
+
 def plummer(n, a=1.0, M=1.0, G=1.0, seed=7):
-    rng = np.random.default_rng(seed)
-    u = np.maximum(rng.random(n), 1e-12)
-    r = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
-    q = np.empty(n)
-    todo = np.arange(n)
-    while todo.size:
-        qq = rng.random(todo.size)
-        g = 0.1 * rng.random(todo.size)
-        ok = g <= qq * qq * (1.0 - qq * qq) ** 3.5
-        q[todo[ok]] = qq[ok]
-        todo = todo[~ok]
-    vesc = np.sqrt(2.0 * G * M) * (r * r + a * a) ** -0.25
-    pos = r[:, None] * _isotropic(rng, n)
-    vel = (q * vesc)[:, None] * _isotropic(rng, n)
-    mass = np.full(n, M / n)
-    pos -= (mass[:, None] * pos).sum(0) / mass.sum()
-    vel -= (mass[:, None] * vel).sum(0) / mass.sum()
-    return pos, vel, mass
+    """makePlummerSphere(PlummerConfig{n, a, M, G, seed}): (pos, vel, mass)."""
+    return _afmm().make_plummer(n, a, M, G, seed)
 
 
 def _hernquist_sigma(r, a, M, rt, G=1.0, extra_mass=None):
@@ -184,3 +187,45 @@ def merger(n_per_galaxy, seed=42, sep=8.0, r_peri=1.0, incl=(30.0, -60.0), G=1.0
     mass = np.concatenate([parts[0][2], parts[1][2]])
     species = np.concatenate([parts[0][3], parts[1][3]])
     return pos, vel, mass, species
+
+
+def batch_systems(count=256, seed=45, nmin=2000, nmax=200000):
+    """[(pos, mass)] of `count` independent systems: N_q = exp(U[ln nmin,
+    ln nmax]) (seeded), even q a Plummer sphere (a = 1, seed 1000 + q, the
+    reference's generator), odd q a Hernquist sphere (a = 0.5, truncated at
+    20 a, seed 1000 + q); total mass 1 each, equal masses."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count)).astype(np.int64)
+    out = []
+    for q, nq in enumerate(ns):
+        if q % 2 == 0:
+            pq, _, mq = plummer(int(nq), seed=1000 + q)
+        else:
+            pq, _, mq = hernquist(int(nq), a=0.5, seed=1000 + q, velocities=False)
+        out.append((np.ascontiguousarray(pq), np.ascontiguousarray(mq)))
+    return out
+
+
+def serving_jobs(count=16, seed=46, rate=20.0, nmin=1e4, nmax=2e6, orders=(4, 8, 12), max_steps=50,
+                 solver=None):
+    """A seeded request stream: arrival times with exponential gaps of mean
+    1/rate; each job a Plummer sphere (eps 0.01) or a two-galaxy merger
+    (eps 0.005, disk velocities from `solver` when given) with equal odds,
+    N = exp(U[ln nmin, ln nmax]), p uniform over `orders`, 1..max_steps
+    leapfrog steps of dt = 0.01. Returns dicts with pos, vel, mass, eps, p,
+    steps, t (arrival, seconds from the start), kind."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    jobs = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        p = int(rng.choice(list(orders)))
+        steps = int(rng.integers(1, max_steps + 1))
+        if rng.random() < 0.5:
+            pos, vel, m = plummer(n, seed=2000 + k)
+            eps, kind = 0.01, "plummer"
+        else:
+            pos, vel, m, _ = merger(max(n // 2, 1), seed=2000 + k, solver=solver)
+            eps, kind = 0.005, "merger"
+        jobs.append(dict(pos=pos, vel=vel, mass=m, eps=eps, p=p, steps=steps, t=float(arrivals[k]), kind=kind))
+    return jobs

@@ -55,6 +55,29 @@ def test_spheres():
     q = (10.0 / 10.5) / np.sqrt(2.0)
     assert abs(np.median(r) - 0.5 * q / (1 - q)) < 0.03
     pos, vel, m = I.plummer(20000, seed=2)
-    r = np.linalg.norm(pos, axis=1)
+    r = np.linalg.norm(pos - (m[:, None] * pos).sum(0), axis=1)
     assert abs(np.median(r) - 1.3048) < 0.05  # Plummer half-mass radius a / sqrt(2^{2/3} - 1)
     assert np.abs((m[:, None] * vel).sum(0)).max() < 1e-12
+
+
+def test_plummer_is_the_reference_stream():
+    """ics.plummer (afmm_make_plummer) draws makePlummerSphere's libstdc++
+    mt19937 stream (InitialConditions3D.hpp:100-152): the C0 input (N=16384,
+    seed 42) has the digest the compiled reference wrote, and other sizes and
+    seeds equal the oracle's generator bit for bit (and the compiled
+    reference's where it is built)."""
+    import hashlib
+    I = ics()
+    pos, vel, m = I.plummer(16384, seed=42)
+    h = hashlib.sha256(pos.tobytes() + vel.tobytes() + m.tobytes()).hexdigest()
+    assert h == "6bb78e6d31f40b2c491a99e471c02b9a0c9f600b93c35e7859564854653dc5d5", h
+    orc = cabi.oracle()
+    ref = cabi.reference()
+    for n, seed, a, M, G in ((1, 0, 1.0, 1.0, 1.0), (777, 7, 0.5, 3.0, 1.7), (5000, 2**32 - 1, 2.0, 1.0, 1.0)):
+        got = I.plummer(n, a=a, M=M, G=G, seed=seed)
+        want = orc.plummer(n, a, M, G, seed)
+        for x, y in zip(got, want):
+            assert np.array_equal(x, y), (n, seed)
+        if ref is not None:
+            for x, y in zip(got, ref.plummer(n, a, M, G, seed)):
+                assert np.array_equal(x, y), (n, seed)

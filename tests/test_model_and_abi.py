@@ -44,7 +44,7 @@ def test_model_all_orders_against_oracle():
 
 def test_library_exports_every_declared_symbol():
     hdr = open(os.path.join(cabi.ROOT, "include", "afmm.h")).read()
-    names = set(re.findall(r"^\s*(?:int|void|const char\*)\s+(afmm_\w+)\s*\(", hdr, re.M))
+    names = set(re.findall(r"^\s*(?:int|int64_t|void|const char\*)\s+(afmm_\w+)\s*\(", hdr, re.M))
     assert {"afmm_create", "afmm_eval", "afmm_eval_device", "afmm_tree_export"} <= names
     lib = C.CDLL(os.path.join(cabi.ROOT, "on-galaxy_amd", "libafmm.so"))
     for n in sorted(names):
@@ -69,3 +69,28 @@ def test_context_switches_reject_null_without_device():
         f.argtypes = [C.c_void_p, C.c_int]
         f.restype = C.c_int
         assert f(None, 1) == -1 and f(None, 0) == -1, name
+
+
+def test_system_offsets_rejected_before_device_work():
+    """The binding's batch and leapfrog wrappers check sys_off against the
+    arrays it indexes (the C side copies 3 * sys_off[-1] doubles from and into
+    them), so a short, non-increasing or shifted offset array raises
+    ValueError instead of overrunning a numpy buffer."""
+    mod = cabi.product()
+    pos = np.zeros((10, 3))
+    mass = np.ones(10)
+    good = mod._offsets([0, 4, 10], 10)
+    assert good.dtype == np.int64 and list(good) == [0, 4, 10]
+    for bad in ([0, 4, 12], [0, 4, 9], [1, 4, 10], [0, 4, 4, 10], [0, 6, 4, 10], [0], []):
+        with pytest.raises(ValueError):
+            mod._offsets(bad, 10)
+    solver = mod.AdaptiveFMM3D.__new__(mod.AdaptiveFMM3D)  # no context: nothing may reach the device
+    solver.lib = None
+    solver.ctx = None
+    for bad in ([0, 4, 12], [0, 11], [2, 10]):
+        with pytest.raises(ValueError):
+            solver.computeBatch(pos, mass, bad)
+        with pytest.raises(ValueError):
+            mod.LeapfrogState(solver, pos, pos, mass, sys_off=bad)
+    with pytest.raises(ValueError):
+        solver.computeBatch(pos, mass[:9], [0, 9])

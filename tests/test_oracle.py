@@ -49,6 +49,58 @@ def test_forces_match_reference_fixture(orc, case):
         assert np.array_equal(acc, g[f"acc_p{p}"]), p
 
 
+@pytest.mark.parametrize("case", FORCE_CASES)
+def test_direct_targets_match_reference_fixture(case):
+    """orc_direct_targets, the checker of the GPU yardstick afmm_direct, over
+    every target against DirectSolver3D's accelerations written by the compiled
+    reference (DirectSolver3D.hpp:15-34). The reference sums pairs i < j once
+    and the target sum runs j in order, so the bits differ by summation order
+    only: max relative difference <= 1e-13 (measured <= 2e-14)."""
+    g = np.load(os.path.join(GOLD, f"force_{case}.npz"))
+    n = len(g["mass"])
+    acc = cabi.direct_targets(g["pos"], g["mass"], np.arange(n), 1.0, float(g["eps"]))
+    ref = g["direct"]
+    mx, _ = cabi.rel_err(ref, acc)
+    assert mx <= 1e-13, mx
+    # a subset of targets in a scrambled order returns the same rows
+    sel = np.random.default_rng(5).permutation(n)[: max(1, n // 3)]
+    assert np.array_equal(cabi.direct_targets(g["pos"], g["mass"], sel, 1.0, float(g["eps"])), acc[sel])
+
+
+def test_direct_targets_closed_forms():
+    """The softened pair law with G != 1 and eps > 0 (Kernel3D.hpp:31-35,
+    DirectSolver3D.hpp:25-30): two bodies, a_1 = G m_2 d / (|d|^2 + eps^2)^{3/2}
+    with d = x_2 - x_1 and the opposite sign on body 2; three bodies are the sum
+    of their pair terms, and the self pair is skipped. A wrong G or eps
+    placement, a sign flip or a self term fails one of these."""
+    G, eps = 1.7, 0.02
+    pos = np.array([[0.1, -0.2, 0.3], [0.4, 0.25, -0.9], [-1.1, 0.6, 0.05]])
+    m = np.array([0.7, 1.9, 0.4])
+
+    def pair(i, j):
+        d = pos[j] - pos[i]
+        return G * m[j] * d / (d @ d + eps * eps) ** 1.5
+
+    two = cabi.direct_targets(pos[:2], m[:2], [0, 1], G, eps)
+    assert np.allclose(two[0], pair(0, 1), rtol=1e-15, atol=0)
+    assert np.allclose(two[1], pair(1, 0), rtol=1e-15, atol=0)
+    three = cabi.direct_targets(pos, m, [2, 0, 1], G, eps)
+    for row, i in zip(three, (2, 0, 1)):
+        want = sum(pair(i, j) for j in range(3) if j != i)
+        assert np.allclose(row, want, rtol=1e-14, atol=0)
+    # G enters as a factor, eps inside the distance: doubling G doubles the
+    # result bit for bit; eps -> 0 gives the plain inverse square
+    assert np.array_equal(cabi.direct_targets(pos, m, [1], 2 * G, eps), 2 * cabi.direct_targets(pos, m, [1], G, eps))
+    d = pos[1] - pos[0]
+    plain = cabi.direct_targets(pos[:2], m[:2], [0], 1.0, 0.0)[0]
+    assert np.allclose(plain, m[1] * d / np.linalg.norm(d) ** 3, rtol=1e-15, atol=0)
+    ref = cabi.reference()
+    if ref is not None:  # the compiled DirectSolver3D at the same G, eps
+        pos4, m4 = cabi.uniform_cube(300, 8)
+        mx, _ = cabi.rel_err(ref.direct(pos4, m4, G, eps), cabi.direct_targets(pos4, m4, np.arange(300), G, eps))
+        assert mx <= 1e-13, mx
+
+
 @pytest.mark.parametrize("p", [4, 8, 12])
 def test_operators_match_reference_fixture(orc, p):
     """P2M/M2M/M2L/L2L/localField/multipoleField/computeY
