@@ -70,24 +70,104 @@ def measured_traffic(kernel_key):
     return None
 
 
+def host_cpu():
+    """Core count and CPU model of the host the CPU figures were taken on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)), "model": model}
+
+
 def cpu_baseline(pos, mass, args):
-    """Times the CPU adaptive FMM on a bounded random subsample (single
-    thread) and scales it to s/eval at the full N (the method is O(N))."""
+    """Times the CPU adaptive FMM (single thread, as the reference has no
+    threads) on two bounded random subsamples of the workload and scales each
+    to s/eval at the full N (the method is O(N)); the value is the larger
+    sample's figure, the smaller one shows how far the linear scaling holds."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import cabi
     ref = cabi.reference()
     impl, kind = (ref, "reference") if ref is not None else (cabi.oracle(), "port")
+    what = ("AdaptiveFMM3D (reference, compiled from its sources)" if kind == "reference"
+            else "oracle restatement")
     rng = np.random.default_rng(1)
-    idx = np.sort(rng.choice(len(mass), args.cpu_sample, replace=False))
-    sp, sm = np.ascontiguousarray(pos[idx]), np.ascontiguousarray(mass[idx])
-    t0 = time.perf_counter()
-    impl.fmm(sp, sm, 1.0, args.eps, args.p, args.cap)
-    dt = time.perf_counter() - t0
-    return {"value": dt * len(mass) / args.cpu_sample, "unit": "s/eval", "cores": 1, "kind": kind,
-            "sample": f"one {('AdaptiveFMM3D (reference, compiled from its sources)' if kind == 'reference' else 'oracle restatement')} "
-                      f"evaluation of a random {args.cpu_sample}-body subsample of the same merger "
-                      f"(p={args.p}, cap {args.cap}, eps {args.eps}) took {dt:.2f} s on 1 host core; "
-                      f"value = that x N/{args.cpu_sample}"}
+    runs = {}
+    for ns in (args.cpu_sample, 4 * args.cpu_sample):
+        idx = np.sort(rng.choice(len(mass), ns, replace=False))
+        sp, sm = np.ascontiguousarray(pos[idx]), np.ascontiguousarray(mass[idx])
+        t0 = time.perf_counter()
+        impl.fmm(sp, sm, 1.0, args.eps, args.p, args.cap)
+        dt = time.perf_counter() - t0
+        runs[ns] = {"sample_bodies": ns, "seconds": dt, "scaled_s_per_eval": dt * len(mass) / ns}
+    big = runs[4 * args.cpu_sample]
+    small = runs[args.cpu_sample]
+    return {"value": big["scaled_s_per_eval"], "unit": "s/eval", "cores": 1, "kind": kind,
+            "host": host_cpu(),
+            "sample": f"one {what} evaluation of a random {big['sample_bodies']}-body subsample of the same "
+                      f"merger (p={args.p}, cap {args.cap}, eps {args.eps}) took {big['seconds']:.2f} s on 1 "
+                      f"host core; value = that x N/{big['sample_bodies']}",
+            "subsamples": list(runs.values()),
+            "scaling_check": big["scaled_s_per_eval"] / small["scaled_s_per_eval"]}
+
+
+def c0_leg(afmm, args):
+    """The reference's CPU configuration: N = 16384 Plummer sphere
+    (makePlummerSphere, a = 1, M = 1, G = 1, seed 42), eps 0.01, p = 8, leaf
+    capacity 64, VelocityVerlet3D initialize + one step of dt 0.01 (two force
+    evaluations), host arrays in and out. GPU: FP64 and both mixed forms on the
+    resident leapfrog (upload, two evaluations, download), median of 5 after a
+    warm-up; CPU: the reference's own integrator over its AdaptiveFMM3D on one
+    core (oracle/_ref), best of 3. Also the GPU FP64 state against the CPU
+    state after the step, and the initial accelerations against FP64 direct
+    all-pairs (softened, and at eps = 0 for the truncation error)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cabi
+    n, eps, p, cap, dt = 16384, 0.01, 8, 64, 0.01
+    pos, vel, mass = afmm.make_plummer(n, seed=42)
+    out = {"n": n, "p": p, "eps": eps, "leaf_cap": cap, "dt": dt, "work": "initialize + 1 step (2 evaluations)"}
+    states = {}
+    for name, prec in (("fp64", afmm.PRECISION_FP64), ("mixed_fp32_split", afmm.PRECISION_MIXED),
+                       ("mixed_bf16_split", afmm.PRECISION_MIXED_BF16)):
+        s = afmm.AdaptiveFMM3D(eps=eps, p=p, threshold=cap, precision=prec)
+        ts = []
+        for rep in range(6):
+            t0 = time.perf_counter()
+            st = afmm.LeapfrogState(s, pos, vel, mass)
+            st.step(dt, 1)
+            res = st.download()
+            t = time.perf_counter() - t0
+            st.close()
+            if rep:
+                ts.append(t)
+        states[name] = res
+        out[name] = {"s_median": float(np.median(ts)), "s_min": float(min(ts))}
+        s.close()
+    sd = afmm.AdaptiveFMM3D(eps=eps, p=p, threshold=cap)
+    a0 = sd.computeAccelerations(pos, mass)
+    allt = np.arange(n)
+    out["fp64_vs_direct_softened"] = rel_errors(sd.direct(pos, mass, allt), a0)
+    sd.close()
+    s0 = afmm.AdaptiveFMM3D(eps=0.0, p=p, threshold=cap)
+    out["fp64_vs_direct_eps0"] = rel_errors(s0.direct(pos, mass, allt), s0.computeAccelerations(pos, mass))
+    s0.close()
+    ref = cabi.reference()
+    if ref is not None:
+        best, cpu_state = float("inf"), None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            cpu_state = ref.leapfrog(pos, vel, mass, 1.0, eps, p, cap, dt, 1)
+            best = min(best, time.perf_counter() - t0)
+        out["cpu_reference"] = {"s": best, "cores": 1, "host": host_cpu(),
+                                "what": "VelocityVerlet3D over AdaptiveFMM3D, compiled from the reference's sources"}
+        out["speedup_fp64_vs_cpu"] = best / out["fp64"]["s_median"]
+        g = states["fp64"]
+        out["fp64_state_vs_cpu"] = {k: float(np.abs(g[i] - cpu_state[i]).max() / np.abs(cpu_state[i]).max())
+                                    for i, k in enumerate(("pos", "vel", "acc"))}
+    return out
 
 
 def drift_leg(afmm, ics, args):
@@ -133,7 +213,7 @@ def rel_errors(ref, a):
     return {"rms": float(np.sqrt((d ** 2).sum() / (rn ** 2).sum())), "max_rel": float((d / np.maximum(rn, floor)).max())}
 
 
-def large_leg(afmm, ics, torch, args):
+def large_leg(afmm, ics, args):
     """C2 at N = 4M (2 x 2M merger): s/eval of FP64 and both mixed forms at
     p = 8, FP64 and bf16 mixed at p = 4 and 12, the eps = 0 truncation error
     of each against FP64 direct on 4096 targets, and the 100-step energy drift
@@ -142,10 +222,7 @@ def large_leg(afmm, ics, torch, args):
     pos, vel, mass, _ = ics.merger(args.large_n // 2, seed=args.seed + 9, solver=ic)
     ic.close()
     n = len(mass)
-    dev = torch.device("cuda:0")
-    d_pos = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
-    d_mass = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
-    d_acc = torch.empty_like(d_pos)
+    d_pos, d_mass, d_acc = afmm.DeviceArray(None, pos), afmm.DeviceArray(None, mass), afmm.DeviceArray(pos.shape)
     idx = np.random.default_rng(args.seed + 10).choice(n, 4096, replace=False)
     out = {"n": n}
     ref0 = None
@@ -156,17 +233,14 @@ def large_leg(afmm, ics, torch, args):
         forms += [(f"p{p}_fp64", afmm.PRECISION_FP64, p), (f"p{p}_mixed_bf16_split", afmm.PRECISION_MIXED_BF16, p)]
     for name, prec, p in forms:
         s = afmm.AdaptiveFMM3D(eps=args.eps, p=p, threshold=args.cap, precision=prec)
-        s.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
-        torch.cuda.synchronize()
+        s.compute_device(d_pos.ptr, d_mass.ptr, d_acc.ptr, n)
         t0 = time.perf_counter()
         for _ in range(3):
-            s.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
-        torch.cuda.synchronize()
+            s.compute_device(d_pos.ptr, d_mass.ptr, d_acc.ptr, n)
         t = (time.perf_counter() - t0) / 3
         s0 = afmm.AdaptiveFMM3D(eps=0.0, p=p, threshold=args.cap, precision=prec)
-        s0.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
-        torch.cuda.synchronize()
-        a0 = d_acc.cpu().numpy()[idx]
+        s0.compute_device(d_pos.ptr, d_mass.ptr, d_acc.ptr, n)
+        a0 = d_acc.numpy()[idx]
         if ref0 is None:
             ref0 = s0.direct(pos, mass, idx)
         e = rel_errors(ref0, a0)
@@ -195,7 +269,7 @@ def large_leg(afmm, ics, torch, args):
     return out
 
 
-def batched_leg(afmm, ics, tune, torch, args):
+def batched_leg(afmm, ics, tune, args):
     """256 independent systems (N log-uniform over 2k-200k; even: Plummer a=1,
     odd: Hernquist a=0.5 truncated at 20a; total mass 1 each), p=8,
     eps=0.01, mixed precision (bf16-split M2L, FP32 P2P), leaf capacity from
@@ -206,41 +280,45 @@ def batched_leg(afmm, ics, tune, torch, args):
     parts = ics.batch_systems(args.batch_systems, seed=args.seed + 3)
     ns = np.array([len(mq) for _, mq in parts], np.int64)
     off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
-    dev = torch.device("cuda:0")
-    d_pos = torch.from_numpy(np.ascontiguousarray(np.concatenate([p for p, _ in parts]))).to(dev)
-    d_mass = torch.from_numpy(np.ascontiguousarray(np.concatenate([m for _, m in parts]))).to(dev)
-    d_acc = torch.empty_like(d_pos)
-    P, E = d_pos.data_ptr(), d_mass.data_ptr()
-    A = d_acc.data_ptr()
-    cap, tune_ms = tune.tune_leaf_cap(afmm, off, P, E, A, 8, afmm.PRECISION_MIXED_BF16, eps=0.01)
+    d_pos = afmm.DeviceArray(None, np.concatenate([p for p, _ in parts]))
+    d_mass = afmm.DeviceArray(None, np.concatenate([m for _, m in parts]))
+    d_acc = afmm.DeviceArray(d_pos.shape)
+    P, E = d_pos.ptr, d_mass.ptr
+    A = d_acc.ptr
+    cfg, tune_log = tune.tune(afmm, off, P, E, A, 8, afmm.PRECISION_MIXED_BF16, eps=0.01)
+    cap, shapes = cfg["leaf_cap"], cfg["shapes"]
 
-    def timed_batch(precision, leaf_cap, reps=3, graph=False):
+    def timed_batch(precision, leaf_cap, reps=3, graph=False, sh=None):
         s = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=leaf_cap, precision=precision)
         s.set_graph(graph)
+        if sh:
+            s.set_shapes(**sh)
         s.compute_batch_device(off, P, E, A)
-        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
             s.compute_batch_device(off, P, E, A)
-        torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps, s.stats()
 
-    t_mix, st = timed_batch(afmm.PRECISION_MIXED_BF16, cap, graph=True)
-    t_mix_stream, _ = timed_batch(afmm.PRECISION_MIXED_BF16, cap)
+    t_mix, st = timed_batch(afmm.PRECISION_MIXED_BF16, cap, graph=True, sh=shapes)
+    t_mix_stream, st_stream = timed_batch(afmm.PRECISION_MIXED_BF16, cap, sh=shapes)
+    t_mix_default, _ = timed_batch(afmm.PRECISION_MIXED_BF16, cap)
     t_b64, _ = timed_batch(afmm.PRECISION_FP64, 64)
     # 256 sequential plain solves (FP64, cap 64), device-resident slices
     s = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=64)
     s.compute_device(P, E, A, int(ns[0]))
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for q in range(len(ns)):
         o = int(off[q])
         s.compute_device(P + 24 * o, E + 8 * o, A + 24 * o, int(ns[q]))
-    torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
     inter = float(np.sum(ns.astype(np.float64) * (ns - 1)))
     return {"systems": int(len(ns)), "bodies": int(off[-1]), "p": 8, "eps": 0.01,
-            "leaf_cap": int(cap), "autotune_ms_total_by_cap": {str(k): round(v, 3) for k, v in tune_ms.items()},
+            "leaf_cap": int(cap), "shapes": shapes,
+            "autotune_ms_total_by_cap": {str(k): round(v, 3) for k, v in tune_log["leaf_cap"].items()},
+            "autotune_shape_trials": tune_log["shapes"],
+            "batched_mixed_default_shapes_s_per_step": t_mix_default,
+            "phases_ms_stream_launch": {k[3:]: st_stream[k] for k in ("ms_tree", "ms_lists", "ms_upward", "ms_m2l",
+                                                                      "ms_downward", "ms_leaf", "ms_p2p", "ms_total")},
             "precision": "mixed_bf16", "graph": {"instantiations": int(st["graph_instantiations"]),
                                                  "in_place_updates": int(st["graph_updates"])},
             "batched_mixed_s_per_step": t_mix, "batched_mixed_stream_launch_s_per_step": t_mix_stream,
@@ -249,6 +327,7 @@ def batched_leg(afmm, ics, tune, torch, args):
             "body_interactions_per_s": inter / t_mix,
             "sequential_body_interactions_per_s": inter / t_seq,
             "speedup_vs_sequential": t_seq / t_mix,
+            "speedup_fp64_batched_vs_sequential_fp64": t_seq / t_b64,
             "actual_p2p_pairs": int(st["p2p_pairs"]), "v_pairs": int(st["n_v"]),
             "metric_note": "body-interactions/s = sum_i N_i (N_i - 1) / t_step (direct-equivalent)"}
 
@@ -256,8 +335,12 @@ def batched_leg(afmm, ics, tune, torch, args):
 def serving_leg(afmm, ics, args):
     """A seeded Poisson stream (mean 20 jobs/s) of Plummer or two-galaxy jobs,
     N log-uniform over 10k-2M, p in {4, 8, 12}, 1-50 leapfrog steps of 0.01,
-    through the continuous-batching server, and the same jobs one at a time
-    (each on its own resident leapfrog, in arrival order)."""
+    through the continuous-batching server in FP64 and in bf16 mixed
+    precision, and the same jobs one at a time (FP64, each on its own resident
+    leapfrog, in arrival order). Also the trace submitted all at once (the
+    server's sustainable job rate for this mix), and for a sample of the mixed
+    server's jobs the error of their final accelerations against the FP64 FMM
+    at the same positions and against FP64 direct summation on 1024 targets."""
     import threading
     ic_solver = afmm.AdaptiveFMM3D(eps=0.005, p=4, threshold=64)
     jobs = ics.serving_jobs(args.serve_jobs, seed=args.seed + 4, rate=20.0, solver=ic_solver)
@@ -269,29 +352,51 @@ def serving_leg(afmm, ics, args):
         return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
                 "mean_s": float(lat.mean()), "makespan_s": makespan, "bodies_steps_per_s": work / makespan}
 
-    sv = afmm.Server(precision=afmm.PRECISION_FP64)
-    for p in (4, 8, 12):
-        sv.warm(p)
-    ids = [None] * len(jobs)
-    t0 = time.perf_counter()
+    def served(precision, at_once=False):
+        sv = afmm.Server(precision=precision)
+        for p in (4, 8, 12):
+            sv.warm(p)
+        ids = [None] * len(jobs)
+        t0 = time.perf_counter()
 
-    def submitter():
-        for k, j in enumerate(jobs):
-            dt = t0 + j["t"] - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            ids[k] = sv.submit(j["pos"], j["vel"], j["mass"], 1.0, j["eps"], 0.01, j["p"], j["steps"])
+        def submitter():
+            for k, j in enumerate(jobs):
+                dt = t0 + j["t"] - time.perf_counter()
+                if dt > 0 and not at_once:
+                    time.sleep(dt)
+                ids[k] = sv.submit(j["pos"], j["vel"], j["mass"], 1.0, j["eps"], 0.01, j["p"], j["steps"])
 
-    th = threading.Thread(target=submitter)
-    th.start()
-    th.join()
-    sv.drain()
-    infos = [sv.info(i) for i in ids]
-    lat = [i["t_done"] - i["t_submit"] for i in infos]
-    span = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
-    batched = summarize(lat, span)
-    batched["max_jobs_in_one_batch_evals"] = int(max(i["batches"] for i in infos))
-    sv.close()
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        sv.drain()
+        infos = [sv.info(i) for i in ids]
+        results = [sv.result(i) for i in ids]
+        sv.close()
+        lat = [i["t_done"] - i["t_submit"] for i in infos]
+        span = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
+        out = summarize(lat, span)
+        out["most_batched_evaluations_of_a_job"] = int(max(i["batches"] for i in infos))
+        return out, results
+
+    fp64, _ = served(afmm.PRECISION_FP64)
+    bf16, res_bf = served(afmm.PRECISION_MIXED_BF16)
+    burst, _ = served(afmm.PRECISION_FP64, at_once=True)
+    burst["sustainable_jobs_per_s"] = len(jobs) / burst["makespan_s"]
+    # accuracy of sampled mixed-precision results at their final positions
+    rng = np.random.default_rng(args.seed + 5)
+    acc_rows = []
+    for k in sorted(rng.choice(len(jobs), min(4, len(jobs)), replace=False).tolist()):
+        j = jobs[k]
+        pos, _, acc = res_bf[k]
+        s = afmm.AdaptiveFMM3D(eps=j["eps"], p=j["p"], threshold=64)
+        a64 = s.computeAccelerations(pos, j["mass"])
+        idx = rng.choice(len(j["mass"]), min(1024, len(j["mass"])), replace=False)
+        ref = s.direct(pos, j["mass"], idx)
+        s.close()
+        acc_rows.append({"job": k, "kind": j["kind"], "n": int(len(j["mass"])), "p": j["p"], "steps": j["steps"],
+                         "mixed_vs_fp64_fmm": rel_errors(a64, acc), "mixed_vs_fp64_direct": rel_errors(ref, acc[idx]),
+                         "fp64_fmm_vs_fp64_direct": rel_errors(ref, a64[idx])})
     # plain one-job-at-a-time path
     lat2 = []
     t0 = time.perf_counter()
@@ -309,8 +414,13 @@ def serving_leg(afmm, ics, args):
     span2 = time.perf_counter() - t0
     return {"jobs": len(jobs), "arrival_rate_per_s": 20.0, "total_bodies_steps": work,
             "n_range": [int(min(len(j["mass"]) for j in jobs)), int(max(len(j["mass"]) for j in jobs))],
-            "orders": sorted({j["p"] for j in jobs}), "precision": "fp64",
-            "server": batched, "one_at_a_time": summarize(lat2, span2)}
+            "orders": sorted({j["p"] for j in jobs}),
+            "server": fp64, "server_mixed_bf16": bf16, "server_all_at_once_fp64": burst,
+            "mixed_accuracy_sampled_jobs": acc_rows,
+            "accuracy_note": "eps > 0: the FMM softens the near field only (reference semantics), so against "
+                             "softened direct summation the far field differs by ~eps^2/r^2; mixed_vs_fp64_fmm "
+                             "is the error the reduced precision adds",
+            "one_at_a_time": summarize(lat2, span2)}
 
 
 def main():
@@ -327,7 +437,7 @@ def main():
     ap.add_argument("--targets", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=256)
-    ap.add_argument("--serve-jobs", type=int, default=16)
+    ap.add_argument("--serve-jobs", type=int, default=32)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=100)
@@ -336,17 +446,17 @@ def main():
     ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-orders", action="store_true")
+    ap.add_argument("--no-c0", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (scale for HBM counters) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
-        import torch
-        a = torch.empty(args.copy_calibration << 17, dtype=torch.float64, device="cuda:0").fill_(1.0)
-        b = torch.empty_like(a)
+        afmm, _, _ = load_pkg()
+        a = afmm.DeviceArray(None, np.ones(args.copy_calibration << 17))
+        b = afmm.DeviceArray(a.shape)
         for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": a.numel() * 8, "launches": 5}), flush=True)
+            b.copy_from(a)
+        print(json.dumps({"copy_bytes_per_launch": a.nbytes, "launches": 5}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: "
@@ -354,14 +464,10 @@ def main():
               "across GPUs, one large system partitioned by Morton key range)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     afmm, ics, tune = load_pkg()
     pos, _, mass, _ = ics.merger(args.n_per_galaxy, seed=args.seed)
     n = len(mass)
-    dev = torch.device("cuda:0")
-    d_pos = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
-    d_mass = torch.from_numpy(np.ascontiguousarray(mass)).to(dev)
-    d_acc = torch.empty_like(d_pos)
+    d_pos, d_mass, d_acc = afmm.DeviceArray(None, pos), afmm.DeviceArray(None, mass), afmm.DeviceArray(pos.shape)
     def timed(precision, p=None, steps=None, overlap=True):
         """W warmup + K timed evaluations; returns (s/eval, averaged phase ms, last stats, solver).
         overlap=False runs every pass in order on one stream."""
@@ -371,18 +477,15 @@ def main():
         solver.set_overlap(overlap)
 
         def step():
-            solver.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
+            solver.compute_device(d_pos.ptr, d_mass.ptr, d_acc.ptr, n)
 
-        torch.cuda.synchronize()
         for _ in range(args.warmup):
             step()
-        torch.cuda.synchronize()
         phases = []
         t0 = time.perf_counter()
         for _ in range(nsteps):
             step()
             phases.append(solver.stats())
-        torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         keys = ("ms_tree", "ms_lists", "ms_upward", "ms_m2l", "ms_downward", "ms_leaf", "ms_total", "ms_p2p")
         avg = {k[3:]: float(np.mean([ph[k] for ph in phases])) for k in keys}
@@ -390,7 +493,7 @@ def main():
 
     progress(f"FP64 timing, n={n}")
     s_per_eval, avg, st, solver = timed(afmm.PRECISION_FP64)
-    acc64 = d_acc.cpu().numpy()
+    acc64 = d_acc.numpy()
     S = (args.p + 1) ** 2
     m2l_flops = 2.0 * S * S * st["n_v"]
     achieved = m2l_flops / (avg["m2l"] * 1e-3) / 1e12
@@ -406,18 +509,23 @@ def main():
     seq_s, seq_avg, _, seq_solver = timed(afmm.PRECISION_FP64, steps=max(3, args.steps // 2), overlap=False)
     seq_solver.close()
 
-    # PCIe-inclusive drop-in semantics (host arrays in and out), reported aside
-    t1 = time.perf_counter()
+    # PCIe-inclusive drop-in semantics (host arrays in and out), reported aside:
+    # median of 5 after a warm-up
     solver.computeAccelerations(pos, mass)
-    host_io = time.perf_counter() - t1
+    host_runs = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        solver.computeAccelerations(pos, mass)
+        host_runs.append(time.perf_counter() - t1)
+    host_io = float(np.median(host_runs))
 
     progress("mixed-precision timing")
     # mixed precision on the same bodies: split-FP32 and three-piece bf16 M2L
     mix_s, mix_avg, _, _ = timed(afmm.PRECISION_MIXED)
-    accmx = d_acc.cpu().numpy()
+    accmx = d_acc.numpy()
     mix_achieved = m2l_flops / (mix_avg["m2l"] * 1e-3) / 1e12
     bf_s, bf_avg, _, _ = timed(afmm.PRECISION_MIXED_BF16)
-    accbf = d_acc.cpu().numpy()
+    accbf = d_acc.numpy()
     bf_achieved = m2l_flops / (bf_avg["m2l"] * 1e-3) / 1e12
 
     # accuracy: FP64 direct summation (on the GPU) at random targets. Like the
@@ -445,9 +553,8 @@ def main():
     for name, prec in (("fp64", afmm.PRECISION_FP64), ("mixed", afmm.PRECISION_MIXED),
                        ("mixed_bf16", afmm.PRECISION_MIXED_BF16)):
         s0 = afmm.AdaptiveFMM3D(G=1.0, eps=0.0, p=args.p, threshold=args.cap, precision=prec)
-        s0.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
-        torch.cuda.synchronize()
-        a0 = d_acc.cpu().numpy()
+        s0.compute_device(d_pos.ptr, d_mass.ptr, d_acc.ptr, n)
+        a0 = d_acc.numpy()
         if ref0 is None:
             ref0 = s0.direct(pos, mass, idx)
         accuracy["eps0"][name + "_vs_direct"] = errs(ref0, a0[idx])
@@ -465,9 +572,8 @@ def main():
                 t_o, ph_o, _, so = timed(prec, p=po, steps=3)
                 so.close()
                 s0 = afmm.AdaptiveFMM3D(G=1.0, eps=0.0, p=po, threshold=args.cap, precision=prec)
-                s0.compute_device(d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), n)
-                torch.cuda.synchronize()
-                a0 = d_acc.cpu().numpy()
+                s0.compute_device(d_pos.ptr, d_mass.ptr, d_acc.ptr, n)
+                a0 = d_acc.numpy()
                 s0.close()
                 row[name] = {"s_per_eval": t_o, "m2l_ms": ph_o["m2l"], "eps0_vs_direct": errs(ref0, a0[idx])}
             row["mixed_bf16_speedup_vs_fp64"] = row["fp64"]["s_per_eval"] / row["mixed_bf16"]["s_per_eval"]
@@ -510,18 +616,21 @@ def main():
            "host_io_s_per_eval": host_io}
     if not args.no_large:
         progress("4M leg")
-        out["large_4m"] = large_leg(afmm, ics, torch, args)
+        out["large_4m"] = large_leg(afmm, ics, args)
     if not args.no_drift:
         progress("energy drift leg")
         out["energy_drift"] = drift_leg(afmm, ics, args)
     if not args.no_batch:
         progress("batched leg")
-        out["batched"] = batched_leg(afmm, ics, tune, torch, args)
+        out["batched"] = batched_leg(afmm, ics, tune, args)
     if not args.no_serve:
         progress("serving leg")
         out["serving"] = serving_leg(afmm, ics, args)
         out["latency_p50_s"] = out["serving"]["server"]["p50_s"]
         out["latency_p99_s"] = out["serving"]["server"]["p99_s"]
+    if not args.no_c0:
+        progress("C0 leg (16384-body Plummer, initialize + one step)")
+        out["c0"] = c0_leg(afmm, args)
     if not args.no_cpu:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(pos, mass, args)

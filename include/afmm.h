@@ -47,6 +47,13 @@ typedef enum {
 #define AFMM_PRECISION_FP64 0
 #define AFMM_PRECISION_MIXED 1
 #define AFMM_PRECISION_MIXED_BF16 2
+/* Attribution modes (no counterpart in the reference): one of the two
+ * reduced-precision phases alone, everything else FP64, so that the error
+ * each contributes can be stated apart. M2L_BF16: the bf16 M2L of
+ * MIXED_BF16 with the FP64 P2P; P2P_FP32: the FP64 M2L with the FP32 P2P of
+ * the mixed modes. */
+#define AFMM_PRECISION_M2L_BF16 3
+#define AFMM_PRECISION_P2P_FP32 4
 
 /* Solver parameters: AdaptiveFMM3D(KernelParams3D{G, eps}, p, threshold)
  * (AdaptiveFMM3D.hpp:22-23; Kernel3D.hpp:24-27). The eps also floors the leaf
@@ -57,7 +64,7 @@ typedef struct {
     double eps;
     int32_t p;          /* expansion order, 1..AFMM_MAX_ORDER */
     int32_t leaf_cap;   /* split a box while it holds > leaf_cap bodies (64) */
-    int32_t precision;  /* AFMM_PRECISION_FP64, _MIXED or _MIXED_BF16 */
+    int32_t precision;  /* AFMM_PRECISION_* */
     int32_t device;     /* HIP device ordinal */
 } afmm_params;
 
@@ -269,6 +276,25 @@ int afmm_synchronize(afmm_ctx* ctx);
  * the previous one updates the executable graph in place. Results are the
  * same as without it. */
 int afmm_set_graph(afmm_ctx* ctx, int enable);
+
+/* Kernel shapes (no counterpart in the reference, whose CUDA twin runs one
+ * thread per target box or body): chosen by the project's autotuner
+ * (on-galaxy_amd/tune.py) or left at their defaults. A zero field keeps the
+ * default. Every shape gives the same tree and lists; M2L waves and the P2P
+ * schedule also leave the results' bits unchanged, while the P2P lane grouping
+ * changes the order of each near-field sum (same accuracy, different last bits). */
+typedef struct {
+    int32_t m2l_waves;        /* waves per M2L workgroup, 16 targets each: 8, 12 or 16 (default 16
+                                 at p <= 8 or with bf16 M2L, else 8; 16 needs one of those) */
+    int32_t p2p_pair_bodies;  /* leaves with more bodies give each P2P lane two targets (default 32) */
+    int32_t p2p_min_group;    /* fewest lanes sharing one target's sources: 1, 2, 4, ..., 64 (default 8) */
+    int32_t p2p_schedule;     /* 1: persistent workgroups over the leaves sorted by work bin, largest
+                                 first; 2: one workgroup per leaf in XCD-local chunks of tree order
+                                 (the default) */
+} afmm_shapes;
+int afmm_set_shapes(afmm_ctx* ctx, const afmm_shapes* shapes);
+/* The shapes in effect, defaults filled in. */
+int afmm_get_shapes(const afmm_ctx* ctx, afmm_shapes* out);
 
 /* Two-stream overlap (no counterpart in the reference, which has one host
  * thread and one stream): when enabled (the default), an evaluation outside

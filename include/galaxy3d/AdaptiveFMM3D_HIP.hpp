@@ -57,6 +57,8 @@ public:
         switch (precision_) {
             case AFMM_PRECISION_MIXED: return "AdaptiveFMM3D(CGR, MI355X HIP mixed split-FP32)";
             case AFMM_PRECISION_MIXED_BF16: return "AdaptiveFMM3D(CGR, MI355X HIP mixed bf16-split)";
+            case AFMM_PRECISION_M2L_BF16: return "AdaptiveFMM3D(CGR, MI355X HIP bf16-split M2L, FP64 P2P)";
+            case AFMM_PRECISION_P2P_FP32: return "AdaptiveFMM3D(CGR, MI355X HIP FP64 M2L, FP32 P2P)";
             default: return "AdaptiveFMM3D(CGR, MI355X HIP FP64)";
         }
     }

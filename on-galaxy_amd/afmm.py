@@ -25,6 +25,13 @@ _bp = C.POINTER(C.c_int8)
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECISION_MIXED_BF16 = 2
+# one reduced-precision phase alone (error attribution): bf16 M2L with FP64
+# P2P, and FP64 M2L with FP32 P2P
+PRECISION_M2L_BF16 = 3
+PRECISION_P2P_FP32 = 4
+
+AFMM_ERR_DEVICE = -5
+AFMM_ERR_NO_MEMORY = -6
 
 
 class AfmmError(RuntimeError):
@@ -49,6 +56,11 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Shapes(C.Structure):
+    _fields_ = [("m2l_waves", C.c_int32), ("p2p_pair_bodies", C.c_int32), ("p2p_min_group", C.c_int32),
+                ("p2p_schedule", C.c_int32)]
 
 
 class Tree(C.Structure):
@@ -128,6 +140,8 @@ def library():
         L.afmm_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.afmm_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.afmm_set_overlap.argtypes = [C.c_void_p, C.c_int]
+        L.afmm_set_shapes.argtypes = [C.c_void_p, C.POINTER(Shapes)]
+        L.afmm_get_shapes.argtypes = [C.c_void_p, C.POINTER(Shapes)]
         L.afmm_tree_export.argtypes = [C.c_void_p, C.POINTER(Tree)]
         L.afmm_make_plummer.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint32, _dp, _dp, _dp]
         _lib = L
@@ -197,7 +211,8 @@ class AdaptiveFMM3D:
                             self.lib.afmm_strerror(rc).decode())
 
     def name(self):
-        prec = {PRECISION_MIXED: "mixed", PRECISION_MIXED_BF16: "mixed-bf16"}.get(self.params.precision, "FP64")
+        prec = {PRECISION_MIXED: "mixed", PRECISION_MIXED_BF16: "mixed-bf16", PRECISION_M2L_BF16: "bf16-M2L",
+                PRECISION_P2P_FP32: "FP32-P2P"}.get(self.params.precision, "FP64")
         return f"AdaptiveFMM3D(MI355X HIP, {prec})"
 
     def computeAccelerations(self, pos, mass):
@@ -283,6 +298,17 @@ class AdaptiveFMM3D:
         """Graph mode: the FMM passes of each evaluation are captured into a
         HIP graph and launched from it (results unchanged)."""
         self._check(self.lib.afmm_set_graph(self.ctx, 1 if enable else 0))
+
+    def set_shapes(self, m2l_waves=0, p2p_pair_bodies=0, p2p_min_group=0, p2p_schedule=0):
+        """Kernel shapes (afmm_set_shapes; 0 keeps a default). p2p_schedule 1 is
+        the persistent work-binned P2P, 2 the per-leaf grid in tree order."""
+        sh = Shapes(m2l_waves, p2p_pair_bodies, p2p_min_group, p2p_schedule)
+        self._check(self.lib.afmm_set_shapes(self.ctx, C.byref(sh)))
+
+    def shapes(self):
+        sh = Shapes()
+        self._check(self.lib.afmm_get_shapes(self.ctx, C.byref(sh)))
+        return {k: getattr(sh, k) for k, _ in Shapes._fields_}
 
     def set_overlap(self, enable=True):
         """Two-stream overlap (default on): the upward pass beside the list
@@ -440,6 +466,75 @@ class Server:
         if getattr(self, "h", None):
             self.lib.afmm_server_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """An array (float64 or int64) in device memory, allocated through the HIP runtime that
+    libafmm.so itself links (libamdhip64.so.7), so that device-pointer entry
+    points (compute_device, compute_batch_device) can be driven without a
+    second runtime in the process. `ptr` is the device address."""
+    _hip = None
+
+    @classmethod
+    def _runtime(cls):
+        if cls._hip is None:
+            library()  # loads libamdhip64.so.7 as libafmm's dependency
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, shape, host=None, dtype=np.float64):
+        self.dtype = np.dtype(dtype if host is None else np.asarray(host).dtype)
+        if self.dtype not in (np.dtype(np.float64), np.dtype(np.int64)):
+            raise ValueError("DeviceArray holds float64 or int64")
+        self.shape = tuple(np.atleast_1d(shape)) if host is None else np.shape(host)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        h = self._runtime()
+        p = C.c_void_p()
+        rc = h.hipMalloc(C.byref(p), max(self.nbytes, 8))
+        if rc:
+            raise AfmmError(AFMM_ERR_NO_MEMORY, f"hipMalloc({self.nbytes}) failed: {rc}")
+        self.ptr = p.value
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        rc = self._runtime().hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.nbytes, 1)
+        if rc:
+            raise AfmmError(AFMM_ERR_DEVICE, f"hipMemcpy H2D failed: {rc}")
+
+    def copy_from(self, other):
+        """Device-to-device copy of another DeviceArray of the same size."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        rc = self._runtime().hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr), self.nbytes, 3)
+        if rc:
+            raise AfmmError(AFMM_ERR_DEVICE, f"hipMemcpy D2D failed: {rc}")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        rc = self._runtime().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, 2)
+        if rc:
+            raise AfmmError(AFMM_ERR_DEVICE, f"hipMemcpy D2H failed: {rc}")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self._runtime().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
 
     def __del__(self):
         try:

@@ -33,8 +33,7 @@ int check_params(const afmm_params* p) {
     if (!(std::isfinite(p->G)) || !(std::isfinite(p->eps)) || p->eps < 0) return AFMM_ERR_ARG;
     if (p->p < 1 || p->p > kMaxOrder) return AFMM_ERR_ORDER;
     if (p->leaf_cap < 1) return AFMM_ERR_LEAF_CAP;
-    if (p->precision != AFMM_PRECISION_FP64 && p->precision != AFMM_PRECISION_MIXED &&
-        p->precision != AFMM_PRECISION_MIXED_BF16)
+    if (p->precision < AFMM_PRECISION_FP64 || p->precision > AFMM_PRECISION_P2P_FP32)
         return AFMM_ERR_PRECISION;
     return AFMM_OK;
 }
@@ -104,7 +103,7 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     // the U, W and X lists that M2L does not need.
     bool prep_done = false;
     auto on_vlists = [&](const TreeView& t, const TreeMeta&) {
-        run_m2l_prep(c->pool, s, t, c->prm.p, c->prm.precision);
+        run_m2l_prep(c->pool, s, t, c->prm.p, c->prm.precision, c->shapes);
         prep_done = true;
     };
     build_forest(c->pool, s, pos, mass, nsys, sys_off, eps, c->prm.leaf_cap, tv, meta, roots,
@@ -123,11 +122,11 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         gside.upward_launched = up_launched;
         gside.prep_done = prep_done;
         // size every buffer first, so that the capture allocates nothing
-        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr, true, gside);
+        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, c->shapes, acc, phi, nullptr, true, gside);
         hipGraph_t g = nullptr;
         AFMM_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, nullptr, false, gside);
+            run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, c->shapes, acc, phi, nullptr, false, gside);
         } catch (...) {
             if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
             throw;
@@ -159,7 +158,7 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     } else {
         SideStream side{};
         if (split) side = SideStream{c->side, c->side_fork, c->side_join, up_launched, c->up_done, prep_done};
-        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, acc, phi, c->ev + 2, false, side);
+        run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, c->shapes, acc, phi, c->ev + 2, false, side);
     }
     AFMM_CHECK(hipStreamSynchronize(s));
     c->tv = tv;
@@ -252,6 +251,7 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
     c->prm = *prm;
     const int r = guarded(c, [&]() -> int {
         AFMM_CHECK(hipSetDevice(prm->device));
+        c->shapes.m2l_waves = default_m2l_waves(prm->p, prm->precision);
         AFMM_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto& e : c->ev) AFMM_CHECK(hipEventCreate(&e));
         const Tables& T = tables_for_order(prm->p);
@@ -264,7 +264,7 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
         AFMM_CHECK(hipMemcpy(c->d_m2m_sw, m2m.data(), sizeof(double) * m2m.size(), hipMemcpyHostToDevice));
         AFMM_CHECK(hipMemcpy(c->d_l2l_sw, l2l.data(), sizeof(double) * l2l.size(), hipMemcpyHostToDevice));
         AFMM_CHECK(hipMemcpy(c->d_m2l_sw, m2l.data(), sizeof(double) * m2l.size(), hipMemcpyHostToDevice));
-        if (prm->precision == AFMM_PRECISION_MIXED) {
+        if (m2l_form(prm->precision) == kPrecMixed) {
             std::vector<float2> hl(m2l.size());
             for (size_t i = 0; i < m2l.size(); ++i) {
                 const float hi = (float)m2l[i];
@@ -273,7 +273,7 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
             c->d_m2l_hl = c->pool.get<float2>("T_m2l_hl", hl.size());
             AFMM_CHECK(hipMemcpy(c->d_m2l_hl, hl.data(), sizeof(float2) * hl.size(), hipMemcpyHostToDevice));
         }
-        if (prm->precision == AFMM_PRECISION_MIXED_BF16) {
+        if (m2l_form(prm->precision) == kPrecMixedBf16) {
             const std::vector<uint16_t> bf = mfma_bf16_layout(T.m2l, T.S, kNumOffsets);
             c->d_m2l_bf = c->pool.get<uint4>("T_m2l_bf", bf.size() / 8);
             AFMM_CHECK(hipMemcpy(c->d_m2l_bf, bf.data(), sizeof(uint16_t) * bf.size(), hipMemcpyHostToDevice));
@@ -351,14 +351,13 @@ int afmm_eval_potential(afmm_ctx* c, int64_t n, const double* pos, const double*
         double* dmass = c->pool.get<double>("in_mass", n);
         double* dacc = c->pool.get<double>("out_acc", 3 * (size_t)n);
         double* dphi = phi ? c->pool.get<double>("out_phi", n) : nullptr;
-        AFMM_CHECK(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
-        AFMM_CHECK(hipMemcpyAsync(dmass, mass, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        c->stage.upload(dpos, pos, sizeof(double) * 3 * n, s);
+        c->stage.upload(dmass, mass, sizeof(double) * n, s);
         const int off[2] = {0, (int)n};
         const int rc = eval_forest(c, 1, off, &c->prm.G, &c->prm.eps, dpos, dmass, dacc, dphi);
         if (rc) return rc;
-        AFMM_CHECK(hipMemcpyAsync(acc, dacc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
-        if (phi) AFMM_CHECK(hipMemcpyAsync(phi, dphi, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        AFMM_CHECK(hipStreamSynchronize(s));
+        c->stage.download(acc, dacc, sizeof(double) * 3 * n, s);
+        if (phi) c->stage.download(phi, dphi, sizeof(double) * n, s);
         return AFMM_OK;
     });
 }
@@ -393,13 +392,12 @@ int afmm_eval_batch(afmm_ctx* c, int32_t nsys, const int64_t* sys_off, const dou
         double* dmass = c->pool.get<double>("in_mass", n);
         double* dacc = c->pool.get<double>("out_acc", 3 * (size_t)n);
         double* dphi = phi ? c->pool.get<double>("out_phi", n) : nullptr;
-        AFMM_CHECK(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
-        AFMM_CHECK(hipMemcpyAsync(dmass, mass, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        c->stage.upload(dpos, pos, sizeof(double) * 3 * n, s);
+        c->stage.upload(dmass, mass, sizeof(double) * n, s);
         const int rc = eval_forest(c, nsys, off.data(), g.data(), e.data(), dpos, dmass, dacc, dphi);
         if (rc) return rc;
-        AFMM_CHECK(hipMemcpyAsync(acc, dacc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
-        if (phi) AFMM_CHECK(hipMemcpyAsync(phi, dphi, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        AFMM_CHECK(hipStreamSynchronize(s));
+        c->stage.download(acc, dacc, sizeof(double) * 3 * n, s);
+        if (phi) c->stage.download(phi, dphi, sizeof(double) * n, s);
         return AFMM_OK;
     });
 }
@@ -432,6 +430,38 @@ int afmm_direct(afmm_ctx* c, int64_t n, const double* pos, const double* mass, i
         AFMM_CHECK(hipStreamSynchronize(s));
         return AFMM_OK;
     });
+}
+
+int afmm_set_shapes(afmm_ctx* c, const afmm_shapes* sh) {
+    if (!c) return AFMM_ERR_ARG;
+    Shapes v{};
+    v.m2l_waves = default_m2l_waves(c->prm.p, c->prm.precision);
+    if (sh) {
+        if (sh->m2l_waves) {
+            const bool w16 = c->prm.p <= 8 || m2l_form(c->prm.precision) == kPrecMixedBf16;
+            if (sh->m2l_waves != 8 && sh->m2l_waves != 12 && !(sh->m2l_waves == 16 && w16))
+                return fail(c, AFMM_ERR_ARG, "m2l_waves must be 8 or 12, or 16 at p <= 8 or with bf16 M2L");
+            v.m2l_waves = sh->m2l_waves;
+        }
+        if (sh->p2p_pair_bodies < 0) return fail(c, AFMM_ERR_ARG, "p2p_pair_bodies must be >= 0");
+        if (sh->p2p_pair_bodies) v.p2p_pair_bodies = sh->p2p_pair_bodies;
+        const int g = sh->p2p_min_group;
+        if (g < 0 || g > 64 || (g & (g - 1))) return fail(c, AFMM_ERR_ARG, "p2p_min_group must be 0 or a power of two <= 64");
+        if (g) v.p2p_min_group = g;
+        if (sh->p2p_schedule < 0 || sh->p2p_schedule > 2) return fail(c, AFMM_ERR_ARG, "p2p_schedule must be 0, 1 or 2");
+        v.p2p_schedule = sh->p2p_schedule == 1 ? 1 : 0;
+    }
+    c->shapes = v;
+    return AFMM_OK;
+}
+
+int afmm_get_shapes(const afmm_ctx* c, afmm_shapes* out) {
+    if (!c || !out) return AFMM_ERR_ARG;
+    out->m2l_waves = c->shapes.m2l_waves;
+    out->p2p_pair_bodies = c->shapes.p2p_pair_bodies;
+    out->p2p_min_group = c->shapes.p2p_min_group;
+    out->p2p_schedule = c->shapes.p2p_schedule ? 1 : 2;
+    return AFMM_OK;
 }
 
 int afmm_get_stats(const afmm_ctx* c, afmm_stats* out) {

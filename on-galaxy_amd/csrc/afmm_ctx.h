@@ -24,6 +24,8 @@ struct afmm_ctx {
     afmm::TreeView tv{};
     afmm::TreeMeta meta;
     afmm::RootInfo root{};  // of system 0
+    afmm::Shapes shapes{};  // kernel shapes (afmm_set_shapes), defaults resolved at create
+    afmm::HostStager stage;  // pinned staging of host arrays (afmm_eval, afmm_eval_batch)
     int nsys = 1;
     afmm_stats stats{};
     hipEvent_t ev[10] = {};

@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/afmm.h"
+
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -119,9 +121,9 @@ struct TreeView {
     int* lo_order;       // [B] boxes sorted by (level, octant); level l is [level_off[l], level_off[l+1])
     int* m2l_order;      // [B] boxes sorted by (octant, level): the M2L target order
     // M2L target tiles over m2l_order, cut inside octant groups: tile k is
-    // ranks [m2l_tiles{256,128}[k], [k+1]), at most 256 (128) targets each
-    int *m2l_tiles256, *m2l_tiles128;
-    int m2l_ntiles256 = 0, m2l_ntiles128 = 0;
+    // ranks [m2l_tiles{256,192,128}[k], [k+1]), at most 256 (192, 128) targets each
+    int *m2l_tiles256, *m2l_tiles192, *m2l_tiles128;
+    int m2l_ntiles256 = 0, m2l_ntiles192 = 0, m2l_ntiles128 = 0;
     int* nonleaf_ids;    // [B - nleaves] non-leaf boxes in BFS order
     // lists
     int *coll, *ncoll;   // [27B], [B]
@@ -195,7 +197,13 @@ struct FmmTables {
     const uint4* m2l_bf;            // device, three-piece bf16 layout (mfma_bf16_layout), or null
 };
 // precision of an evaluation (the values of AFMM_PRECISION_*)
-constexpr int kPrecFP64 = 0, kPrecMixed = 1, kPrecMixedBf16 = 2;
+constexpr int kPrecFP64 = 0, kPrecMixed = 1, kPrecMixedBf16 = 2, kPrecM2LBf16 = 3, kPrecP2PFp32 = 4;
+// The M2L form of a precision mode (kPrecFP64, kPrecMixed: split FP32, or
+// kPrecMixedBf16: three-piece bf16) and whether its P2P runs in FP32.
+inline int m2l_form(int prec) {
+    return prec == kPrecMixed ? kPrecMixed : (prec == kPrecMixedBf16 || prec == kPrecM2LBf16) ? kPrecMixedBf16 : kPrecFP64;
+}
+inline bool p2p_fp32(int prec) { return prec == kPrecMixed || prec == kPrecMixedBf16 || prec == kPrecP2PFp32; }
 // Runs the FMM passes on a built tree (forest) and writes G*(far + near) to
 // d_acc in input order (AoS n x 3) and, when d_phi is non-null, the potential
 // -G*(far + near) per body, with G and eps of each body's system (tv.sys_G,
@@ -221,13 +229,31 @@ struct SideStream {
 };
 // The M2L preparation alone on stream s (needs the V lists and the M2L order);
 // a run_fmm with side.prep_done then skips it.
-void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec);
+// Kernel shapes (afmm_shapes of afmm.h with the defaults resolved): waves per
+// M2L workgroup (16 targets each), the leaf size above which a P2P lane takes
+// two targets, the fewest lanes per P2P target, and the P2P schedule (0: leaves
+// in XCD chunks in tree order; 1: persistent workgroups over the leaves sorted
+// by work bin, largest first).
+struct Shapes {
+    int m2l_waves = 16;  // 8, 12 or 16
+    int p2p_pair_bodies = 32;
+    int p2p_min_group = 8;
+    int p2p_schedule = 0;
+};
+// The default M2L workgroup of order p and precision mode prec: 16 waves
+// while the accumulators fit four waves per SIMD, else 8 (the bf16 form keeps
+// an FP64 total beside its FP32 accumulator, which needs 8 from p = 8; 12
+// waves are the autotuner's to choose).
+inline int default_m2l_waves(int p, int prec) {
+    return m2l_form(prec) == kPrecMixedBf16 ? (p <= 7 ? 16 : 8) : (p <= 8 ? 16 : 8);
+}
+void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec, const Shapes& sh);
 // P2M and M2M alone on stream s (events e0/e1 around them when non-null); a
 // run_fmm with side.upward_launched then skips them.
 void run_upward(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
                 hipEvent_t e0, hipEvent_t e1);
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
-             const FmmTables& tab, int prec, double* d_acc, double* d_phi,
+             const FmmTables& tab, int prec, const Shapes& sh, double* d_acc, double* d_phi,
              hipEvent_t* ev, bool reserve_only = false, SideStream side = SideStream{});
 
 // ---- integrate.hip -----------------------------------------------------------
@@ -247,6 +273,27 @@ void leapfrog_kick(hipStream_t s, int n, int njobs, const int* d_boff, const Job
                    const double* bphi);
 // d_out[9q..9q+9) = {KE, PE = 0.5 sum m phi, KE+PE, momentum xyz, angular momentum xyz}
 void diagnostics(hipStream_t s, int njobs, const JobRef* d_jobs, const int* d_n, double* d_out);
+
+// ---- host_io.hip -------------------------------------------------------------
+// Host <-> device copies of the drop-in entry points through two pinned
+// staging buffers per context: the host copies one chunk (several threads)
+// while the DMA of the previous one runs. upload returns once the host array
+// has been read (the DMA may still run on s); download returns with the host
+// array written.
+class HostStager {
+public:
+    HostStager() = default;
+    HostStager(const HostStager&) = delete;
+    HostStager& operator=(const HostStager&) = delete;
+    ~HostStager();
+    void upload(void* d_dst, const void* h_src, size_t bytes, hipStream_t s);
+    void download(void* h_dst, const void* d_src, size_t bytes, hipStream_t s);
+
+private:
+    void reserve();
+    void* pin_[2] = {nullptr, nullptr};
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+};
 
 // ---- direct.hip --------------------------------------------------------------
 // FP64 direct sum of the n bodies (pos AoS n x 3) onto the bodies d_tgt[0..ntgt):

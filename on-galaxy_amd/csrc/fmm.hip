@@ -9,6 +9,7 @@
 // level-independent tables of afmm_tables.cpp (column-major S x S).
 // ============================================================================
 #include <algorithm>
+#include <type_traits>
 
 #include "afmm_internal.h"
 #include "afmm_math.h"
@@ -354,15 +355,12 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
     srcT[(size_t)o * np + rank[b]] = c;
 }
 
-// Waves per M2L workgroup: 16 at p <= 8 (4 per SIMD share the MFMA pipe and
-// hide each other's LDS and gather latency; measured 8% faster than 8 waves of
-// 32 targets), 8 above (the accumulators of 11 row blocks need more registers).
-template <int P>
-constexpr int m2l_waves() { return P <= 8 ? 16 : 8; }
-template <int P>
-constexpr int m2l_cb() { return 1; }  // 16-target column blocks per wave
-template <int P>
-constexpr int m2l_tile() { return m2l_waves<P>() * 16 * m2l_cb<P>(); }
+// Waves per M2L workgroup (W, a kernel shape the autotuner may choose): by
+// default 16 at p <= 8 (4 per SIMD share the MFMA pipe and hide each other's
+// LDS and gather latency; measured 8% faster than 8 waves of 32 targets), 8
+// above (the accumulators of 11 row blocks need more registers). Each wave
+// owns one 16-target column block, so a tile holds 16 W targets.
+constexpr int kM2LCb = 1;  // 16-target column blocks per wave
 
 // present[wg][o] = 1 when some target of tile wg (ranks tstart[wg] ..
 // tstart[wg+1]) has a source at offset o (one wave per (tile, offset)).
@@ -399,8 +397,8 @@ struct M2LTypes<true> {
 __device__ __forceinline__ double a_value(double a) { return a; }
 __device__ __forceinline__ double a_value(float2 a) { return (double)a.x + (double)a.y; }
 
-template <int P, bool MIX>
-__global__ void __launch_bounds__(m2l_waves<P>() * 64) k_m2l_mfma(int np, int ntiles, const int* __restrict__ tstart,
+template <int P, bool MIX, int W>
+__global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const int* __restrict__ tstart,
                                                            const int* __restrict__ order,
                                                            const int* __restrict__ srcT,
                                                            const unsigned char* __restrict__ present_g,
@@ -410,9 +408,9 @@ __global__ void __launch_bounds__(m2l_waves<P>() * 64) k_m2l_mfma(int np, int nt
     using Acc = typename M2LTypes<MIX>::Acc;
     constexpr int S = nterms(P);
     constexpr int RB = m2l_mrb(S), TR = m2l_tail(S), NKC = m2l_mnkc(S), KC = m2l_mkc(S), CH = m2l_mchunk(S);
-    constexpr int CB = m2l_cb<P>();
-    constexpr int TILE = m2l_tile<P>();
-    constexpr int NT = m2l_waves<P>() * 64;
+    constexpr int CB = kM2LCb;
+    constexpr int TILE = W * 16 * CB;
+    constexpr int NT = W * 64;
     constexpr int PER = (CH + NT - 1) / NT;
     // WHOLE: both buffers hold a whole operator (all NKC chunks), so the
     // workgroup synchronises once per offset; otherwise once per chunk.
@@ -433,12 +431,23 @@ __global__ void __launch_bounds__(m2l_waves<P>() * 64) k_m2l_mfma(int np, int nt
         valid[cb] = rank[cb] < end;
     }
     Acc acc[CB][RB];
+    // MIX: each offset's products are summed in FP32 from zero and folded into
+    // FP64 totals when the offset is done, so FP32 rounding never compounds
+    // across the up to 189 offsets of a target.
+    dbl4 accd[MIX ? CB : 1][MIX ? RB : 1];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[cb][rb][r] = 0;
+    if constexpr (MIX)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) accd[cb][rb][r] = 0.0;
     (void)TILE;
     // tail rows 16 RB + tr: lane (j, kq) sums its k = kq (mod 4) terms
     double tacc[CB][TR > 0 ? TR : 1];
@@ -563,6 +572,17 @@ __global__ void __launch_bounds__(m2l_waves<P>() * 64) k_m2l_mfma(int np, int nt
                             for (int cb = 0; cb < CB; ++cb) tacc[cb][tr] = fma(a, bcur[cb][kk], tacc[cb][tr]);
                         }
                 }
+                if constexpr (MIX)
+                    if (cn == 0)  // offset o done: fold its FP32 sums into FP64
+#pragma unroll
+                        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    accd[cb][rb][r] += (double)acc[cb][rb][r];
+                                    acc[cb][rb][r] = 0.f;
+                                }
             }
             if (pf_o < kNumOffsets) {
                 AT* dst = abuf[buf ^ 1] + (WHOLE ? pf_c * CH : 0);
@@ -611,7 +631,10 @@ __global__ void __launch_bounds__(m2l_waves<P>() * 64) k_m2l_mfma(int np, int nt
             for (int r = 0; r < 4; ++r) {
                 // D(i, j) of lane j + 16 kq, register r: i = kq + 4r (f64), 4kq + r (f32)
                 const int row = MIX ? 16 * rb + 4 * kq + r : 16 * rb + kq + 4 * r;
-                if (row < S) L[(size_t)t * S + row] = (double)acc[cb][rb][r];
+                if (row < S) {
+                    if constexpr (MIX) L[(size_t)t * S + row] = accd[cb][rb][r];
+                    else L[(size_t)t * S + row] = acc[cb][rb][r];
+                }
             }
         if constexpr (TR > 0)
             if (kq == 0)
@@ -669,12 +692,9 @@ __global__ void __launch_bounds__(kT) k_split_bf16(int nb, const double* __restr
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-// 16 waves of 16 targets (measured 10% faster than 8 waves at p = 8)
-constexpr int kM2LBfWaves = 16;
-constexpr int kM2LBfTile = kM2LBfWaves * 16;
-
-template <int P>
-__global__ void __launch_bounds__(kM2LBfWaves * 64) k_m2l_bf16(int np, int ntiles, const int* __restrict__ tstart,
+// W waves of 16 targets (by default 16: measured 10% faster than 8 waves at p = 8)
+template <int P, int W>
+__global__ void __launch_bounds__(W * 64) k_m2l_bf16(int np, int ntiles, const int* __restrict__ tstart,
                                                            const int* __restrict__ order,
                                                            const int* __restrict__ srcT,
                                                            const unsigned char* __restrict__ present_g,
@@ -683,7 +703,7 @@ __global__ void __launch_bounds__(kM2LBfWaves * 64) k_m2l_bf16(int np, int ntile
     constexpr int S = nterms(P);
     constexpr int RB = m2l_rb(S), KB = m2l_bf_kb(S), NKC = m2l_bf_nkc(S), KC = m2l_bf_kc(S);
     constexpr int CHU = m2l_bf_chunk(S);
-    constexpr int NT = kM2LBfWaves * 64;
+    constexpr int NT = W * 64;
     constexpr int PER = (CHU + NT - 1) / NT;
     __shared__ uint4 abuf[2][CHU];
     __shared__ unsigned char present[kNumOffsets + 1];
@@ -691,14 +711,20 @@ __global__ void __launch_bounds__(kM2LBfWaves * 64) k_m2l_bf16(int np, int ntile
     const int j = lane & 15, kq = lane >> 4;
     const int tile = xcd_item();
     if (tile >= ntiles) return;
-    const int base = tstart[tile], end = tstart[tile + 1];  // end - base <= kM2LBfTile
+    const int base = tstart[tile], end = tstart[tile + 1];  // end - base <= 16 W
     const int rank = base + wave * 16 + j;
     const bool valid = rank < end;
+    // each offset's products are summed in FP32 from zero and folded into FP64
+    // totals when the offset is done (FP32 rounding never compounds across offsets)
     flt4 acc[RB];
+    dbl4 accd[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[rb][r] = 0.f;
+        for (int r = 0; r < 4; ++r) {
+            acc[rb][r] = 0.f;
+            accd[rb][r] = 0.0;
+        }
     {
         for (int t = tid; t < kNumOffsets; t += NT) present[t] = present_g[(size_t)tile * kNumOffsets + t];
         if (tid == 0) present[kNumOffsets] = 1;  // sentinel
@@ -771,6 +797,14 @@ __global__ void __launch_bounds__(kM2LBfWaves * 64) k_m2l_bf16(int np, int ntile
                         acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, a, 0, 0, 0);
                     }
                 }
+                if (cn == 0)  // offset o done
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            accd[rb][r] += (double)acc[rb][r];
+                            acc[rb][r] = 0.f;
+                        }
             }
             if (on < kNumOffsets) {
 #pragma unroll
@@ -801,7 +835,7 @@ __global__ void __launch_bounds__(kM2LBfWaves * 64) k_m2l_bf16(int np, int ntile
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * rb + 4 * kq + r;
-            if (row < S) L[(size_t)t * S + row] = (double)acc[rb][r];
+            if (row < S) L[(size_t)t * S + row] = accd[rb][r];
         }
 }
 
@@ -967,17 +1001,14 @@ struct P2PAcc {
 };
 
 // MIX: the pair difference stays FP64 (it carries the cancellation), then
-// r^2, v_rsq_f32 and the pair term run in FP32; each lane sums its terms of
-// one 64-body source tile in FP32 and adds that partial to its FP64 accumulator.
+// r^2, v_rsq_f32 and the kernel factor m / r^3 run in FP32; that factor times
+// the FP64 difference is added to the target's FP64 accumulator pair by pair.
 template <bool MIX, bool POT, int R, bool MASK>
 __device__ __forceinline__ void p2p_tile(const double* sx, const double* sy, const double* sz, const double* sm,
                                          const int* sj, int kfirst, int cnt, int Gs, const double (&xi)[R][3],
                                          const int (&ii)[R], double eps2, P2PAcc<POT> (&a)[R]) {
     if constexpr (MIX) {
         const float e2 = (float)eps2;
-        float fx[R], fy[R], fz[R], fp[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) fx[r] = fy[r] = fz[r] = fp[r] = 0.f;
 #pragma unroll 1
         for (int k = kfirst; k < cnt; k += Gs) {
             const double xk = sx[k], yk = sy[k], zk = sz[k];
@@ -985,24 +1016,18 @@ __device__ __forceinline__ void p2p_tile(const double* sx, const double* sy, con
             const int j = MASK ? sj[k] : 0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const float dx = (float)(xk - xi[r][0]), dy = (float)(yk - xi[r][1]), dz = (float)(zk - xi[r][2]);
+                const double ddx = xk - xi[r][0], ddy = yk - xi[r][1], ddz = zk - xi[r][2];
+                const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
                 const bool self = MASK && j == ii[r];
                 const float r2 = self ? 1.f : dx * dx + dy * dy + dz * dz + e2;
                 const float m = self ? 0.f : mk;
                 const float rin = __builtin_amdgcn_rsqf(r2);
-                const float sc = m * (rin * rin * rin);
-                fx[r] += sc * dx;
-                fy[r] += sc * dy;
-                fz[r] += sc * dz;
-                if constexpr (POT) fp[r] += m * rin;
+                const double sc = (double)(m * (rin * rin * rin));
+                a[r].x = fma(sc, ddx, a[r].x);
+                a[r].y = fma(sc, ddy, a[r].y);
+                a[r].z = fma(sc, ddz, a[r].z);
+                if constexpr (POT) a[r].p += (double)(m * rin);
             }
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            a[r].x += (double)fx[r];
-            a[r].y += (double)fy[r];
-            a[r].z += (double)fz[r];
-            a[r].p += (double)fp[r];
         }
     } else {
 #pragma unroll 1
@@ -1033,7 +1058,7 @@ __device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const
                                          double* __restrict__ acc, const double* __restrict__ fpot,
                                          double* __restrict__ phi, double* __restrict__ nr,
                                          double* __restrict__ npot, StreamLds& sl, double* sx, double* sy,
-                                         double* sz, double* sm, int* sj, double (*red)[64]) {
+                                         double* sz, double* sm, int* sj, double (*red)[64], int min_group) {
     const int f = t.seg_first[b], nt = t.seg_count[b];
     const int lane = threadIdx.x;
     const int q = t.bsys[b];
@@ -1045,7 +1070,7 @@ __device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const
     // m/eps is not zero) and for eps = 0.
     const bool mask = POT || !(eps2 >= 1e-20);
     const int ua = t.u_off[li], nu = t.u_off[li + 1] - ua;
-    const int T = min(64, max(8, (nt + R - 1) / R));
+    const int T = min(64, max(min_group, (nt + R - 1) / R));
     const int Gs = 64 / T;
     const int g = lane / T, tl = lane % T;
     const int kfirst = g < Gs ? g : 64;  // lanes past the last whole group idle
@@ -1121,21 +1146,56 @@ __device__ __forceinline__ void p2p_leaf(const TreeView& t, int b, int li, const
     }
 }
 
+// One wave per leaf. Leaves with more than pair_bodies bodies give each lane
+// two targets; at least min_group lanes share one target's sources. Without
+// `order` the grid covers the leaves in XCD chunks of tree order; with it the
+// workgroups are persistent and take order[k] for k = blockIdx.x, +gridDim.x,
+// ... (order lists the leaves by work bin, largest first, so the largest
+// leaves start first and the small ones fill in behind them).
 template <bool MIX, bool POT, bool NEAR>
 __global__ void __launch_bounds__(64) k_p2p(TreeView t, const double* __restrict__ far, double* __restrict__ acc,
                                            const double* __restrict__ fpot, double* __restrict__ phi,
-                                           double* __restrict__ nr, double* __restrict__ npot) {
+                                           double* __restrict__ nr, double* __restrict__ npot, int pair_bodies,
+                                           int min_group, const int* __restrict__ order) {
     __shared__ double sx[64], sy[64], sz[64], sm[64];
     __shared__ int sj[64];
     __shared__ StreamLds sl;
     __shared__ double red[4][64];
-    const int li = xcd_chunk_item();
+    auto leaf = [&](int li) {
+        const int b = t.leaf_ids[li];
+        if (t.seg_count[b] > pair_bodies)
+            p2p_leaf<MIX, POT, 2, NEAR>(t, b, li, far, acc, fpot, phi, nr, npot, sl, sx, sy, sz, sm, sj, red,
+                                        min_group);
+        else
+            p2p_leaf<MIX, POT, 1, NEAR>(t, b, li, far, acc, fpot, phi, nr, npot, sl, sx, sy, sz, sm, sj, red,
+                                        min_group);
+    };
+    if (order) {
+        for (int k = blockIdx.x; k < t.nleaves; k += gridDim.x) {
+            __syncthreads();  // the previous leaf's readers of the LDS tiles are done
+            leaf(order[k]);
+        }
+    } else {
+        const int li = xcd_chunk_item();
+        if (li < t.nleaves) leaf(li);
+    }
+}
+
+// Work bin of each leaf for the persistent P2P schedule: klo = 63 - bit
+// length of (bodies x U-list bodies), so an ascending stable sort puts the
+// largest bins first and keeps tree order inside a bin.
+__global__ void __launch_bounds__(kT) k_p2p_work(TreeView t, uint64_t* __restrict__ khi, uint32_t* __restrict__ klo,
+                                                 int* __restrict__ val) {
+    const int li = blockIdx.x * kT + threadIdx.x;
     if (li >= t.nleaves) return;
     const int b = t.leaf_ids[li];
-    if (t.seg_count[b] > 32)
-        p2p_leaf<MIX, POT, 2, NEAR>(t, b, li, far, acc, fpot, phi, nr, npot, sl, sx, sy, sz, sm, sj, red);
-    else
-        p2p_leaf<MIX, POT, 1, NEAR>(t, b, li, far, acc, fpot, phi, nr, npot, sl, sx, sy, sz, sm, sj, red);
+    long long src = 0;
+    for (int e = t.u_off[li]; e < t.u_off[li + 1]; ++e) src += t.seg_count[t.u_idx[e]];
+    const unsigned long long w = (unsigned long long)t.seg_count[b] * (unsigned long long)src;
+    const int bits = w ? 64 - __builtin_clzll(w) : 0;
+    khi[li] = 0;
+    klo[li] = (uint32_t)(63 - bits);
+    val[li] = li;
 }
 
 // acc = G (far + near) and phi = -G (fpot + npot) per body in input order:
@@ -1161,18 +1221,41 @@ __global__ void __launch_bounds__(64) k_combine(TreeView t, const double* __rest
     }
 }
 
+// Persistent P2P workgroups: enough one-wave workgroups for every SIMD of the
+// 256 CUs to hold several.
+constexpr int kP2PPersistentGrid = 256 * 4 * 4;
+
+// order: the work-binned leaf order of the persistent schedule, or null.
 template <bool NEAR>
 void launch_p2p(hipStream_t s, const TreeView& t, bool mixed, const double* far, double* acc, const double* fpot,
-                double* phi, double* nr, double* npot) {
-    const dim3 pg(xcd_chunk_grid(t.nleaves)), pb(64);
+                double* phi, double* nr, double* npot, const Shapes& sh, const int* order) {
+    const dim3 pg(order ? std::max(1, std::min(t.nleaves, kP2PPersistentGrid)) : xcd_chunk_grid(t.nleaves)), pb(64);
+    const int pb2 = sh.p2p_pair_bodies, mg = sh.p2p_min_group;
     if (mixed && phi)
-        hipLaunchKernelGGL((k_p2p<true, true, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
+        hipLaunchKernelGGL((k_p2p<true, true, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot, pb2, mg, order);
     else if (mixed)
-        hipLaunchKernelGGL((k_p2p<true, false, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
+        hipLaunchKernelGGL((k_p2p<true, false, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot, pb2, mg, order);
     else if (phi)
-        hipLaunchKernelGGL((k_p2p<false, true, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
+        hipLaunchKernelGGL((k_p2p<false, true, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot, pb2, mg, order);
     else
-        hipLaunchKernelGGL((k_p2p<false, false, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot);
+        hipLaunchKernelGGL((k_p2p<false, false, NEAR>), pg, pb, 0, s, t, far, acc, fpot, phi, nr, npot, pb2, mg,
+                           order);
+}
+
+// The persistent schedule's leaf order on stream s: work bins, one radix pass.
+const int* p2p_order(Pool& pool, hipStream_t s, const TreeView& t, bool reserve_only) {
+    const int nl = t.nleaves;
+    SortBufs sb;
+    for (int k = 0; k < 2; ++k) {
+        sb.khi[k] = pool.get<uint64_t>(k ? "pw_khi1" : "pw_khi0", nl);
+        sb.klo[k] = pool.get<uint32_t>(k ? "pw_klo1" : "pw_klo0", nl);
+        sb.val[k] = pool.get<int>(k ? "pw_val1" : "pw_val0", nl);
+    }
+    sb.hist = pool.get<int>("pw_hist", 512 * (size_t)sort_blocks(nl));
+    sb.scan_tmp = pool.get<int>("pw_scan", scan_tmp_ints(512 * sort_blocks(nl) + nl) + 16);
+    if (reserve_only) return nullptr;
+    hipLaunchKernelGGL(k_p2p_work, dim3(blocks(nl)), dim3(kT), 0, s, t, sb.khi[0], sb.klo[0], sb.val[0]);
+    return sb.val[radix_sort(sb, nl, 0, 1, s)];
 }
 
 // M2L preparation: target ranks in the M2L order, the transposed source table
@@ -1192,12 +1275,13 @@ void m2l_prep(hipStream_t s, const TreeView& t, int np, int ntiles, const int* t
 // The M2L buffers of order P and precision prec, taken from the pool under
 // the names run<P> uses.
 template <int P>
-void m2l_prep_for(Pool& pool, hipStream_t s, const TreeView& t, int prec) {
+void m2l_prep_for(Pool& pool, hipStream_t s, const TreeView& t, int prec, const Shapes& sh) {
     const int nb = t.nb, np = nb;
-    const int TILE = prec == kPrecMixedBf16 ? kM2LBfTile : m2l_tile<P>();
-    if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
-    const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
-    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
+    (void)prec;
+    const int TILE = sh.m2l_waves * 16 * kM2LCb;
+    if (TILE != 256 && TILE != 192 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
+    const int* tstart = TILE == 256 ? t.m2l_tiles256 : TILE == 192 ? t.m2l_tiles192 : t.m2l_tiles128;
+    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : TILE == 192 ? t.m2l_ntiles192 : t.m2l_ntiles128;
     int* rank = pool.get<int>("m2l_rank", nb);
     int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
     unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
@@ -1225,8 +1309,9 @@ void upward(hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTab
 // stream capture allocates nothing.
 template <int P>
 void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTables& tab, int prec,
-         double* acc, double* phi, hipEvent_t* ev, bool reserve_only, const SideStream& side) {
-    const bool mixed = prec != kPrecFP64;
+         const Shapes& sh, double* acc, double* phi, hipEvent_t* ev, bool reserve_only, const SideStream& side) {
+    const bool mixed = p2p_fp32(prec);
+    const int form = m2l_form(prec);
     constexpr int S = nterms(P);
     const int nb = t.nb;
     double* M = pool.get<double>("Mexp", (size_t)nb * S);
@@ -1234,19 +1319,20 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     double* far = pool.get<double>("far", 3 * (size_t)t.n);
     double* fpot = phi ? pool.get<double>("farpot", t.n) : nullptr;
     // M2L targets in (octant, level) order, tiles cut inside octant groups
-    const int TILE = prec == kPrecMixedBf16 ? kM2LBfTile : m2l_tile<P>();
-    if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
-    const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
-    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
+    const int TILE = sh.m2l_waves * 16 * kM2LCb;
+    if (TILE != 256 && TILE != 192 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
+    const int* tstart = TILE == 256 ? t.m2l_tiles256 : TILE == 192 ? t.m2l_tiles192 : t.m2l_tiles128;
+    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : TILE == 192 ? t.m2l_ntiles192 : t.m2l_ntiles128;
     const int np = nb;
     int* rank = pool.get<int>("m2l_rank", nb);
     int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
     unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
-    uint4* Mbf = prec == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
+    uint4* Mbf = form == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
                                         : nullptr;
     const bool split = side.s != nullptr;
     double* nr = split ? pool.get<double>("near", 3 * (size_t)t.n) : nullptr;
     double* npot = split && phi ? pool.get<double>("nearpot", t.n) : nullptr;
+    if (sh.p2p_schedule) p2p_order(pool, s, t, true);  // its buffers too
     if (reserve_only) return;
     // P2P needs only the tree and the U lists: on the second stream it runs
     // beside the far-field passes. It is forked before M2L in FP64 (it shares
@@ -1256,30 +1342,42 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         AFMM_CHECK(hipEventRecord(side.fork, s));
         AFMM_CHECK(hipStreamWaitEvent(side.s, side.fork, 0));
         if (ev) AFMM_CHECK(hipEventRecord(ev[6], side.s));
-        launch_p2p<true>(side.s, t, mixed, far, acc, fpot, phi, nr, npot);
+        launch_p2p<true>(side.s, t, mixed, far, acc, fpot, phi, nr, npot, sh,
+                         sh.p2p_schedule ? p2p_order(pool, side.s, t, false) : nullptr);
         if (ev) AFMM_CHECK(hipEventRecord(ev[7], side.s));
         AFMM_CHECK(hipEventRecord(side.join, side.s));
     };
-    const bool p2p_after_m2l = prec == kPrecMixedBf16;
+    const bool p2p_after_m2l = form == kPrecMixedBf16;
     if (split && !p2p_after_m2l) fork_p2p();
     if (!side.upward_launched) upward<P>(s, t, meta, tab, M, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     {
         if (!side.prep_done) m2l_prep(s, t, np, ntiles, tstart, rank, srcT, present);
         if (side.upward_launched && side.up_done) AFMM_CHECK(hipStreamWaitEvent(s, side.up_done, 0));  // M done
-        if (prec == kPrecMixedBf16) {
+        if (form == kPrecMixedBf16) {
             constexpr int KB = m2l_bf_kb(S);
             hipLaunchKernelGGL(k_split_bf16<P>, dim3(blocks((long long)nb * KB * 4)), dim3(kT), 0, s, nb, M, Mbf);
         }
         if (ev) AFMM_CHECK(hipEventRecord(ev[5], s));
-        if (prec == kPrecMixedBf16)
-            hipLaunchKernelGGL(k_m2l_bf16<P>, dim3(xcd_grid(ntiles)), dim3(kM2LBfWaves * 64), 0, s, np, ntiles, tstart,
-                               t.m2l_order, srcT, present, tab.m2l_bf, Mbf, L);
-        else if (prec == kPrecMixed)
-            hipLaunchKernelGGL((k_m2l_mfma<P, true>), dim3(xcd_grid(ntiles)), dim3(m2l_waves<P>() * 64), 0, s, np, ntiles,
-                               tstart, t.m2l_order, srcT, present, tab.m2l_hl, M, L);
-        else
-            hipLaunchKernelGGL((k_m2l_mfma<P, false>), dim3(xcd_grid(ntiles)), dim3(m2l_waves<P>() * 64), 0, s, np, ntiles,
-                               tstart, t.m2l_order, srcT, present, tab.m2l_sw, M, L);
+        auto launch_m2l = [&](auto wtag) {
+            constexpr int W = decltype(wtag)::value;
+            const dim3 g(xcd_grid(ntiles)), b(W * 64);
+            if (form == kPrecMixedBf16)
+                hipLaunchKernelGGL((k_m2l_bf16<P, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT, present,
+                                   tab.m2l_bf, Mbf, L);
+            else if constexpr (W <= 12 || P <= 8) {
+                if (form == kPrecMixed)
+                    hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                                       present, tab.m2l_hl, M, L);
+                else
+                    hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                                       present, tab.m2l_sw, M, L);
+            } else {
+                throw std::runtime_error("16-wave M2L workgroups need p <= 8 or the bf16 M2L");
+            }
+        };
+        if (sh.m2l_waves == 16) launch_m2l(std::integral_constant<int, 16>{});
+        else if (sh.m2l_waves == 12) launch_m2l(std::integral_constant<int, 12>{});
+        else launch_m2l(std::integral_constant<int, 8>{});
     }
     if (split && p2p_after_m2l) fork_p2p();
     if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
@@ -1305,7 +1403,8 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         else hipLaunchKernelGGL(k_combine<false>, cg, cb, 0, s, t, far, nr, fpot, npot, acc, phi);
     } else {
         if (ev) AFMM_CHECK(hipEventRecord(ev[6], s));
-        launch_p2p<false>(s, t, mixed, far, acc, fpot, phi, nullptr, nullptr);
+        launch_p2p<false>(s, t, mixed, far, acc, fpot, phi, nullptr, nullptr, sh,
+                          sh.p2p_schedule ? p2p_order(pool, s, t, false) : nullptr);
         if (ev) AFMM_CHECK(hipEventRecord(ev[7], s));
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[4], s));
@@ -1327,10 +1426,10 @@ void run_upward(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& m
     throw std::runtime_error("unsupported expansion order");
 }
 
-void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec) {
+void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec, const Shapes& sh) {
     switch (p) {
 #define AFMM_CASE(PP) \
-    case PP: m2l_prep_for<PP>(pool, s, tv, prec); return;
+    case PP: m2l_prep_for<PP>(pool, s, tv, prec, sh); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
 #undef AFMM_CASE
@@ -1339,12 +1438,13 @@ void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec
 }
 
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
-             int prec, double* d_acc, double* d_phi, hipEvent_t* ev, bool reserve_only, SideStream side) {
-    if (prec == kPrecMixed && !tab.m2l_hl) throw std::runtime_error("split FP32 M2L operators not loaded");
-    if (prec == kPrecMixedBf16 && !tab.m2l_bf) throw std::runtime_error("bf16 M2L operators not loaded");
+             int prec, const Shapes& sh, double* d_acc, double* d_phi, hipEvent_t* ev, bool reserve_only,
+             SideStream side) {
+    if (m2l_form(prec) == kPrecMixed && !tab.m2l_hl) throw std::runtime_error("split FP32 M2L operators not loaded");
+    if (m2l_form(prec) == kPrecMixedBf16 && !tab.m2l_bf) throw std::runtime_error("bf16 M2L operators not loaded");
     switch (p) {
 #define AFMM_CASE(PP) \
-    case PP: run<PP>(pool, s, tv, meta, tab, prec, d_acc, d_phi, ev, reserve_only, side); return;
+    case PP: run<PP>(pool, s, tv, meta, tab, prec, sh, d_acc, d_phi, ev, reserve_only, side); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
 #undef AFMM_CASE

@@ -334,8 +334,7 @@ int afmm_server_create(const afmm_server_params* prm, afmm_server** out) {
     if (!prm || !out) return AFMM_ERR_ARG;
     *out = nullptr;
     if (prm->leaf_cap < 1) return AFMM_ERR_LEAF_CAP;
-    if (prm->precision != AFMM_PRECISION_FP64 && prm->precision != AFMM_PRECISION_MIXED &&
-        prm->precision != AFMM_PRECISION_MIXED_BF16)
+    if (prm->precision < AFMM_PRECISION_FP64 || prm->precision > AFMM_PRECISION_P2P_FP32)
         return AFMM_ERR_PRECISION;
     afmm_server* sv = new (std::nothrow) afmm_server();
     if (!sv) return AFMM_ERR_NO_MEMORY;

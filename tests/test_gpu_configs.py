@@ -162,7 +162,6 @@ def test_merger_mixed_vs_fp64(afmm, ics, n_per, orders):
 def test_batch_256_systems(afmm, ics):
     import importlib.util
     import os
-    import torch
     spec = importlib.util.spec_from_file_location("tune_cfg", os.path.join(cabi.ROOT, "on-galaxy_amd", "tune.py"))
     tune = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(tune)
@@ -172,12 +171,11 @@ def test_batch_256_systems(afmm, ics):
     pos = np.concatenate([pq for pq, _ in sys_])
     mass = np.concatenate([mq for _, mq in sys_])
     print(f"batch: {len(sys_)} systems, {off[-1]} bodies", flush=True)
-    dev = torch.device("cuda:0")
-    d_pos = torch.from_numpy(pos).to(dev)
-    d_mass = torch.from_numpy(mass).to(dev)
-    d_acc = torch.empty_like(d_pos)
+    d_pos = afmm.DeviceArray(None, pos)
+    d_mass = afmm.DeviceArray(None, mass)
+    d_acc = afmm.DeviceArray(pos.shape)
     prec = afmm.PRECISION_MIXED_BF16
-    cap, times = tune.tune_leaf_cap(afmm, off, d_pos.data_ptr(), d_mass.data_ptr(), d_acc.data_ptr(), 8, prec,
+    cap, times = tune.tune_leaf_cap(afmm, off, d_pos.ptr, d_mass.ptr, d_acc.ptr, 8, prec,
                                     eps=0.01)
     print(f"autotuned leaf capacity {cap}: {times}", flush=True)
     s = afmm.AdaptiveFMM3D(G=1.0, eps=0.01, p=8, threshold=cap, precision=prec)

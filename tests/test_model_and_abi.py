@@ -69,6 +69,11 @@ def test_context_switches_reject_null_without_device():
         f.argtypes = [C.c_void_p, C.c_int]
         f.restype = C.c_int
         assert f(None, 1) == -1 and f(None, 0) == -1, name
+    for name in ("afmm_set_shapes", "afmm_get_shapes"):
+        f = getattr(lib, name)
+        f.argtypes = [C.c_void_p, C.c_void_p]
+        f.restype = C.c_int
+        assert f(None, None) == -1, name
 
 
 def test_system_offsets_rejected_before_device_work():
