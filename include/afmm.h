@@ -253,6 +253,21 @@ void afmm_server_destroy(afmm_server* sv);
 int afmm_direct(afmm_ctx* ctx, int64_t n, const double* pos_xyz, const double* mass, int64_t ntgt,
                 const int64_t* tgt, double* acc_xyz, double* phi);
 
+/* Allocation-free afmm_direct on device arrays. It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops). Same sums, same bits as afmm_direct
+ * for the same bodies and targets: d_tgt holds ntgt int64 body indices on the
+ * device (an index outside [0, n) gives NaN for that target; nothing outside
+ * the arrays is read). The only scratch, the per-split partial sums, is a
+ * static device array of 10 MiB that each GPU the library runs on holds until
+ * the process ends, so ntgt <= AFMM_DIRECT_NOALLOC_MAX_TARGETS
+ * (AFMM_ERR_TOO_LARGE beyond). Work is enqueued on `stream` (a hipStream_t;
+ * NULL: the context's stream) and not waited for. Every call on a GPU shares
+ * that GPU's scratch: calls that overlap on one GPU race. */
+#define AFMM_DIRECT_NOALLOC_MAX_TARGETS 65536
+int afmm_direct_device_noalloc(afmm_ctx* ctx, int64_t n, const double* d_pos_xyz, const double* d_mass, int64_t ntgt,
+                               const int64_t* d_tgt, double* d_acc_xyz, double* d_phi, void* stream);
+
 /* afmm_eval / afmm_eval_device plus the FMM potential per body (input
  * order) when phi is non-null: phi[i] = -G (far-field expansions at x_i +
  * sum_{j in U, j != i} m_j / sqrt(|d|^2 + eps^2)), the approximation of

@@ -142,6 +142,8 @@ def library():
         L.afmm_set_overlap.argtypes = [C.c_void_p, C.c_int]
         L.afmm_set_shapes.argtypes = [C.c_void_p, C.POINTER(Shapes)]
         L.afmm_get_shapes.argtypes = [C.c_void_p, C.POINTER(Shapes)]
+        L.afmm_direct_device_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.afmm_tree_export.argtypes = [C.c_void_p, C.POINTER(Tree)]
         L.afmm_make_plummer.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint32, _dp, _dp, _dp]
         _lib = L

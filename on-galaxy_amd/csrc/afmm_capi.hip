@@ -251,6 +251,7 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
     c->prm = *prm;
     const int r = guarded(c, [&]() -> int {
         AFMM_CHECK(hipSetDevice(prm->device));
+        direct_scratch_setup(prm->device);  // once per device: the static scratch's address
         c->shapes.m2l_waves = default_m2l_waves(prm->p, prm->precision);
         AFMM_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto& e : c->ev) AFMM_CHECK(hipEventCreate(&e));
@@ -428,6 +429,23 @@ int afmm_direct(afmm_ctx* c, int64_t n, const double* pos, const double* mass, i
         AFMM_CHECK(hipMemcpyAsync(acc, dacc, sizeof(double) * 3 * ntgt, hipMemcpyDeviceToHost, s));
         if (phi) AFMM_CHECK(hipMemcpyAsync(phi, dphi, sizeof(double) * ntgt, hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipStreamSynchronize(s));
+        return AFMM_OK;
+    });
+}
+
+int afmm_direct_device_noalloc(afmm_ctx* c, int64_t n, const double* d_pos, const double* d_mass, int64_t ntgt,
+                               const int64_t* d_tgt, double* d_acc, double* d_phi, void* stream) {
+    if (!c || n < 0 || ntgt < 0) return AFMM_ERR_ARG;
+    if (ntgt == 0) return AFMM_OK;
+    if (!d_pos || !d_mass || !d_tgt || !d_acc) return AFMM_ERR_ARG;
+    if (n >= (1LL << 30)) return fail(c, AFMM_ERR_TOO_LARGE, "n beyond 2^30");
+    if (ntgt > AFMM_DIRECT_NOALLOC_MAX_TARGETS)
+        return fail(c, AFMM_ERR_TOO_LARGE, "ntgt beyond AFMM_DIRECT_NOALLOC_MAX_TARGETS (static scratch)");
+    return guarded(c, [&]() -> int {
+        AFMM_CHECK(hipSetDevice(c->prm.device));
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+        direct_targets_noalloc(c->prm.device, s, d_pos, d_mass, (int)n, d_tgt, (int)ntgt, c->prm.G, c->prm.eps, d_acc,
+                               d_phi);
         return AFMM_OK;
     });
 }

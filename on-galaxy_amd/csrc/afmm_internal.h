@@ -300,5 +300,14 @@ private:
 // acc (AoS ntgt x 3) and, if d_phi is non-null, the softened potential.
 void direct_targets(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int n, const int* d_tgt,
                     int ntgt, double G, double eps, double* d_acc, double* d_phi);
+// The same sum with its partial sums in a static device array instead of the
+// pool: no allocation. Targets are device int64 indices (outside [0, n) give
+// NaN); at most kNoAllocMaxTargets of them. direct_scratch_setup(device) must
+// have run once with that device current (afmm_create does it).
+constexpr int kMaxDevices = 64;
+constexpr int kNoAllocMaxTargets = AFMM_DIRECT_NOALLOC_MAX_TARGETS;
+void direct_scratch_setup(int device);
+void direct_targets_noalloc(int device, hipStream_t s, const double* d_pos, const double* d_mass, int n,
+                            const int64_t* d_tgt, int ntgt, double G, double eps, double* d_acc, double* d_phi);
 
 }  // namespace afmm
