@@ -563,8 +563,8 @@ def main():
     # as the headline) and accuracy against FP64 direct at eps = 0
     orders = {}
     if not args.no_orders:
-        progress("orders 4 and 12")
-        for po in (4, 12):
+        progress("orders 4, 12, 14 and 16")
+        for po in (4, 12, 14, 16):
             if po == args.p:
                 continue
             row = {}

@@ -33,7 +33,7 @@ typedef enum {
     AFMM_ERR_PRECISION = -9   /* precision mode not supported */
 } afmm_status;
 
-#define AFMM_MAX_ORDER 12
+#define AFMM_MAX_ORDER 16
 /* FP64: every phase in FP64 (the reference CPU path's arithmetic).
  * MIXED: M2L as split-FP32 MFMA products (each operand carried as an FP32
  * hi/lo pair, three products per block) and P2P in FP32 with FP64
@@ -287,9 +287,11 @@ int afmm_synchronize(afmm_ctx* ctx);
 /* Graph mode (no counterpart in the reference, which launches from the host
  * per pass): when enabled, each evaluation captures its FMM passes (M2L
  * through P2P, or P2M through P2P with afmm_set_overlap off; after the tree
- * build, which needs host round trips) into a HIP graph and launches that; an evaluation whose passes have the same shape as
- * the previous one updates the executable graph in place. Results are the
- * same as without it. */
+ * build, which needs host round trips) into a HIP graph and launches that.
+ * With the overlap on, the P2P near field is a branch of the graph on the
+ * context's own second stream, beside M2L and the passes after it. An
+ * evaluation whose passes have the same shape as the previous one updates the
+ * executable graph in place. Results are the same as without it. */
 int afmm_set_graph(afmm_ctx* ctx, int enable);
 
 /* Kernel shapes (no counterpart in the reference, whose CUDA twin runs one
@@ -299,8 +301,9 @@ int afmm_set_graph(afmm_ctx* ctx, int enable);
  * schedule also leave the results' bits unchanged, while the P2P lane grouping
  * changes the order of each near-field sum (same accuracy, different last bits). */
 typedef struct {
-    int32_t m2l_waves;        /* waves per M2L workgroup, 16 targets each: 8, 12 or 16 (default 16
-                                 at p <= 8 or with bf16 M2L, else 8; 16 needs one of those) */
+    int32_t m2l_waves;        /* waves per M2L workgroup, 16 targets each: 8 or 16 (default 16 at
+                                 p <= 8, or p <= 7 with the bf16 M2L, else 8; 16 needs p <= 8 or
+                                 the bf16 M2L) */
     int32_t p2p_pair_bodies;  /* leaves with more bodies give each P2P lane two targets (default 32) */
     int32_t p2p_min_group;    /* fewest lanes sharing one target's sources: 1, 2, 4, ..., 64 (default 8) */
     int32_t p2p_schedule;     /* 1: persistent workgroups over the leaves sorted by work bin, largest
@@ -312,10 +315,10 @@ int afmm_set_shapes(afmm_ctx* ctx, const afmm_shapes* shapes);
 int afmm_get_shapes(const afmm_ctx* ctx, afmm_shapes* out);
 
 /* Two-stream overlap (no counterpart in the reference, which has one host
- * thread and one stream): when enabled (the default), an evaluation outside
- * graph mode runs the upward pass beside the interaction-list build and the
- * P2P near field beside the far-field passes on a second stream shared by the
- * process's contexts on that device. Disabled, everything runs in order on the
+ * thread and one stream): when enabled (the default), an evaluation runs the
+ * upward pass beside the interaction-list build and the P2P near field beside
+ * the far-field passes on a second stream shared by the process's contexts on
+ * that device (in graph mode the captured P2P branch uses the context's own). Disabled, everything runs in order on the
  * context's stream (afmm_stats.ms_p2p then times P2P alone). Results are the
  * same bits either way. */
 int afmm_set_overlap(afmm_ctx* ctx, int enable);

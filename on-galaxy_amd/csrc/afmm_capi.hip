@@ -116,11 +116,20 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     if (c->use_graph) {
         // the upward pass and the M2L preparation already ran outside the graph
         // (beside and within the list build); the stream waits for them before
-        // the capture, so the graph holds no cross-stream dependency
+        // the capture
         if (up_launched) AFMM_CHECK(hipStreamWaitEvent(s, c->up_done, 0));
         SideStream gside{};
         gside.upward_launched = up_launched;
         gside.prep_done = prep_done;
+        // P2P on a second stream inside the graph too (a branch of the graph
+        // beside the far-field passes); the capture takes the context's own
+        // side stream, so no other context's work can join it
+        if (split) {
+            if (!c->graph_side) AFMM_CHECK(hipStreamCreateWithFlags(&c->graph_side, hipStreamNonBlocking));
+            gside.s = c->graph_side;
+            gside.fork = c->side_fork;
+            gside.join = c->side_join;
+        }
         // size every buffer first, so that the capture allocates nothing
         run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, c->shapes, acc, phi, nullptr, true, gside);
         hipGraph_t g = nullptr;
@@ -227,7 +236,7 @@ const char* afmm_strerror(int code) {
     switch (code) {
         case AFMM_OK: return "ok";
         case AFMM_ERR_ARG: return "invalid argument";
-        case AFMM_ERR_ORDER: return "expansion order outside [1, 12]";
+        case AFMM_ERR_ORDER: return "expansion order outside [1, 16]";
         case AFMM_ERR_LEAF_CAP: return "leaf capacity must be >= 1";
         case AFMM_ERR_NONFINITE: return "position or mass is NaN or Inf";
         case AFMM_ERR_DEVICE: return "HIP runtime error";
@@ -258,7 +267,13 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
         const Tables& T = tables_for_order(prm->p);
         const std::vector<double> m2m = mfma_a_layout(T.m2m, T.S, 8);
         const std::vector<double> l2l = mfma_a_layout(T.l2l, T.S, 8);
-        const std::vector<double> m2l = m2l_a_layout(T.m2l, T.S, kNumOffsets);
+        // M2L operators in slot order (the order the M2L kernels step through them)
+        std::vector<double> m2l_slots(T.m2l.size());
+        const size_t SS = (size_t)T.S * T.S;
+        for (int k = 0; k < kNumOffsets; ++k)
+            std::copy(T.m2l.begin() + kM2LSlotsHost.offset_of[k] * SS,
+                      T.m2l.begin() + (kM2LSlotsHost.offset_of[k] + 1) * SS, m2l_slots.begin() + k * SS);
+        const std::vector<double> m2l = m2l_a_layout(m2l_slots, T.S, kNumOffsets);
         c->d_m2m_sw = c->pool.get<double>("T_m2m_sw", m2m.size());
         c->d_l2l_sw = c->pool.get<double>("T_l2l_sw", l2l.size());
         c->d_m2l_sw = c->pool.get<double>("T_m2l_sw", m2l.size());
@@ -275,7 +290,7 @@ int afmm_create(const afmm_params* prm, afmm_ctx** out) {
             AFMM_CHECK(hipMemcpy(c->d_m2l_hl, hl.data(), sizeof(float2) * hl.size(), hipMemcpyHostToDevice));
         }
         if (m2l_form(prm->precision) == kPrecMixedBf16) {
-            const std::vector<uint16_t> bf = mfma_bf16_layout(T.m2l, T.S, kNumOffsets);
+            const std::vector<uint16_t> bf = mfma_bf16_layout(m2l_slots, T.S, kNumOffsets);
             c->d_m2l_bf = c->pool.get<uint4>("T_m2l_bf", bf.size() / 8);
             AFMM_CHECK(hipMemcpy(c->d_m2l_bf, bf.data(), sizeof(uint16_t) * bf.size(), hipMemcpyHostToDevice));
         }
@@ -311,6 +326,7 @@ void afmm_destroy(afmm_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {c->side_fork, c->side_join, c->up_fork, c->up_done})
         if (e) (void)hipEventDestroy(e);
+    if (c->graph_side) (void)hipStreamDestroy(c->graph_side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -457,8 +473,8 @@ int afmm_set_shapes(afmm_ctx* c, const afmm_shapes* sh) {
     if (sh) {
         if (sh->m2l_waves) {
             const bool w16 = c->prm.p <= 8 || m2l_form(c->prm.precision) == kPrecMixedBf16;
-            if (sh->m2l_waves != 8 && sh->m2l_waves != 12 && !(sh->m2l_waves == 16 && w16))
-                return fail(c, AFMM_ERR_ARG, "m2l_waves must be 8 or 12, or 16 at p <= 8 or with bf16 M2L");
+            if (sh->m2l_waves != 8 && !(sh->m2l_waves == 16 && w16))
+                return fail(c, AFMM_ERR_ARG, "m2l_waves must be 8, or 16 at p <= 8 or with bf16 M2L");
             v.m2l_waves = sh->m2l_waves;
         }
         if (sh->p2p_pair_bodies < 0) return fail(c, AFMM_ERR_ARG, "p2p_pair_bodies must be >= 0");

@@ -40,6 +40,7 @@ struct afmm_ctx {
     // graph of the last evaluation, updated in place when the next capture has
     // the same shape, re-instantiated otherwise
     bool use_graph = false;
+    hipStream_t graph_side = nullptr;  // the captured P2P branch's stream (graph mode with the overlap on)
     hipGraphExec_t gexec = nullptr;
     int64_t graph_instantiations = 0, graph_updates = 0;
 };

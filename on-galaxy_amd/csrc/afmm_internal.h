@@ -121,9 +121,9 @@ struct TreeView {
     int* lo_order;       // [B] boxes sorted by (level, octant); level l is [level_off[l], level_off[l+1])
     int* m2l_order;      // [B] boxes sorted by (octant, level): the M2L target order
     // M2L target tiles over m2l_order, cut inside octant groups: tile k is
-    // ranks [m2l_tiles{256,192,128}[k], [k+1]), at most 256 (192, 128) targets each
-    int *m2l_tiles256, *m2l_tiles192, *m2l_tiles128;
-    int m2l_ntiles256 = 0, m2l_ntiles192 = 0, m2l_ntiles128 = 0;
+    // ranks [m2l_tiles{256,128}[k], [k+1]), at most 256 (128) targets each
+    int *m2l_tiles256, *m2l_tiles128;
+    int m2l_ntiles256 = 0, m2l_ntiles128 = 0;
     int* nonleaf_ids;    // [B - nleaves] non-leaf boxes in BFS order
     // lists
     int *coll, *ncoll;   // [27B], [B]
@@ -235,15 +235,14 @@ struct SideStream {
 // in XCD chunks in tree order; 1: persistent workgroups over the leaves sorted
 // by work bin, largest first).
 struct Shapes {
-    int m2l_waves = 16;  // 8, 12 or 16
+    int m2l_waves = 16;  // 8 or 16
     int p2p_pair_bodies = 32;
     int p2p_min_group = 8;
     int p2p_schedule = 0;
 };
 // The default M2L workgroup of order p and precision mode prec: 16 waves
 // while the accumulators fit four waves per SIMD, else 8 (the bf16 form keeps
-// an FP64 total beside its FP32 accumulator, which needs 8 from p = 8; 12
-// waves are the autotuner's to choose).
+// an FP64 total beside its FP32 accumulator, which needs 8 from p = 8).
 inline int default_m2l_waves(int p, int prec) {
     return m2l_form(prec) == kPrecMixedBf16 ? (p <= 7 ? 16 : 8) : (p <= 8 ? 16 : 8);
 }

@@ -32,7 +32,7 @@
 
 namespace afmm {
 
-constexpr int kMaxOrder = 12;
+constexpr int kMaxOrder = 16;
 
 // Real-storage offset of Re a_n^a (Im a_n^a follows it for a > 0).
 AFMM_HD constexpr int rid(int n, int a) { return n * n + (a == 0 ? 0 : 2 * a - 1); }
@@ -262,6 +262,47 @@ AFMM_HD constexpr int offset_index(int dx, int dy, int dz) {
     return (dx + 3) * 49 + (dy + 3) * 7 + (dz + 3);
 }
 constexpr int kNumOffsets = 343;
+
+// The order in which an M2L workgroup steps through the offsets ("slots"). A
+// tile's targets are boxes of one octant at one level, so they lie on a
+// lattice of spacing 2, and their sources at offset o lie on that lattice
+// shifted by o: offsets of one parity class (o mod 2) reach the same source
+// lattice, offsets of different classes disjoint ones. Slots run class by
+// class, and inside a class along a boustrophedon path in which consecutive
+// offsets differ by 2 in one component, so consecutive offsets' sources are
+// mostly the same boxes shifted by one lattice step (kept in the L2 between
+// them) instead of a disjoint set at every step.
+struct M2LSlots {
+    int offset_of[kNumOffsets];  // slot -> offset_index
+    int slot_of[kNumOffsets];    // offset_index -> slot
+};
+constexpr M2LSlots make_m2l_slots() {
+    M2LSlots t{};
+    int k = 0;
+    for (int cls = 0; cls < 8; ++cls) {
+        const int par[3] = {cls >> 2, (cls >> 1) & 1, cls & 1};
+        int vals[3][4] = {}, nv[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a)
+            for (int d = -3; d <= 3; ++d)
+                if (((d + 4) & 1) == par[a]) vals[a][nv[a]++] = d;
+        int line = 0;
+        for (int ix = 0; ix < nv[0]; ++ix) {
+            for (int j = 0; j < nv[1]; ++j) {
+                const int iy = (ix & 1) ? nv[1] - 1 - j : j;
+                for (int l = 0; l < nv[2]; ++l) {
+                    const int iz = (line & 1) ? nv[2] - 1 - l : l;
+                    const int o = offset_index(vals[0][ix], vals[1][iy], vals[2][iz]);
+                    t.offset_of[k] = o;
+                    t.slot_of[o] = k;
+                    ++k;
+                }
+                ++line;
+            }
+        }
+    }
+    return t;
+}
+constexpr M2LSlots kM2LSlotsHost = make_m2l_slots();
 
 // M2L operator tiling for v_mfma_f64_16x16x4_f64: RB row blocks of 16 and the
 // k steps of 4 split into NKC chunks of KC steps, one chunk of all row blocks

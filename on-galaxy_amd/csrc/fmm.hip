@@ -81,7 +81,9 @@ __device__ __forceinline__ void wave_halving_sum(double* v, int lane, int& first
 //   KIND 0: targets = leaves, sources = the leaf's own bodies, Mhat = result.
 //   KIND 1: targets = boxes, sources = bodies of X(box) leaves, Lhat += result.
 template <int S>
-constexpr int coef_window() { return S <= 96 ? S : (S + 1) / 2; }
+constexpr int coef_windows() { return S <= 96 ? 1 : ((S + 127) / 128 > 2 ? (S + 127) / 128 : 2); }
+template <int S>
+constexpr int coef_window() { return (S + coef_windows<S>() - 1) / coef_windows<S>(); }
 
 // Bodies of a list of boxes staged through LDS in chunks of kStreamChunk list
 // entries: their (first body, count) and the running body offset, so that
@@ -224,7 +226,11 @@ __global__ void __launch_bounds__(64) k_accumulate(TreeView t, double* __restric
     const int f0 = KIND == 0 ? t.seg_first[b] : 0, n0 = KIND == 0 ? t.seg_count[b] : 0;
     accumulate_window<P, KIND, 0, CW>(t, list, nlist, f0, n0, ch, ih, lane, sl, out, (size_t)b * S);
     if constexpr (CW < S)
-        accumulate_window<P, KIND, CW, S - CW>(t, list, nlist, f0, n0, ch, ih, lane, sl, out, (size_t)b * S);
+        accumulate_window<P, KIND, CW, (2 * CW < S ? CW : S - CW)>(t, list, nlist, f0, n0, ch, ih, lane, sl, out,
+                                                                   (size_t)b * S);
+    if constexpr (2 * CW < S)
+        accumulate_window<P, KIND, 2 * CW, S - 2 * CW>(t, list, nlist, f0, n0, ch, ih, lane, sl, out, (size_t)b * S);
+    static_assert(3 * CW >= S, "three coefficient windows cover every order built");
 }
 
 // ---- M2P over W (AdaptiveFMM3D.hpp:95, FmmExpansions3D.hpp:180-200) --------------
@@ -342,6 +348,10 @@ __global__ void __launch_bounds__(kT) k_fill_int(int* a, size_t n, int v) {
 
 // srcT[offset * np + rank(t)] = source box of target t at that offset; one
 // thread per (box, V slot), so the slot reads are coalesced.
+// The offset slots of afmm_math.h for device code (srcT rows, presence and
+// operators are all kept in slot order).
+__constant__ M2LSlots kM2LSlots = make_m2l_slots();
+
 __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __restrict__ rank, const int* __restrict__ vl,
                                                const int* __restrict__ nv, const int* __restrict__ ic,
                                                int* __restrict__ srcT) {
@@ -352,7 +362,7 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
     const int c = vl[g];
     const int o = offset_index(ic[3 * (size_t)b] - ic[3 * (size_t)c], ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1],
                                ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2]);
-    srcT[(size_t)o * np + rank[b]] = c;
+    srcT[(size_t)kM2LSlots.slot_of[o] * np + rank[b]] = c;
 }
 
 // Waves per M2L workgroup (W, a kernel shape the autotuner may choose): by
@@ -362,12 +372,13 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
 // owns one 16-target column block, so a tile holds 16 W targets.
 constexpr int kM2LCb = 1;  // 16-target column blocks per wave
 
-// present[wg][o] = 1 when some target of tile wg (ranks tstart[wg] ..
-// tstart[wg+1]) has a source at offset o (one wave per (tile, offset)).
+// present[wg][k] = 1 when some target of tile wg (ranks tstart[wg] ..
+// tstart[wg+1]) has a source at the offset of slot k (one wave per (tile, slot)).
 __global__ void __launch_bounds__(64) k_m2l_plan(const int* __restrict__ tstart, int np,
                                                 const int* __restrict__ srcT, unsigned char* __restrict__ present) {
-    const int wg = blockIdx.x / kNumOffsets, o = blockIdx.x % kNumOffsets;
-    const int dx = o / 49 - 3, dy = (o / 7) % 7 - 3, dz = o % 7 - 3;
+    const int wg = blockIdx.x / kNumOffsets, o = blockIdx.x % kNumOffsets;  // o: a slot
+    const int off = kM2LSlots.offset_of[o];
+    const int dx = off / 49 - 3, dy = (off / 7) % 7 - 3, dz = off % 7 - 3;
     const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
     bool any = false;
     if (!near)
@@ -1279,9 +1290,9 @@ void m2l_prep_for(Pool& pool, hipStream_t s, const TreeView& t, int prec, const 
     const int nb = t.nb, np = nb;
     (void)prec;
     const int TILE = sh.m2l_waves * 16 * kM2LCb;
-    if (TILE != 256 && TILE != 192 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
-    const int* tstart = TILE == 256 ? t.m2l_tiles256 : TILE == 192 ? t.m2l_tiles192 : t.m2l_tiles128;
-    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : TILE == 192 ? t.m2l_ntiles192 : t.m2l_ntiles128;
+    if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
+    const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
+    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
     int* rank = pool.get<int>("m2l_rank", nb);
     int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
     unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
@@ -1320,9 +1331,9 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     double* fpot = phi ? pool.get<double>("farpot", t.n) : nullptr;
     // M2L targets in (octant, level) order, tiles cut inside octant groups
     const int TILE = sh.m2l_waves * 16 * kM2LCb;
-    if (TILE != 256 && TILE != 192 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
-    const int* tstart = TILE == 256 ? t.m2l_tiles256 : TILE == 192 ? t.m2l_tiles192 : t.m2l_tiles128;
-    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : TILE == 192 ? t.m2l_ntiles192 : t.m2l_ntiles128;
+    if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
+    const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
+    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
     const int np = nb;
     int* rank = pool.get<int>("m2l_rank", nb);
     int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
@@ -1364,7 +1375,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
             if (form == kPrecMixedBf16)
                 hipLaunchKernelGGL((k_m2l_bf16<P, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT, present,
                                    tab.m2l_bf, Mbf, L);
-            else if constexpr (W <= 12 || P <= 8) {
+            else if constexpr (W == 8 || P <= 8) {
                 if (form == kPrecMixed)
                     hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
                                        present, tab.m2l_hl, M, L);
@@ -1376,7 +1387,6 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
             }
         };
         if (sh.m2l_waves == 16) launch_m2l(std::integral_constant<int, 16>{});
-        else if (sh.m2l_waves == 12) launch_m2l(std::integral_constant<int, 12>{});
         else launch_m2l(std::integral_constant<int, 8>{});
     }
     if (split && p2p_after_m2l) fork_p2p();
@@ -1421,6 +1431,7 @@ void run_upward(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& m
     case PP: upward<PP>(s, tv, meta, tab, M, e0, e1); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+        AFMM_CASE(13) AFMM_CASE(14) AFMM_CASE(15) AFMM_CASE(16)
 #undef AFMM_CASE
     }
     throw std::runtime_error("unsupported expansion order");
@@ -1432,6 +1443,7 @@ void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec
     case PP: m2l_prep_for<PP>(pool, s, tv, prec, sh); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+        AFMM_CASE(13) AFMM_CASE(14) AFMM_CASE(15) AFMM_CASE(16)
 #undef AFMM_CASE
     }
     throw std::runtime_error("unsupported expansion order");
@@ -1447,6 +1459,7 @@ void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta
     case PP: run<PP>(pool, s, tv, meta, tab, prec, sh, d_acc, d_phi, ev, reserve_only, side); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+        AFMM_CASE(13) AFMM_CASE(14) AFMM_CASE(15) AFMM_CASE(16)
 #undef AFMM_CASE
     }
     throw std::runtime_error("unsupported expansion order");

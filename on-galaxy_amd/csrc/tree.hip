@@ -1354,23 +1354,19 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
             meta.nonleaf_off[l] = bounds[l] - (bounds[l] < nb ? got[l] : tv.nleaves);
         // M2L tiles (tile starts built on the device from the same counts)
         int tot = 0;
-        tv.m2l_ntiles256 = tv.m2l_ntiles192 = tv.m2l_ntiles128 = 0;
+        tv.m2l_ntiles256 = tv.m2l_ntiles128 = 0;
         for (int q = 0; q < 8; ++q) {
             const int c = got[nlev + q];
             if (c <= 0) continue;
             tot += c;
             tv.m2l_ntiles256 += m2l_tiles_of(c, 256);
-            tv.m2l_ntiles192 += m2l_tiles_of(c, 192);
             tv.m2l_ntiles128 += m2l_tiles_of(c, 128);
         }
         if (tot != nb) throw std::runtime_error("octant counts do not cover the boxes");
         tv.m2l_tiles256 = pool.get<int>("m2l_tiles256", tv.m2l_ntiles256 + 1);
-        tv.m2l_tiles192 = pool.get<int>("m2l_tiles192", tv.m2l_ntiles192 + 1);
         tv.m2l_tiles128 = pool.get<int>("m2l_tiles128", tv.m2l_ntiles128 + 1);
         hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_out + nlev, 256, tv.m2l_ntiles256,
                            tv.m2l_tiles256);
-        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_out + nlev, 192, tv.m2l_ntiles192,
-                           tv.m2l_tiles192);
         hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_out + nlev, 128, tv.m2l_ntiles128,
                            tv.m2l_tiles128);
     }

@@ -19,7 +19,7 @@ import math
 
 CAPS = (16, 32, 48, 64, 96, 128, 192, 256)
 # candidate values per shape (0 in a field keeps the library default)
-SHAPES = (("m2l_waves", (16, 12, 8)), ("p2p_schedule", (2, 1)), ("p2p_pair_bodies", (32, 16, 48)),
+SHAPES = (("m2l_waves", (16, 8)), ("p2p_schedule", (2, 1)), ("p2p_pair_bodies", (32, 16, 48)),
           ("p2p_min_group", (8, 4, 16)))
 _cache = {}
 

@@ -128,6 +128,7 @@ int model_fmm(int64_t n, const double* pos, const double* mass, double G, double
     case P: model_fmm<P>(t, G, eps, acc); return 0;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+        AFMM_CASE(13) AFMM_CASE(14) AFMM_CASE(15) AFMM_CASE(16)
 #undef AFMM_CASE
     }
     return -2;

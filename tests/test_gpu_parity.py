@@ -74,7 +74,9 @@ def test_forces_vs_reference_fixture(afmm, case):
     (6000, 0.0, 4, 64, 1), (6000, 0.01, 8, 64, 2), (5000, 0.005, 12, 64, 3),
     (4000, 0.02, 10, 16, 4), (3000, 0.0, 6, 8, 5), (4000, 0.01, 3, 100, 6),
     (3000, 0.01, 1, 32, 7), (3000, 0.01, 2, 32, 8), (3000, 0.0, 5, 32, 9), (3000, 0.01, 7, 48, 10),
-    (3000, 0.0, 9, 32, 11), (3000, 0.01, 11, 64, 12)])
+    (3000, 0.0, 9, 32, 11), (3000, 0.01, 11, 64, 12),
+    # orders above 12 (the oracle takes tens of seconds here)
+    (2500, 0.003, 13, 48, 13), (2000, 0.0, 14, 64, 14), (2000, 0.01, 15, 48, 15), (2500, 0.0, 16, 48, 16)])
 def test_forces_vs_oracle_plummer(afmm, n, eps, p, cap, seed):
     orc = cabi.oracle()
     pos, _, m = orc.plummer(n, seed=seed)

@@ -42,6 +42,18 @@ def test_model_all_orders_against_oracle():
         assert mx <= 1e-12, (p, mx)
 
 
+def test_model_orders_above_12_against_oracle():
+    """Orders 13-16 (S up to 289, three coefficient windows in P2M/P2L) on
+    a smaller sample: the oracle's O(p^4) M2L is slow on the CPU."""
+    orc = cabi.oracle()
+    pos, _, m = orc.plummer(600, seed=24)
+    for p in (13, 16):
+        ref, _ = orc.fmm(pos, m, 1.0, 0.003, p, 24)
+        acc = cabi.model_fmm(pos, m, 1.0, 0.003, p, 24)
+        mx, _ = cabi.rel_err(ref, acc)
+        assert mx <= 1e-12, (p, mx)
+
+
 def test_library_exports_every_declared_symbol():
     hdr = open(os.path.join(cabi.ROOT, "include", "afmm.h")).read()
     names = set(re.findall(r"^\s*(?:int|int64_t|void|const char\*)\s+(afmm_\w+)\s*\(", hdr, re.M))
@@ -54,7 +66,7 @@ def test_library_exports_every_declared_symbol():
 def test_parameter_validation_without_device():
     """Argument errors are reported before any device work."""
     mod = cabi.product()
-    for kw, code in (({"p": 0}, -2), ({"p": 13}, -2), ({"threshold": 0}, -3), ({"eps": -1.0}, -1),
+    for kw, code in (({"p": 0}, -2), ({"p": 17}, -2), ({"threshold": 0}, -3), ({"eps": -1.0}, -1),
                      ({"precision": 7}, -9)):
         with pytest.raises(mod.AfmmError) as e:
             mod.AdaptiveFMM3D(**kw)
