@@ -612,7 +612,15 @@ def main():
            "orders": orders,
            "tree": {"boxes": st["n_boxes"], "leaves": st["n_leaves"], "max_level": st["max_level"],
                     "max_leaf_occ": st["max_leaf_occ"], "v_pairs": st["n_v"], "p2p_pairs": st["p2p_pairs"],
-                    "u_entries": st["n_u"], "w_entries": st["n_w"]},
+                    "colleague_entries": st["n_colleague"], "u_entries": st["n_u"], "w_entries": st["n_w"],
+                    "x_entries": st["n_x"],
+                    # algorithmic bytes of the build (68 N + 32 N + 80 B + 4 E, E = all list entries)
+                    # over the one-stream tree phase
+                    "build_bytes": 100.0 * n + 80.0 * st["n_boxes"] + 4.0 * (st["n_colleague"] + st["n_u"] + st["n_v"]
+                                                                           + st["n_w"] + st["n_x"]),
+                    "build_gbps_one_stream": (100.0 * n + 80.0 * st["n_boxes"] + 4.0 * (
+                        st["n_colleague"] + st["n_u"] + st["n_v"] + st["n_w"] + st["n_x"]))
+                    / (seq_avg["tree"] * 1e-3) / 1e9},
            "host_io_s_per_eval": host_io}
     if not args.no_large:
         progress("4M leg")

@@ -90,6 +90,25 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         AFMM_CHECK(hipEventCreateWithFlags(&c->up_done, hipEventDisableTiming));
         c->side = shared_side_stream();
     }
+    // With the bf16 M2L (outside graph mode, overlap on), M2L is enqueued on s
+    // right after its preparation (behind the V lists), and the U, W and X
+    // lists, P2L, M2P and P2P run on the second stream beside it, after the
+    // upward pass; the lists' size readback then synchronises the second
+    // stream only, while M2L keeps the GPU busy. The bf16 M2L leaves room in
+    // the register files for those waves (2 waves per SIMD at 216 VGPRs); the
+    // FP64 and split-FP32 M2L fill them (4 waves at 124-128 VGPRs), so beside
+    // them the second stream's kernels only run where M2L tiles have finished,
+    // and those forms keep the lists and the X/W passes in order on s.
+    const bool lists_beside = split && !c->use_graph && m2l_form(c->prm.precision) == kPrecMixedBf16;
+    if (lists_beside && !c->list_fork) {
+        AFMM_CHECK(hipEventCreateWithFlags(&c->list_fork, hipEventDisableTiming));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->lists_done, hipEventDisableTiming));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->m2l_done, hipEventDisableTiming));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->x_done, hipEventDisableTiming));
+        AFMM_CHECK(hipEventCreateWithFlags(&c->w_done, hipEventDisableTiming));
+    }
+    tv.sys_G = d_ge;
+    tv.sys_eps2 = d_ge + nsys;
     FmmTables tab{c->d_m2m_sw, c->d_l2l_sw, c->d_m2l_sw, c->d_m2l_hl, c->d_m2l_bf};
     bool up_launched = false;
     auto on_core = [&](const TreeView& t, const TreeMeta& m) {
@@ -101,18 +120,35 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     };
     // ... and the M2L preparation is enqueued right after the V lists, ahead of
     // the U, W and X lists that M2L does not need.
-    bool prep_done = false;
+    bool prep_done = false, m2l_launched = false;
     auto on_vlists = [&](const TreeView& t, const TreeMeta&) {
         run_m2l_prep(c->pool, s, t, c->prm.p, c->prm.precision, c->shapes);
         prep_done = true;
+        if (lists_beside) {
+            AFMM_CHECK(hipStreamWaitEvent(s, c->up_done, 0));  // the multipoles are done
+            run_m2l(c->pool, s, t, c->prm.p, tab, c->prm.precision, c->shapes, c->ev[7], nullptr);
+            AFMM_CHECK(hipEventRecord(c->m2l_done, s));
+            m2l_launched = true;
+        }
     };
     build_forest(c->pool, s, pos, mass, nsys, sys_off, eps, c->prm.leaf_cap, tv, meta, roots,
-                 split ? TreeHook(on_core) : TreeHook(), split ? TreeHook(on_vlists) : TreeHook());
+                 split ? TreeHook(on_core) : TreeHook(), split ? TreeHook(on_vlists) : TreeHook(),
+                 lists_beside ? c->side : nullptr, c->list_fork);
     for (const RootInfo& r : roots)
         if (r.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
-    tv.sys_G = d_ge;
-    tv.sys_eps2 = d_ge + nsys;
-    AFMM_CHECK(hipEventRecord(c->ev[1], s));
+    if (lists_beside) {
+        // the tree phase ends with the lists on the second stream; P2P follows
+        // them there (after M2L in the bf16 form, whose short M2L it slowed)
+        AFMM_CHECK(hipEventRecord(c->ev[1], c->side));
+        AFMM_CHECK(hipEventRecord(c->lists_done, c->side));
+        // P2L and M2P (needed first, by L2L and L2P on s), then P2P
+        run_far_passes(c->pool, c->side, tv, c->prm.p, phi != nullptr, c->x_done, c->w_done);
+        if (m2l_form(c->prm.precision) == kPrecMixedBf16) AFMM_CHECK(hipStreamWaitEvent(c->side, c->m2l_done, 0));
+        run_p2p_near(c->pool, c->side, tv, c->prm.precision, c->shapes, phi != nullptr, c->ev[8], c->ev[9]);
+        AFMM_CHECK(hipEventRecord(c->side_join, c->side));
+    } else {
+        AFMM_CHECK(hipEventRecord(c->ev[1], s));
+    }
     if (c->use_graph) {
         // the upward pass and the M2L preparation already ran outside the graph
         // (beside and within the list build); the stream waits for them before
@@ -167,6 +203,12 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     } else {
         SideStream side{};
         if (split) side = SideStream{c->side, c->side_fork, c->side_join, up_launched, c->up_done, prep_done};
+        side.m2l_launched = m2l_launched;
+        side.p2p_launched = lists_beside;
+        side.lists_done = lists_beside ? c->lists_done : nullptr;
+        side.far_beside = lists_beside;
+        side.x_done = c->x_done;
+        side.w_done = c->w_done;
         run_fmm(c->pool, s, tv, meta, c->prm.p, tab, c->prm.precision, c->shapes, acc, phi, c->ev + 2, false, side);
     }
     AFMM_CHECK(hipStreamSynchronize(s));
@@ -324,7 +366,8 @@ void afmm_destroy(afmm_ctx* c) {
     c->pool.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {c->side_fork, c->side_join, c->up_fork, c->up_done})
+    for (hipEvent_t e : {c->side_fork, c->side_join, c->up_fork, c->up_done, c->list_fork, c->lists_done, c->m2l_done,
+                         c->x_done, c->w_done})
         if (e) (void)hipEventDestroy(e);
     if (c->graph_side) (void)hipStreamDestroy(c->graph_side);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -36,6 +36,8 @@ struct afmm_ctx {
     bool split_p2p = true;
     hipStream_t side = nullptr;
     hipEvent_t side_fork = nullptr, side_join = nullptr, up_fork = nullptr, up_done = nullptr;
+    // the U, W and X lists on the second stream beside M2L (overlap on, graph off)
+    hipEvent_t list_fork = nullptr, lists_done = nullptr, m2l_done = nullptr, x_done = nullptr, w_done = nullptr;
     // FMM passes replayed as a HIP graph (afmm_set_graph): the executable
     // graph of the last evaluation, updated in place when the next capture has
     // the same shape, re-instantiated otherwise

@@ -180,11 +180,16 @@ int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, 
 // on_core (optional) is called once the boxes, bodies, geometry and level
 // orders are enqueued and before the interaction lists are: everything the
 // upward pass reads is then in place on s. on_vlists (optional) is called
-// once the colleague and V lists are enqueued, before U, W and X.
+// once the colleague and V lists are enqueued, before U, W and X. With ls
+// (and its event ls_ready), the U, W and X lists are built on stream ls after
+// it waits for the colleague lists on s (recorded before on_vlists runs; their
+// size readback then synchronises ls only, so work that on_vlists enqueues on s
+// keeps the GPU busy meanwhile).
 using TreeHook = std::function<void(const TreeView&, const TreeMeta&)>;
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
-                  std::vector<RootInfo>& roots, const TreeHook& on_core = {}, const TreeHook& on_vlists = {});
+                  std::vector<RootInfo>& roots, const TreeHook& on_core = {}, const TreeHook& on_vlists = {},
+                  hipStream_t ls = nullptr, hipEvent_t ls_ready = nullptr);
 
 // ---- fmm.hip -----------------------------------------------------------------
 struct FmmTimes {
@@ -226,6 +231,18 @@ struct SideStream {
     // set when run_m2l_prep already enqueued the M2L preparation on the
     // evaluation's own stream (after the V lists)
     bool prep_done = false;
+    // set when run_m2l already enqueued M2L itself there too (the U, W and X
+    // lists then build on the second stream beside it)
+    bool m2l_launched = false;
+    // set when run_p2p_near already enqueued P2P on the second stream (after
+    // the lists there); lists_done is recorded on it between the lists and P2P,
+    // and s waits for it before P2L and M2P
+    bool p2p_launched = false;
+    hipEvent_t lists_done = nullptr;
+    // set when run_far_passes already enqueued P2L (into Lx, then x_done) and
+    // M2P (into farw, then w_done) on the second stream
+    bool far_beside = false;
+    hipEvent_t x_done = nullptr, w_done = nullptr;
 };
 // The M2L preparation alone on stream s (needs the V lists and the M2L order);
 // a run_fmm with side.prep_done then skips it.
@@ -251,6 +268,19 @@ void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec
 // run_fmm with side.upward_launched then skips them.
 void run_upward(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p, const FmmTables& tab,
                 hipEvent_t e0, hipEvent_t e1);
+// M2L alone on stream s, after the M2L preparation (the split of the
+// multipoles into bf16 planes included); events e0/e1 around it when non-null.
+void run_m2l(Pool& pool, hipStream_t s, const TreeView& tv, int p, const FmmTables& tab, int prec, const Shapes& sh,
+             hipEvent_t e0, hipEvent_t e1);
+// The P2P near field alone on stream s into the pool's near-field buffers
+// (potential too when pot), for a later run_fmm with side.p2p_launched.
+void run_p2p_near(Pool& pool, hipStream_t s, const TreeView& tv, int prec, const Shapes& sh, bool pot, hipEvent_t e0,
+                  hipEvent_t e1);
+// P2L over X into the pool's Lx and M2P over W into its farw (and farwpot when
+// pot) on stream s, recording x_done and w_done after each, for a later
+// run_fmm with side.far_beside.
+void run_far_passes(Pool& pool, hipStream_t s, const TreeView& tv, int p, bool pot, hipEvent_t x_done,
+                    hipEvent_t w_done);
 void run_fmm(Pool& pool, hipStream_t s, const TreeView& tv, const TreeMeta& meta, int p,
              const FmmTables& tab, int prec, const Shapes& sh, double* d_acc, double* d_phi,
              hipEvent_t* ev, bool reserve_only = false, SideStream side = SideStream{});

@@ -80,6 +80,8 @@ __device__ __forceinline__ void wave_halving_sum(double* v, int lane, int& first
 // (two passes over the stream) to bound registers.
 //   KIND 0: targets = leaves, sources = the leaf's own bodies, Mhat = result.
 //   KIND 1: targets = boxes, sources = bodies of X(box) leaves, Lhat += result.
+//   KIND 2: as 1 into a separate array Lx (k_add_x then adds it to Lhat after
+//   M2L: the same sums in the same order, and free to run beside M2L).
 template <int S>
 constexpr int coef_windows() { return S <= 96 ? 1 : ((S + 127) / 128 > 2 ? (S + 127) / 128 : 2); }
 template <int S>
@@ -191,8 +193,8 @@ __device__ __forceinline__ void accumulate_window(const TreeView& t, const int* 
 #pragma unroll
     for (int k = 0; k < 2; ++k)
         if (k < cnt && C0 + first + k < S) {
-            if (KIND == 0) out[out_base + C0 + first + k] = acc[k];
-            else out[out_base + C0 + first + k] += acc[k];
+            if constexpr (KIND == 1) out[out_base + C0 + first + k] += acc[k];
+            else out[out_base + C0 + first + k] = acc[k];
         }
 }
 
@@ -251,7 +253,9 @@ constexpr int m2p_groups() { return P <= 8 ? 32 : 16; }
 template <int P>
 constexpr int m2p_min_waves() { return P <= 8 ? 4 : 1; }
 
-template <int P, bool POT>
+// ADD: far += the field (after L2P wrote far); otherwise far = the field, zero
+// for leaves without W (the separate array L2P adds later; the same sums).
+template <int P, bool POT, bool ADD>
 __global__ void __launch_bounds__(kM2PWaves * 64, m2p_min_waves<P>()) k_m2p(TreeView t, const double* __restrict__ M,
                                                        double* __restrict__ far, double* __restrict__ fpot) {
     constexpr int S = nterms(P), L = kM2PWaves * 64, GM = m2p_groups<P>();
@@ -260,10 +264,20 @@ __global__ void __launch_bounds__(kM2PWaves * 64, m2p_min_waves<P>()) k_m2p(Tree
     const int li = xcd_chunk_item();
     if (li >= t.nleaves) return;
     const int wa = t.w_off[li], nw = t.w_off[li + 1] - wa;
-    if (nw == 0) return;
     const int b = t.leaf_ids[li];
     const int f = t.seg_first[b], nt = t.seg_count[b];
     const int tid = threadIdx.x;
+    if (nw == 0) {  // no W: a zero field, so that L2P can add farw everywhere
+        if constexpr (ADD) return;
+        for (int k = tid; k < nt; k += kM2PWaves * 64) {
+            const size_t i = (size_t)(f + k);
+            far[3 * i] = 0.0;
+            far[3 * i + 1] = 0.0;
+            far[3 * i + 2] = 0.0;
+            if constexpr (POT) fpot[i] = 0.0;
+        }
+        return;
+    }
     const int T = nt > L ? L : max(nt, L / GM);
     const int Gs = L / T;  // <= GM
     const int g = tid / T, tl = tid % T;
@@ -313,10 +327,17 @@ __global__ void __launch_bounds__(kM2PWaves * 64, m2p_min_waves<P>()) k_m2p(Tree
                 }
         }
         if (act && g == 0) {
-            far[3 * (size_t)i] += fx;
-            far[3 * (size_t)i + 1] += fy;
-            far[3 * (size_t)i + 2] += fz;
-            if constexpr (POT) fpot[i] += fp;
+            if constexpr (ADD) {
+                far[3 * (size_t)i] += fx;
+                far[3 * (size_t)i + 1] += fy;
+                far[3 * (size_t)i + 2] += fz;
+                if constexpr (POT) fpot[i] += fp;
+            } else {
+                far[3 * (size_t)i] = fx;
+                far[3 * (size_t)i + 1] = fy;
+                far[3 * (size_t)i + 2] = fz;
+                if constexpr (POT) fpot[i] = fp;
+            }
         }
     }
 }
@@ -956,8 +977,10 @@ __global__ void __launch_bounds__(kLevelWaves * 64) k_level_mfma(int n, const in
 
 // ---- L2P: one thread per sorted body (far field from its leaf's local).
 template <int P>
+// far = L2P + farw (the M2P field, when there are W lists); potential alike.
 __global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict__ L, double* __restrict__ far,
-                                            double* __restrict__ fpot) {
+                                            double* __restrict__ fpot, const double* __restrict__ farw,
+                                            const double* __restrict__ fpotw) {
     constexpr int S = nterms(P);
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= t.n) return;
@@ -976,10 +999,29 @@ __global__ void __launch_bounds__(kT) k_far(TreeView t, const double* __restrict
     (void)S;
     local_grad<P>([&](int k) { return lb[k]; }, dx, dy, dz, gx, gy, gz);
     const double s2 = t.invh[b] * t.invh[b];
-    far[3 * (size_t)i] = gx * s2;
-    far[3 * (size_t)i + 1] = gy * s2;
-    far[3 * (size_t)i + 2] = gz * s2;
-    if (fpot) fpot[i] = local_pot<P>([&](int k) { return lb[k]; }, dx, dy, dz) * t.invh[b];
+    double fx = gx * s2, fy = gy * s2, fz = gz * s2;
+    if (farw) {
+        fx += farw[3 * (size_t)i];
+        fy += farw[3 * (size_t)i + 1];
+        fz += farw[3 * (size_t)i + 2];
+    }
+    far[3 * (size_t)i] = fx;
+    far[3 * (size_t)i + 1] = fy;
+    far[3 * (size_t)i + 2] = fz;
+    if (fpot) {
+        double pt = local_pot<P>([&](int k) { return lb[k]; }, dx, dy, dz) * t.invh[b];
+        if (fpotw) pt += fpotw[i];
+        fpot[i] = pt;
+    }
+}
+
+// L += Lx for the boxes with an X list (the P2L totals, after M2L).
+__global__ void __launch_bounds__(kT) k_add_x(int nb, int S, const int* __restrict__ x_off,
+                                              const double* __restrict__ Lx, double* __restrict__ L) {
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= (long long)nb * S) return;
+    const int b = (int)(g / S);
+    if (x_off[b + 1] > x_off[b]) L[g] += Lx[g];
 }
 
 // ---- P2P over U + final combine (AdaptiveFMM3D.hpp:96-106) ------------------
@@ -1315,6 +1357,68 @@ void upward(hipStream_t s, const TreeView& t, const TreeMeta& meta, const FmmTab
     if (e1) AFMM_CHECK(hipEventRecord(e1, s));
 }
 
+// M2L on stream s for a tree whose M2L preparation is done: the bf16 split of
+// the multipoles (bf16 form) and the M2L kernel of the order, precision form
+// and workgroup shape; events e0/e1 (optional) just before and after the kernel.
+template <int P>
+void m2l_launch(Pool& pool, hipStream_t s, const TreeView& t, const FmmTables& tab, int prec, const Shapes& sh,
+                hipEvent_t e0, hipEvent_t e1) {
+    constexpr int S = nterms(P);
+    const int form = m2l_form(prec);
+    const int nb = t.nb, np = nb;
+    double* M = pool.get<double>("Mexp", (size_t)nb * S);
+    double* L = pool.get<double>("Lexp", (size_t)nb * S);
+    const int TILE = sh.m2l_waves * 16 * kM2LCb;
+    if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
+    const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
+    const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
+    const int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
+    const unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
+    uint4* Mbf = form == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
+                                        : nullptr;
+    if (form == kPrecMixedBf16) {
+        constexpr int KB = m2l_bf_kb(S);
+        hipLaunchKernelGGL(k_split_bf16<P>, dim3(blocks((long long)nb * KB * 4)), dim3(kT), 0, s, nb, M, Mbf);
+    }
+    if (e0) AFMM_CHECK(hipEventRecord(e0, s));
+    auto launch_m2l = [&](auto wtag) {
+        constexpr int W = decltype(wtag)::value;
+        const dim3 g(xcd_grid(ntiles)), b(W * 64);
+        if (form == kPrecMixedBf16)
+            hipLaunchKernelGGL((k_m2l_bf16<P, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT, present,
+                               tab.m2l_bf, Mbf, L);
+        else if constexpr (W == 8 || P <= 8) {
+            if (form == kPrecMixed)
+                hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                                   present, tab.m2l_hl, M, L);
+            else
+                hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                                   present, tab.m2l_sw, M, L);
+        } else {
+            throw std::runtime_error("16-wave M2L workgroups need p <= 8 or the bf16 M2L");
+        }
+    };
+    if (sh.m2l_waves == 16) launch_m2l(std::integral_constant<int, 16>{});
+    else launch_m2l(std::integral_constant<int, 8>{});
+    if (e1) AFMM_CHECK(hipEventRecord(e1, s));
+}
+
+// P2L over X into Lx (then x_done) and M2P over W into farw (then w_done) on
+// stream s; either buffer is null when its lists are empty. They read only the
+// tree, the lists, the bodies and the multipoles, so they can run beside M2L.
+template <int P>
+void far_passes(hipStream_t s, const TreeView& t, const double* M, double* Lx, double* farw, double* fpotw,
+                hipEvent_t x_done = nullptr, hipEvent_t w_done = nullptr) {
+    if (Lx) hipLaunchKernelGGL((k_accumulate<P, 2>), dim3(xcd_chunk_grid(t.nb)), dim3(64), 0, s, t, Lx);
+    if (x_done) AFMM_CHECK(hipEventRecord(x_done, s));
+    if (farw) {
+        const dim3 mg(xcd_chunk_grid(t.nleaves)), mb(kM2PWaves * 64);
+        if (fpotw) hipLaunchKernelGGL((k_m2p<P, true, false>), mg, mb, 0, s, t, M, farw, fpotw);
+        else hipLaunchKernelGGL((k_m2p<P, false, false>), mg, mb, 0, s, t, M, farw, fpotw);
+    }
+    if (w_done) AFMM_CHECK(hipEventRecord(w_done, s));
+}
+
 // All device buffers are taken from the pool before the first launch, so that
 // a call with reserve_only (no launches) sizes them and a following call under
 // stream capture allocates nothing.
@@ -1340,6 +1444,13 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
     uint4* Mbf = form == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
                                         : nullptr;
+    (void)Mbf;  // (sized here for the capture; m2l_launch takes it)
+    // with P2L and M2P beside M2L: their totals (added to L after M2L) and field
+    // (added by L2P); in order they add into L and far directly (the same sums)
+    const bool fb = side.far_beside;
+    double* Lx = fb && t.n_x > 0 ? pool.get<double>("Lx", (size_t)nb * S) : nullptr;
+    double* farw = fb && t.n_w > 0 ? pool.get<double>("farw", 3 * (size_t)t.n) : nullptr;
+    double* fpotw = fb && t.n_w > 0 && phi ? pool.get<double>("farwpot", t.n) : nullptr;
     const bool split = side.s != nullptr;
     double* nr = split ? pool.get<double>("near", 3 * (size_t)t.n) : nullptr;
     double* npot = split && phi ? pool.get<double>("nearpot", t.n) : nullptr;
@@ -1359,40 +1470,24 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
         AFMM_CHECK(hipEventRecord(side.join, side.s));
     };
     const bool p2p_after_m2l = form == kPrecMixedBf16;
-    if (split && !p2p_after_m2l) fork_p2p();
+    if (split && !p2p_after_m2l && !side.p2p_launched) fork_p2p();
     if (!side.upward_launched) upward<P>(s, t, meta, tab, M, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
-    {
+    if (!side.m2l_launched) {
         if (!side.prep_done) m2l_prep(s, t, np, ntiles, tstart, rank, srcT, present);
         if (side.upward_launched && side.up_done) AFMM_CHECK(hipStreamWaitEvent(s, side.up_done, 0));  // M done
-        if (form == kPrecMixedBf16) {
-            constexpr int KB = m2l_bf_kb(S);
-            hipLaunchKernelGGL(k_split_bf16<P>, dim3(blocks((long long)nb * KB * 4)), dim3(kT), 0, s, nb, M, Mbf);
-        }
-        if (ev) AFMM_CHECK(hipEventRecord(ev[5], s));
-        auto launch_m2l = [&](auto wtag) {
-            constexpr int W = decltype(wtag)::value;
-            const dim3 g(xcd_grid(ntiles)), b(W * 64);
-            if (form == kPrecMixedBf16)
-                hipLaunchKernelGGL((k_m2l_bf16<P, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT, present,
-                                   tab.m2l_bf, Mbf, L);
-            else if constexpr (W == 8 || P <= 8) {
-                if (form == kPrecMixed)
-                    hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
-                                       present, tab.m2l_hl, M, L);
-                else
-                    hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
-                                       present, tab.m2l_sw, M, L);
-            } else {
-                throw std::runtime_error("16-wave M2L workgroups need p <= 8 or the bf16 M2L");
-            }
-        };
-        if (sh.m2l_waves == 16) launch_m2l(std::integral_constant<int, 16>{});
-        else launch_m2l(std::integral_constant<int, 8>{});
+        m2l_launch<P>(pool, s, t, tab, prec, sh, ev ? ev[5] : nullptr, nullptr);
     }
-    if (split && p2p_after_m2l) fork_p2p();
+    if (split && p2p_after_m2l && !side.p2p_launched) fork_p2p();
     if (ev) AFMM_CHECK(hipEventRecord(ev[2], s));
-    if (t.n_x > 0)
+    // U, W and X lists built on the second stream (beside M2L) are needed from here
+    if (side.lists_done) AFMM_CHECK(hipStreamWaitEvent(s, side.lists_done, 0));
+    // P2L: beside M2L into Lx (added here), or here into L
+    if (fb) {
+        AFMM_CHECK(hipStreamWaitEvent(s, side.x_done, 0));
+        if (Lx) hipLaunchKernelGGL(k_add_x, dim3(blocks((long long)nb * S)), dim3(kT), 0, s, nb, S, t.x_off, Lx, L);
+    } else if (t.n_x > 0) {
         hipLaunchKernelGGL((k_accumulate<P, 1>), dim3(xcd_chunk_grid(nb)), dim3(64), 0, s, t, L);
+    }
     for (int l = 1; l <= t.max_level; ++l) {
         const int off = meta.level_off[l], cnt = meta.level_off[l + 1] - off;
         hipLaunchKernelGGL((k_level_mfma<P, 1>), dim3((cnt + 15) / 16), dim3(kLevelWaves * 64), 0, s, cnt,
@@ -1400,11 +1495,12 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
                            t.child_first, t.child_mask, t.parent, t.octant, tab.l2l_sw, L, L);
     }
     if (ev) AFMM_CHECK(hipEventRecord(ev[3], s));
-    hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far, fpot);
-    if (t.n_w > 0) {
+    if (fb) AFMM_CHECK(hipStreamWaitEvent(s, side.w_done, 0));
+    hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far, fpot, farw, fpotw);
+    if (!fb && t.n_w > 0) {  // M2P in order: added to far after L2P
         const dim3 mg(xcd_chunk_grid(t.nleaves)), mb(kM2PWaves * 64);
-        if (fpot) hipLaunchKernelGGL((k_m2p<P, true>), mg, mb, 0, s, t, M, far, fpot);
-        else hipLaunchKernelGGL((k_m2p<P, false>), mg, mb, 0, s, t, M, far, fpot);
+        if (fpot) hipLaunchKernelGGL((k_m2p<P, true, true>), mg, mb, 0, s, t, M, far, fpot);
+        else hipLaunchKernelGGL((k_m2p<P, false, true>), mg, mb, 0, s, t, M, far, fpot);
     }
     if (split) {
         AFMM_CHECK(hipStreamWaitEvent(s, side.join, 0));
@@ -1441,6 +1537,50 @@ void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec
     switch (p) {
 #define AFMM_CASE(PP) \
     case PP: m2l_prep_for<PP>(pool, s, tv, prec, sh); return;
+        AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
+        AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+        AFMM_CASE(13) AFMM_CASE(14) AFMM_CASE(15) AFMM_CASE(16)
+#undef AFMM_CASE
+    }
+    throw std::runtime_error("unsupported expansion order");
+}
+
+void run_m2l(Pool& pool, hipStream_t s, const TreeView& tv, int p, const FmmTables& tab, int prec, const Shapes& sh,
+             hipEvent_t e0, hipEvent_t e1) {
+    switch (p) {
+#define AFMM_CASE(PP) \
+    case PP: m2l_launch<PP>(pool, s, tv, tab, prec, sh, e0, e1); return;
+        AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
+        AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
+        AFMM_CASE(13) AFMM_CASE(14) AFMM_CASE(15) AFMM_CASE(16)
+#undef AFMM_CASE
+    }
+    throw std::runtime_error("unsupported expansion order");
+}
+
+void run_p2p_near(Pool& pool, hipStream_t s, const TreeView& tv, int prec, const Shapes& sh, bool pot, hipEvent_t e0,
+                  hipEvent_t e1) {
+    double* nr = pool.get<double>("near", 3 * (size_t)tv.n);
+    double* npot = pot ? pool.get<double>("nearpot", tv.n) : nullptr;
+    const int* order = sh.p2p_schedule ? p2p_order(pool, s, tv, false) : nullptr;
+    if (e0) AFMM_CHECK(hipEventRecord(e0, s));
+    // (the near-field kernel reads neither far nor acc; its potential flag is
+    // the npot buffer)
+    launch_p2p<true>(s, tv, p2p_fp32(prec), nullptr, nullptr, nullptr, npot, nr, npot, sh, order);
+    if (e1) AFMM_CHECK(hipEventRecord(e1, s));
+    AFMM_CHECK(hipGetLastError());
+}
+
+void run_far_passes(Pool& pool, hipStream_t s, const TreeView& tv, int p, bool pot, hipEvent_t x_done,
+                    hipEvent_t w_done) {
+    const int S = nterms(p);
+    const double* M = pool.get<double>("Mexp", (size_t)tv.nb * S);
+    double* Lx = tv.n_x > 0 ? pool.get<double>("Lx", (size_t)tv.nb * S) : nullptr;
+    double* farw = tv.n_w > 0 ? pool.get<double>("farw", 3 * (size_t)tv.n) : nullptr;
+    double* fpotw = tv.n_w > 0 && pot ? pool.get<double>("farwpot", tv.n) : nullptr;
+    switch (p) {
+#define AFMM_CASE(PP) \
+    case PP: far_passes<PP>(s, tv, M, Lx, farw, fpotw, x_done, w_done); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
         AFMM_CASE(13) AFMM_CASE(14) AFMM_CASE(15) AFMM_CASE(16)

@@ -992,6 +992,9 @@ __global__ void __launch_bounds__(64) k_xlist(int nb, const RootInfo* __restrict
         if (mode == 0 && lane == 0) xcnt[w] = 0;
         return;
     }
+    // the filling launch skips the boxes the counting launch found without X
+    // (most of them: X is non-empty only at refinement boundaries)
+    if (mode && xoff[w + 1] == xoff[w]) return;
     const double tol = roots[bsys[w]].tol;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     if (lane == 0) {
@@ -1064,7 +1067,8 @@ int read_int(const int* d, hipStream_t s) {
 
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
-                  std::vector<RootInfo>& roots, const TreeHook& on_core, const TreeHook& on_vlists) {
+                  std::vector<RootInfo>& roots, const TreeHook& on_core, const TreeHook& on_vlists,
+                  hipStream_t ls, hipEvent_t ls_ready) {
     const int n = h_sys_off[nsys];
     // ---- per-system root cubes: chunks of at most kBBoxChunk bodies of one system
     std::vector<int> hb;  // [cbeg | cend | cfirst(nsys+1) | sys_off(nsys+1)] as one upload
@@ -1127,13 +1131,11 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     const int sdig = nsys <= 1 ? 0 : (nsys <= 512 ? 1 : 2);
     int r1 = radix_sort(sb, n, 8 - ndig, ndig, s);
     if (sdig > 0) r1 = radix_sort(sb, n, 8, sdig, s, r1);
-    // keep pass-1 keys/perm: copy into stable names
-    uint64_t* khi = pool.get<uint64_t>("key1_hi", n);
-    uint32_t* klo = pool.get<uint32_t>("key1_lo", n);
-    int* perm1 = pool.get<int>("perm1", n);
-    AFMM_CHECK(hipMemcpyAsync(khi, sb.khi[r1], sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
-    AFMM_CHECK(hipMemcpyAsync(klo, sb.klo[r1], sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
-    AFMM_CHECK(hipMemcpyAsync(perm1, sb.val[r1], sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    // the pass-1 keys and permutation stay where the sort left them (nothing
+    // below sorts with these buffers again)
+    const uint64_t* khi = sb.khi[r1];
+    const uint32_t* klo = sb.klo[r1];
+    const int* perm1 = sb.val[r1];
 
     // ---- topology, level by level
     size_t cap_boxes = std::max<size_t>(1024, (size_t)n / 4 + 64);
@@ -1302,11 +1304,19 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
                            tv.seg_first, tv.seg_count, perm1, lp, tv.leaf_of);
         leaf_perm = lp;
     } else {
+        SortBufs lk;  // buffers of their own: the pass-1 keys in sb are read later
+        for (int k = 0; k < 2; ++k) {
+            lk.khi[k] = pool.get<uint64_t>(k ? "lk_khi1" : "lk_khi0", n);
+            lk.klo[k] = pool.get<uint32_t>(k ? "lk_klo1" : "lk_klo0", n);
+            lk.val[k] = pool.get<int>(k ? "lk_val1" : "lk_val0", n);
+        }
+        lk.hist = sb.hist;
+        lk.scan_tmp = sb.scan_tmp;
         hipLaunchKernelGGL(k_leaf_keys, dim3(blocks(64LL * tv.nleaves)), dim3(kT), 0, s, tv.nleaves, tv.leaf_ids,
-                           tv.seg_first, tv.seg_count, perm1, sb.khi[0], sb.klo[0], sb.val[0], tv.leaf_of);
+                           tv.seg_first, tv.seg_count, perm1, lk.khi[0], lk.klo[0], lk.val[0], tv.leaf_of);
         int bits = 1;
         while ((1LL << bits) <= n) ++bits;
-        leaf_perm = sb.val[radix_sort(sb, n, 1, (bits + 8) / 9, s)];
+        leaf_perm = lk.val[radix_sort(lk, n, 1, (bits + 8) / 9, s)];
     }
     tv.x = pool.get<double>("bx", n);
     tv.y = pool.get<double>("by", n);
@@ -1387,9 +1397,16 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     }
     // (the overflow flag is read back with the list sizes below; list counts
     // are capped at their slots, so the kernels in between stay in bounds)
+    // (the list stream waits for the colleague lists before on_vlists enqueues
+    // more work on s, which the lists do not need)
+    if (ls && ls != s) {
+        AFMM_CHECK(hipEventRecord(ls_ready, s));
+        AFMM_CHECK(hipStreamWaitEvent(ls, ls_ready, 0));
+    }
     if (on_vlists) on_vlists(tv, meta);
 
     // ---- U and W per leaf, X per box (count, scan, one size readback, fill)
+    if (ls && ls != s) s = ls;
     const int nl = tv.nleaves;
     int* ucnt = pool.get<int>("ucnt", nl + 1);
     int* wcnt = pool.get<int>("wcnt", nl + 1);
