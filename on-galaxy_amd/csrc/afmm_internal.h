@@ -119,7 +119,7 @@ struct TreeView {
     int* ic;             // [3B] integer coordinates at the box's level
     int* leaf_ids;       // [nleaves] BFS ids of the leaves
     int* lo_order;       // [B] boxes sorted by (level, octant); level l is [level_off[l], level_off[l+1])
-    int* m2l_order;      // [B] boxes sorted by (octant, level): the M2L target order
+    int* m2l_order;      // [B] by (octant, level, BFS), then by V signature inside each 256-target tile
     // M2L target tiles over m2l_order, cut inside octant groups: tile k is
     // ranks [m2l_tiles{256,128}[k], [k+1]), at most 256 (128) targets each
     int *m2l_tiles256, *m2l_tiles128;

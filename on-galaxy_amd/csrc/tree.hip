@@ -490,6 +490,30 @@ __global__ void __launch_bounds__(kT) k_m2l_keys(int nb, const int* level, const
     klo[b] = 0;
     val[b] = b;
 }
+// Inside each 256-target M2L tile, a stable sort by V signature (vsig): the
+// 16 targets of a column block then mostly have sources at the same offsets
+// (on a 1M-body merger 13% fewer (block, offset) items than in BFS order),
+// while the tile keeps the same targets, and so the same source boxes for the
+// L2 to reuse from one offset to the next (a global sort by signature cuts
+// more items but scatters a tile's sources and measured slower). Any order
+// gives the same sums. One block per tile.
+__global__ void __launch_bounds__(256) k_m2l_tile_sort(const int* __restrict__ tstart, const int* __restrict__ vsig,
+                                                      int* __restrict__ order) {
+    __shared__ int box[256], key[256];
+    const int a = tstart[blockIdx.x], n = tstart[blockIdx.x + 1] - a;  // n <= 256
+    const int i = threadIdx.x;
+    if (i < n) {
+        box[i] = order[a + i];
+        key[i] = vsig[box[i]];
+    }
+    __syncthreads();
+    if (i < n) {
+        const int k = key[i];
+        int r = 0;
+        for (int j = 0; j < n; ++j) r += key[j] < k || (key[j] == k && j < i);
+        order[a + r] = box[i];
+    }
+}
 // Boxes per octant (one block of 1024 threads; integer sums, any order).
 constexpr int kOctCountThreads = 1024;
 __global__ void __launch_bounds__(kOctCountThreads) k_octant_count(int nb, const int* octant, int* cnt) {
@@ -670,17 +694,47 @@ __device__ __forceinline__ bool adjacent(const double* cfp, const double* half, 
            fabs(cfp[3 * (size_t)a + 2] - cfp[3 * (size_t)b + 2]) <= reach;
 }
 
+// Direction of colleague q of box p (same level) as 9 (dx+1) + 3 (dy+1) + (dz+1)
+// in 0..26 (13 = p itself); steps are clamped to [-1, 1] (only a grouping key).
+__device__ __forceinline__ int neighbour_block(const int* __restrict__ ic, int p, int q) {
+    int d = 0;
+    for (int a = 0; a < 3; ++a) d = 3 * d + (min(1, max(-1, ic[3 * (size_t)q + a] - ic[3 * (size_t)p + a])) + 1);
+    return d;
+}
+
 // Colleagues and V of every box of one level (AdaptiveOctree.hpp:143-161):
 // children of the parent's colleagues, colleague order then octant order.
 // One wave per box: lane k takes candidate k = 8*slot + octant (<= 216), and
 // ballots keep the emitted order (= ascending id = Morton order in a level).
+// vsig[b] has bit 26 - d set when some V entry of b is a child of the
+// parent's colleague in direction d (the M2L grouping key).
+//
+// The same pass carries, level by level, the two upward-looking sets the U,
+// W and X lists are built from (AdaptiveOctree.hpp:163-189):
+//  * A(b) = the leaves of b's level or coarser that are adjacent to b: a leaf
+//    colleague of b (b itself when b is a leaf), or an entry of A(parent)
+//    adjacent to b. At most one such leaf covers each of the 27 cells around
+//    b, so |A(b)| <= 27. For a leaf b, A(b) is the coarser and same-level part
+//    of U(b).
+//  * X(b) = the entries of A(parent) not adjacent to b. An entry c of
+//    A(parent) is a leaf colleague of an ancestor A' of b, and every box
+//    between A' and b's parent is adjacent to c: descendW from c enters them
+//    (adjacent non-leaves) and stops at b, which it puts in W(c) because b is
+//    separated from c. So X(b) is exactly {c : b in W(c)}, the reference's X
+//    (:172-173), ranked here by reference id, the order it is filled in.
+// The finer part of U and all of W are the transposes: b in U(c) for every
+// leaf b with c in A(b) coarser than b, and b in W(c) for every c in X(b).
 __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restrict__ roots,
                                               const int* __restrict__ bsys, const int* __restrict__ parent,
-                                              const int* __restrict__ child_first,
+                                              const int* __restrict__ child_first, const int* __restrict__ nchild,
                                               const unsigned char* __restrict__ mask, const double* __restrict__ cfp,
                                               const double* __restrict__ half, const int* __restrict__ ic,
+                                              const int* __restrict__ refid,
                                               int* __restrict__ coll, int* __restrict__ ncoll, int* __restrict__ vl,
-                                              int* __restrict__ nv, int* __restrict__ overflow) {
+                                              int* __restrict__ nv, int* __restrict__ vsig, int* __restrict__ aset,
+                                              int* __restrict__ acnt, int* __restrict__ xset, int* __restrict__ xcnt,
+                                              int* __restrict__ overflow) {
+    __shared__ int xbox[kCollSlots], xkey[kCollSlots];
     const int b = off + blockIdx.x;
     const int lane = threadIdx.x;
     const int par = parent[b];
@@ -689,27 +743,55 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
             coll[kCollSlots * (size_t)b] = b;
             ncoll[b] = 1;
             nv[b] = 0;
+            vsig[b] = 0;
+            const bool leaf = nchild[b] == 0;
+            if (leaf) aset[kCollSlots * (size_t)b] = b;
+            acnt[b] = leaf ? 1 : 0;
+            xcnt[b] = 0;
         }
         return;
     }
     const double tol = roots[bsys[b]].tol;
     const int npc = ncoll[par];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // A(parent) -> A(b) or X(b)
+    int ka = 0, kx = 0;
+    {
+        const int npa = acnt[par];
+        int c = -1;
+        bool adj = false;
+        if (lane < npa) {
+            c = aset[kCollSlots * (size_t)par + lane];
+            adj = adjacent(cfp, half, tol, c, b);
+        }
+        const uint64_t ba = __ballot(c >= 0 && adj), bx = __ballot(c >= 0 && !adj);
+        if (c >= 0 && adj) aset[kCollSlots * (size_t)b + __popcll(ba & lt)] = c;  // < npa <= 27
+        if (c >= 0 && !adj) {
+            xbox[__popcll(bx & lt)] = c;
+            xkey[__popcll(bx & lt)] = refid[c];
+        }
+        ka = __popcll(ba);
+        kx = __popcll(bx);
+    }
     int kc = 0, kv = 0;
+    unsigned sig = 0;
     for (int base = 0; base < 8 * npc; base += 64) {
         const int k = base + lane;
         const int slot = k >> 3, o = k & 7;
         int c = -1;
         bool adj = false;
+        int pc = -1;
         if (slot < npc) {
-            const int pc = coll[kCollSlots * (size_t)par + slot];
+            pc = coll[kCollSlots * (size_t)par + slot];
             const unsigned mk = mask[pc];
             if ((mk >> o) & 1u) {
                 c = child_first[pc] + __popc(mk & ((1u << o) - 1u));
                 adj = adjacent(cfp, half, tol, b, c);
             }
         }
-        const uint64_t bc = __ballot(c >= 0 && adj), bv = __ballot(c >= 0 && !adj);
+        if (c >= 0 && !adj) sig |= 1u << (26 - neighbour_block(ic, par, pc));
+        const bool leafc = c >= 0 && adj && nchild[c] == 0;
+        const uint64_t bc = __ballot(c >= 0 && adj), bv = __ballot(c >= 0 && !adj), bl = __ballot(leafc);
         if (c >= 0 && adj) {
             const int pos = kc + __popcll(bc & lt);
             if (pos < kCollSlots) coll[kCollSlots * (size_t)b + pos] = c;
@@ -722,189 +804,146 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
                       dz = ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2];
             if (dx < -3 || dx > 3 || dy < -3 || dy > 3 || dz < -3 || dz > 3) atomicOr(overflow, 2);
         }
+        if (leafc) {
+            const int pos = ka + __popcll(bl & lt);
+            if (pos < kCollSlots) aset[kCollSlots * (size_t)b + pos] = c;
+            else atomicOr(overflow, 1);
+        }
         kc += __popcll(bc);
         kv += __popcll(bv);
+        ka += __popcll(bl);
+    }
+    for (int m = 32; m >= 1; m >>= 1) sig |= __shfl_xor(sig, m, 64);
+    // X(b) ranked by reference id (ids are distinct)
+    __syncthreads();
+    if (lane < kx) {
+        const int key = xkey[lane];
+        int r = 0;
+        for (int j = 0; j < kx; ++j) r += xkey[j] < key;
+        xset[kCollSlots * (size_t)b + r] = xbox[lane];
     }
     if (lane == 0) {
         ncoll[b] = kc < kCollSlots ? kc : kCollSlots;
         nv[b] = kv < kVSlots ? kv : kVSlots;
+        vsig[b] = (int)sig;
+        acnt[b] = ka < kCollSlots ? ka : kCollSlots;
+        xcnt[b] = kx;
     }
 }
 
-// Next box in DFS order after finishing `node`'s subtree, or -1 on reaching `stop`.
-__device__ __forceinline__ int dfs_next(int node, int stop, const int* parent, const int* child_first,
-                                        const int* nchild) {
-    while (node != stop) {
-        const int par = parent[node];
-        if (node + 1 < child_first[par] + nchild[par]) return node + 1;
-        node = par;
-    }
-    return -1;
-}
-
-__device__ __forceinline__ int wave_excl_scan(int v, int lane) {
-    int incl = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += t;
-    }
-    return incl - v;
-}
 __device__ __forceinline__ int wave_total(int v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     return v;
 }
 
-// U and W of each leaf b (AdaptiveOctree.hpp:163-169, :176-189), one wave per
-// leaf. U = every non-empty leaf adjacent to b: a coarser or same-level one is a
-// leaf colleague of b's ancestor at its own level (it contains the ancestor's
-// neighbourhood cell touching b); a finer one lies under a non-leaf colleague of
-// b. Lanes split the (ancestor, colleague slot) candidates; lane s descends
-// non-leaf colleague s the reference's way (descendW: a separated box goes to W
-// and is not entered, an adjacent leaf is a finer U entry, an adjacent non-leaf
-// is entered, octant order). W keeps colleague order then DFS order through a
-// wave prefix sum; U is then ranked by first body, which for disjoint leaves
-// is the reference's DFS-from-root order. mode 0 counts, mode 1 writes.
-constexpr int kUWMaxAnc = kMaxDepth + 1;
-constexpr int kUWSortCap = 1024;
+// ---- U, W and X from the A and X sets -------------------------------------------
+// List sizes, one thread per box: |U(b)| = |A(b)| plus one for every finer
+// leaf that has b in its A set; |W(c)| = the boxes whose X holds c. Integer
+// atomics: the counts do not depend on the order.
+__global__ void __launch_bounds__(kT) k_list_count(int nb, const int* __restrict__ nchild,
+                                                  const int* __restrict__ level, const int* __restrict__ leaf_scan,
+                                                  const int* __restrict__ aset, const int* __restrict__ acnt,
+                                                  const int* __restrict__ xset, const int* __restrict__ xcnt,
+                                                  int* __restrict__ ucnt, int* __restrict__ wcnt) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    if (nchild[b] == 0) {
+        const int na = acnt[b], lb = level[b];
+        atomicAdd(&ucnt[leaf_scan[b]], na);
+        for (int k = 0; k < na; ++k) {
+            const int c = aset[kCollSlots * (size_t)b + k];
+            if (level[c] < lb) atomicAdd(&ucnt[leaf_scan[c]], 1);
+        }
+    }
+    const int nx = xcnt[b];
+    for (int k = 0; k < nx; ++k) atomicAdd(&wcnt[leaf_scan[xset[kCollSlots * (size_t)b + k]]], 1);
+}
 
-// MODE 0 counts, MODE 1 fills (only the fill pass needs the sort buffers in
-// LDS; the counting pass keeps its footprint small for occupancy).
-template <int mode>
-__global__ void __launch_bounds__(64) k_uwlist(int nleaves, const int* __restrict__ leaf_ids,
-                                              const RootInfo* __restrict__ roots, const int* __restrict__ bsys,
-                                              const int* __restrict__ parent, const int* __restrict__ child_first,
-                                              const int* __restrict__ nchild, const int* __restrict__ seg_first,
-                                              const int* __restrict__ seg_count,
-                                              const double* __restrict__ cfp, const double* __restrict__ half,
-                                              const int* __restrict__ coll, const int* __restrict__ ncoll,
-                                              int* __restrict__ ucnt, int* __restrict__ wcnt,
-                                              const int* __restrict__ uoff, const int* __restrict__ woff,
-                                              int* __restrict__ uidx, int* __restrict__ widx,
-                                              int* __restrict__ usrc, int* __restrict__ lane_cnt) {
-    __shared__ int anc[kUWMaxAnc], cum[kUWMaxAnc + 1];
-    __shared__ int nanc;
-    __shared__ int ubuf[mode ? kUWSortCap : 1];
-    const int li = blockIdx.x;
-    const int b = leaf_ids[li];
-    const int lane = threadIdx.x;
-    const double tol = roots[bsys[b]].tol;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    if (lane == 0) {
-        int k = 0, a = b, run = 0;
-        for (;;) {
-            anc[k] = a;
-            cum[k] = run;
-            run += ncoll[a];
-            ++k;
-            if (parent[a] < 0) break;
-            a = parent[a];
-        }
-        cum[k] = run;
-        nanc = k;
-    }
-    __syncthreads();
-    const int ncand = cum[nanc];
-    int* ud = mode ? uidx + uoff[li] : nullptr;
-    int* wd = mode ? widx + woff[li] : nullptr;
-    // coarser and same-level adjacent leaves
-    int ku = 0;
-    for (int base = 0; base < ncand; base += 64) {
-        const int q = base + lane;
-        int c = -1;
-        if (q < ncand) {
-            int k = 0;
-            while (cum[k + 1] <= q) ++k;
-            c = coll[kCollSlots * (size_t)anc[k] + (q - cum[k])];
-            if (!(nchild[c] == 0 && adjacent(cfp, half, tol, c, b))) c = -1;
-        }
-        const uint64_t bal = __ballot(c >= 0);
-        if (mode && c >= 0) ud[ku + __popcll(bal & lt)] = c;
-        ku += __popcll(bal);
-    }
-    // finer U and W: lane s descends non-leaf colleague s
-    const int nc = ncoll[b];
-    int myu = 0, myw = 0;
-    int c = -1;
-    if (lane < nc) {
-        c = coll[kCollSlots * (size_t)b + lane];
-        if (c == b || nchild[c] == 0) c = -1;
-    }
-    // the counting launch keeps each lane's counts, so the filling launch
-    // descends once
-    int* lc = lane_cnt + (size_t)li * 2 * kCollSlots;
-    if (mode && lane < kCollSlots) {
-        myu = lc[lane];
-        myw = lc[kCollSlots + lane];
-    }
-    for (int pass = mode ? 1 : 0; pass < (mode ? 2 : 1); ++pass) {
-        int uo = 0, wo = 0;
-        if (pass == 1) {
-            uo = ku + wave_excl_scan(myu, lane);
-            wo = wave_excl_scan(myw, lane);
-        }
-        int nu_ = 0, nw_ = 0;
-        if (c >= 0) {
-            int node = child_first[c];
-            while (node >= 0) {
-                if (!adjacent(cfp, half, tol, node, b)) {
-                    if (pass == 1) wd[wo + nw_] = node;
-                    ++nw_;
-                    node = dfs_next(node, c, parent, child_first, nchild);
-                } else if (nchild[node] > 0) {
-                    node = child_first[node];
-                } else {
-                    if (pass == 1) ud[uo + nu_] = node;
-                    ++nu_;
-                    node = dfs_next(node, c, parent, child_first, nchild);
-                }
+// Fills the lists, one thread per box: U(b) starts with A(b); the transposed
+// entries (finer U, W) take slots from per-list cursors (zeroed before), in
+// an order k_list_sort then fixes. X(b) is copied in its ranked order.
+__global__ void __launch_bounds__(kT) k_list_fill(int nb, const int* __restrict__ nchild,
+                                                 const int* __restrict__ level, const int* __restrict__ leaf_scan,
+                                                 const int* __restrict__ aset, const int* __restrict__ acnt,
+                                                 const int* __restrict__ xset, const int* __restrict__ xcnt,
+                                                 const int* __restrict__ uoff, const int* __restrict__ woff,
+                                                 const int* __restrict__ xoff, int* __restrict__ ucur,
+                                                 int* __restrict__ wcur, int* __restrict__ uidx,
+                                                 int* __restrict__ widx, int* __restrict__ xidx) {
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    if (nchild[b] == 0) {
+        const int na = acnt[b], lb = level[b];
+        int* ud = uidx + uoff[leaf_scan[b]];
+        for (int k = 0; k < na; ++k) {
+            const int c = aset[kCollSlots * (size_t)b + k];
+            ud[k] = c;
+            if (level[c] < lb) {
+                const int lc = leaf_scan[c];
+                uidx[uoff[lc] + acnt[c] + atomicAdd(&ucur[lc], 1)] = b;
             }
         }
-        myu = nu_;
-        myw = nw_;
     }
-    const int totu = ku + wave_total(myu), totw = wave_total(myw);
-    if (!mode) {
-        if (lane < kCollSlots) {
-            lc[lane] = myu;
-            lc[kCollSlots + lane] = myw;
-        }
-        if (lane == 0) {
-            ucnt[li] = totu;
-            wcnt[li] = totw;
-        }
-        return;
+    const int nx = xcnt[b];
+    int* xd = xidx + xoff[b];
+    for (int k = 0; k < nx; ++k) {
+        const int c = xset[kCollSlots * (size_t)b + k];
+        xd[k] = c;
+        const int lc = leaf_scan[c];
+        widx[woff[lc] + atomicAdd(&wcur[lc], 1)] = b;
     }
-    // U in ascending first body (keys are distinct: disjoint leaves)
-    __threadfence_block();
-    __syncthreads();
-    // bodies in the U leaves (the P2P source count of this leaf; order-free)
+}
+
+// Puts U and W of each leaf in the reference's order, one wave per leaf: U
+// in DFS order from the root, W in colleague order then DFS order
+// (AdaptiveOctree.hpp:163-189). The entries of each list are disjoint boxes,
+// whose DFS order is the order of their first bodies, and colleagues come in
+// Morton order, which is again body order, so both are ranks by first body.
+// Also the P2P source count of the leaf (bodies in its U leaves).
+constexpr int kListSortCap = 512;
+__device__ void sort_by_first_body(int* __restrict__ d, int n, const int* __restrict__ seg_first, int* box,
+                                   int* key, int lane) {
+    if (n <= 1) return;
+    if (n <= kListSortCap) {
+        for (int i = lane; i < n; i += 64) {
+            box[i] = d[i];
+            key[i] = seg_first[box[i]];
+        }
+        __syncthreads();
+        for (int i = lane; i < n; i += 64) {
+            const int k = key[i];
+            int r = 0;
+            for (int j = 0; j < n; ++j) r += key[j] < k;
+            d[r] = box[i];
+        }
+        __syncthreads();
+    } else if (lane == 0) {
+        for (int i = 1; i < n; ++i) {
+            const int v = d[i], k = seg_first[v];
+            int j = i - 1;
+            while (j >= 0 && seg_first[d[j]] > k) {
+                d[j + 1] = d[j];
+                --j;
+            }
+            d[j + 1] = v;
+        }
+    }
+}
+__global__ void __launch_bounds__(64) k_list_sort(const int* __restrict__ uoff, const int* __restrict__ woff,
+                                                 const int* __restrict__ seg_first,
+                                                 const int* __restrict__ seg_count, int* __restrict__ uidx,
+                                                 int* __restrict__ widx, int* __restrict__ usrc) {
+    __shared__ int box[kListSortCap], key[kListSortCap];
+    const int li = blockIdx.x, lane = threadIdx.x;
+    const int u0 = uoff[li], nu = uoff[li + 1] - u0;
     int ns = 0;
-    for (int i = lane; i < totu; i += 64) ns += seg_count[ud[i]];
+    for (int i = lane; i < nu; i += 64) ns += seg_count[uidx[u0 + i]];
     ns = wave_total(ns);
     if (lane == 0) usrc[li] = ns;
-    if (totu <= kUWSortCap) {
-        for (int i = lane; i < totu; i += 64) ubuf[i] = ud[i];
-        __syncthreads();
-        for (int i = lane; i < totu; i += 64) {
-            const int v = ubuf[i], key = seg_first[v];
-            int r = 0;
-            for (int k2 = 0; k2 < totu; ++k2) r += seg_first[ubuf[k2]] < key;
-            ud[r] = v;
-        }
-    } else if (lane == 0) {
-        for (int i = 1; i < totu; ++i) {
-            const int v = ud[i], key = seg_first[v];
-            int j2 = i - 1;
-            while (j2 >= 0 && seg_first[ud[j2]] > key) {
-                ud[j2 + 1] = ud[j2];
-                --j2;
-            }
-            ud[j2 + 1] = v;
-        }
-    }
+    sort_by_first_body(uidx + u0, nu, seg_first, box, key, lane);
+    const int w0 = woff[li];
+    sort_by_first_body(widx + w0, woff[li + 1] - w0, seg_first, box, key, lane);
 }
 
 // The three list sizes and the slot-overflow flag, for one readback.
@@ -963,96 +1002,6 @@ __global__ void __launch_bounds__(kTotThreads) k_list_totals(int nb, int nleaves
         }
         if (threadIdx.x == 2) atomicMax(&tot[2], t);
         else atomicAdd(&tot[threadIdx.x], t);
-    }
-}
-
-// X of every box w (AdaptiveOctree.hpp:172-173), one wave per box, without
-// transposing W: w is in W(c) exactly when c is a leaf colleague of a proper
-// ancestor A of w (c != A), c is adjacent to parent(w) and c is not adjacent
-// to w (descendW enters only boxes adjacent to c, and a box adjacent to
-// parent(w) is adjacent to every ancestor of it). Lanes split the (ancestor,
-// colleague slot) candidates; each list is then ranked by reference id, the
-// reference's order (X filled from `leaves` in ascending id). mode 0 counts,
-// mode 1 writes.
-constexpr int kXSortCap = 1024;
-template <int mode>
-__global__ void __launch_bounds__(64) k_xlist(int nb, const RootInfo* __restrict__ roots, const int* __restrict__ bsys,
-                                             const int* __restrict__ parent, const int* __restrict__ nchild,
-                                             const int* __restrict__ refid, const double* __restrict__ cfp,
-                                             const double* __restrict__ half, const int* __restrict__ coll,
-                                             const int* __restrict__ ncoll, int* __restrict__ xcnt,
-                                             const int* __restrict__ xoff, int* __restrict__ xidx) {
-    __shared__ int anc[kUWMaxAnc], cum[kUWMaxAnc + 1];
-    __shared__ int nanc;
-    __shared__ int xbuf[mode ? kXSortCap : 1];
-    const int w = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int pw = parent[w];
-    if (pw < 0) {
-        if (mode == 0 && lane == 0) xcnt[w] = 0;
-        return;
-    }
-    // the filling launch skips the boxes the counting launch found without X
-    // (most of them: X is non-empty only at refinement boundaries)
-    if (mode && xoff[w + 1] == xoff[w]) return;
-    const double tol = roots[bsys[w]].tol;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    if (lane == 0) {
-        int k = 0, a = pw, run = 0;
-        for (;;) {
-            anc[k] = a;
-            cum[k] = run;
-            run += ncoll[a];
-            ++k;
-            if (parent[a] < 0) break;
-            a = parent[a];
-        }
-        cum[k] = run;
-        nanc = k;
-    }
-    __syncthreads();
-    const int ncand = cum[nanc];
-    int* xd = mode ? xidx + xoff[w] : nullptr;
-    int kx = 0;
-    for (int base = 0; base < ncand; base += 64) {
-        const int q = base + lane;
-        int c = -1;
-        if (q < ncand) {
-            int k = 0;
-            while (cum[k + 1] <= q) ++k;
-            const int A = anc[k];
-            c = coll[kCollSlots * (size_t)A + (q - cum[k])];
-            if (c == A || nchild[c] != 0 || !adjacent(cfp, half, tol, c, pw) || adjacent(cfp, half, tol, c, w)) c = -1;
-        }
-        const uint64_t bal = __ballot(c >= 0);
-        if (mode && c >= 0) xd[kx + __popcll(bal & lt)] = c;
-        kx += __popcll(bal);
-    }
-    if (!mode) {
-        if (lane == 0) xcnt[w] = kx;
-        return;
-    }
-    __threadfence_block();
-    __syncthreads();
-    if (kx <= kXSortCap) {
-        for (int i = lane; i < kx; i += 64) xbuf[i] = xd[i];
-        __syncthreads();
-        for (int i = lane; i < kx; i += 64) {
-            const int v = xbuf[i], key = refid[v];
-            int r = 0;
-            for (int k2 = 0; k2 < kx; ++k2) r += refid[xbuf[k2]] < key;
-            xd[r] = v;
-        }
-    } else if (lane == 0) {
-        for (int i = 1; i < kx; ++i) {
-            const int v = xd[i], key = refid[v];
-            int j = i - 1;
-            while (j >= 0 && refid[xd[j]] > key) {
-                xd[j + 1] = xd[j];
-                --j;
-            }
-            xd[j + 1] = v;
-        }
     }
 }
 
@@ -1389,12 +1338,23 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     tv.vl = pool.get<int>("vl", kVSlots * (size_t)nb);
     tv.nv = pool.get<int>("nv", nb);
     int* overflow = pool.get<int>("overflow", 1);
+    int* vsig = pool.get<int>("vsig", nb);
+    int* aset = pool.get<int>("aset", kCollSlots * (size_t)nb);
+    int* acnt = pool.get<int>("acnt", nb);
+    int* xset = pool.get<int>("xset", kCollSlots * (size_t)nb);
+    int* xcnt = pool.get<int>("xcnt", nb + 1);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, overflow, 1, 0);
     for (int lv = 0; lv <= tv.max_level; ++lv) {
         const int off = meta.level_off[lv], cnt = meta.level_off[lv + 1] - off;
         hipLaunchKernelGGL(k_coll_v, dim3(cnt), dim3(64), 0, s, off, d_roots, tv.bsys, tv.parent, tv.child_first,
-                           tv.child_mask, tv.cfp, tv.half, tv.ic, tv.coll, tv.ncoll, tv.vl, tv.nv, overflow);
+                           tv.nchild, tv.child_mask, tv.cfp, tv.half, tv.ic, tv.refid, tv.coll, tv.ncoll, tv.vl,
+                           tv.nv, vsig, aset, acnt, xset, xcnt, overflow);
     }
+    // (the 128-target tiles cut the same octant groups, so each holds targets
+    // of at most two neighbouring 256-target tiles)
+    if (tv.m2l_ntiles256 > 0)
+        hipLaunchKernelGGL(k_m2l_tile_sort, dim3(tv.m2l_ntiles256), dim3(256), 0, s, tv.m2l_tiles256, vsig,
+                           tv.m2l_order);
     // (the overflow flag is read back with the list sizes below; list counts
     // are capped at their slots, so the kernels in between stay in bounds)
     // (the list stream waits for the colleague lists before on_vlists enqueues
@@ -1405,25 +1365,22 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     }
     if (on_vlists) on_vlists(tv, meta);
 
-    // ---- U and W per leaf, X per box (count, scan, one size readback, fill)
+    // ---- U and W per leaf, X per box (count, scan, one size readback, fill, order)
     if (ls && ls != s) s = ls;
     const int nl = tv.nleaves;
     int* ucnt = pool.get<int>("ucnt", nl + 1);
     int* wcnt = pool.get<int>("wcnt", nl + 1);
+    int* ucur = pool.get<int>("ucur", nl + 1);
+    int* wcur = pool.get<int>("wcur", nl + 1);
+    int* usrc = pool.get<int>("usrc", nl + 1);
     tv.u_off = pool.get<int>("u_off", nl + 1);
     tv.w_off = pool.get<int>("w_off", nl + 1);
-    int* uw_lane_cnt = pool.get<int>("uw_lane_cnt", 2 * kCollSlots * (size_t)nl);
-    hipLaunchKernelGGL(k_uwlist<0>, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
-                       tv.child_first,
-                       tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, ucnt, wcnt, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, uw_lane_cnt);
+    tv.x_off = pool.get<int>("x_off", nb + 1);
+    for (int* a : {ucnt, wcnt, ucur, wcur}) hipLaunchKernelGGL(k_fill, dim3(blocks(nl)), dim3(kT), 0, s, a, nl, 0);
+    hipLaunchKernelGGL(k_list_count, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.level, fscan, aset, acnt,
+                       xset, xcnt, ucnt, wcnt);
     scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
     scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp + scan_tmp_ints(scan_cap), s);
-    // X counts too, so that one round trip returns all three list sizes
-    int* xcnt = pool.get<int>("xcnt", nb + 1);
-    tv.x_off = pool.get<int>("x_off", nb + 1);
-    hipLaunchKernelGGL(k_xlist<0>, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid,
-                       tv.cfp, tv.half, tv.coll, tv.ncoll, xcnt, nullptr, nullptr);
     scan_exclusive(xcnt, tv.x_off, nb, tv.x_off + nb, scan_tmp, s);
     {
         int tot[4];
@@ -1441,15 +1398,12 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     tv.u_idx = pool.get<int>("u_idx", tv.n_u);
     tv.w_idx = pool.get<int>("w_idx", tv.n_w);
     tv.x_idx = pool.get<int>("x_idx", tv.n_x);
-    hipLaunchKernelGGL(k_uwlist<1>, dim3(nl), dim3(64), 0, s, nl, tv.leaf_ids, d_roots, tv.bsys, tv.parent,
-                       tv.child_first,
-                       tv.nchild, tv.seg_first, tv.seg_count, tv.cfp, tv.half, tv.coll, tv.ncoll, nullptr, nullptr, tv.u_off,
-                       tv.w_off, tv.u_idx, tv.w_idx, ucnt, uw_lane_cnt);
+    hipLaunchKernelGGL(k_list_fill, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.level, fscan, aset, acnt,
+                       xset, xcnt, tv.u_off, tv.w_off, tv.x_off, ucur, wcur, tv.u_idx, tv.w_idx, tv.x_idx);
+    hipLaunchKernelGGL(k_list_sort, dim3(nl), dim3(64), 0, s, tv.u_off, tv.w_off, tv.seg_first, tv.seg_count,
+                       tv.u_idx, tv.w_idx, usrc);
     hipLaunchKernelGGL(k_list_totals, dim3(blocks(std::max(nb, nl), kTotThreads)), dim3(kTotThreads), 0, s, nb, nl,
-                       tv.ncoll, tv.nv,
-                       tv.leaf_ids, tv.seg_count, ucnt, tv.list_totals);
-    hipLaunchKernelGGL(k_xlist<1>, dim3(nb), dim3(64), 0, s, nb, d_roots, tv.bsys, tv.parent, tv.nchild, tv.refid,
-                       tv.cfp, tv.half, tv.coll, tv.ncoll, nullptr, tv.x_off, tv.x_idx);
+                       tv.ncoll, tv.nv, tv.leaf_ids, tv.seg_count, usrc, tv.list_totals);
     AFMM_CHECK(hipGetLastError());
 }
 
