@@ -286,13 +286,15 @@ def batched_leg(afmm, ics, tune, args):
     P, E = d_pos.ptr, d_mass.ptr
     A = d_acc.ptr
     cfg, tune_log = tune.tune(afmm, off, P, E, A, 8, afmm.PRECISION_MIXED_BF16, eps=0.01)
-    cap, shapes = cfg["leaf_cap"], cfg["shapes"]
+    cap, shapes, overlap = cfg["leaf_cap"], cfg["shapes"], cfg["overlap"]
 
-    def timed_batch(precision, leaf_cap, reps=3, graph=False, sh=None):
+    def timed_batch(precision, leaf_cap, reps=3, graph=False, sh=None, ov=None):
         s = afmm.AdaptiveFMM3D(eps=0.01, p=8, threshold=leaf_cap, precision=precision)
         s.set_graph(graph)
         if sh:
             s.set_shapes(**sh)
+        if ov is not None:
+            s.set_overlap(ov)
         s.compute_batch_device(off, P, E, A)
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -300,7 +302,7 @@ def batched_leg(afmm, ics, tune, args):
         return (time.perf_counter() - t0) / reps, s.stats()
 
     t_mix, st = timed_batch(afmm.PRECISION_MIXED_BF16, cap, graph=True, sh=shapes)
-    t_mix_stream, st_stream = timed_batch(afmm.PRECISION_MIXED_BF16, cap, sh=shapes)
+    t_mix_stream, st_stream = timed_batch(afmm.PRECISION_MIXED_BF16, cap, sh=shapes, ov=overlap)
     t_mix_default, _ = timed_batch(afmm.PRECISION_MIXED_BF16, cap)
     t_b64, _ = timed_batch(afmm.PRECISION_FP64, 64)
     # 256 sequential plain solves (FP64, cap 64), device-resident slices
@@ -313,9 +315,9 @@ def batched_leg(afmm, ics, tune, args):
     t_seq = time.perf_counter() - t0
     inter = float(np.sum(ns.astype(np.float64) * (ns - 1)))
     return {"systems": int(len(ns)), "bodies": int(off[-1]), "p": 8, "eps": 0.01,
-            "leaf_cap": int(cap), "shapes": shapes,
+            "leaf_cap": int(cap), "shapes": shapes, "overlap_stream_launch": int(overlap),
             "autotune_ms_total_by_cap": {str(k): round(v, 3) for k, v in tune_log["leaf_cap"].items()},
-            "autotune_shape_trials": tune_log["shapes"],
+            "autotune_shape_trials": tune_log["shapes"], "autotune_overlap_trials": tune_log["overlap"],
             "batched_mixed_default_shapes_s_per_step": t_mix_default,
             "phases_ms_stream_launch": {k[3:]: st_stream[k] for k in ("ms_tree", "ms_lists", "ms_upward", "ms_m2l",
                                                                       "ms_downward", "ms_leaf", "ms_p2p", "ms_total")},

@@ -301,9 +301,9 @@ int afmm_set_graph(afmm_ctx* ctx, int enable);
  * schedule also leave the results' bits unchanged, while the P2P lane grouping
  * changes the order of each near-field sum (same accuracy, different last bits). */
 typedef struct {
-    int32_t m2l_waves;        /* waves per M2L workgroup, 16 targets each: 8 or 16 (default 16 at
-                                 p <= 8, or p <= 7 with the bf16 M2L, else 8; 16 needs p <= 8 or
-                                 the bf16 M2L) */
+    int32_t m2l_waves;        /* waves per M2L workgroup, 16 targets each: 8 or 16 (default 16 with
+                                 the FP64 M2L at p <= 8 and the bf16 M2L at p <= 7, else 8; 16
+                                 needs p <= 8 or the bf16 M2L; split FP32 defaults to 8) */
     int32_t p2p_pair_bodies;  /* leaves with more bodies give each P2P lane two targets (default 32) */
     int32_t p2p_min_group;    /* fewest lanes sharing one target's sources: 1, 2, 4, ..., 64 (default 8) */
     int32_t p2p_schedule;     /* 1: persistent workgroups over the leaves sorted by work bin, largest
@@ -315,13 +315,24 @@ int afmm_set_shapes(afmm_ctx* ctx, const afmm_shapes* shapes);
 int afmm_get_shapes(const afmm_ctx* ctx, afmm_shapes* out);
 
 /* Two-stream overlap (no counterpart in the reference, which has one host
- * thread and one stream): when enabled (the default), an evaluation runs the
- * upward pass beside the interaction-list build and the P2P near field beside
- * the far-field passes on a second stream shared by the process's contexts on
- * that device (in graph mode the captured P2P branch uses the context's own). Disabled, everything runs in order on the
- * context's stream (afmm_stats.ms_p2p then times P2P alone). Results are the
- * same bits either way. */
-int afmm_set_overlap(afmm_ctx* ctx, int enable);
+ * thread and one stream). Modes:
+ *   AFMM_OVERLAP_OFF: everything in order on the context's stream
+ *     (afmm_stats.ms_p2p then times P2P alone);
+ *   AFMM_OVERLAP_AUTO (the default): a second stream, shared by the process's
+ *     contexts on that device, runs the upward pass beside the interaction-list
+ *     build and P2P beside the far-field passes; with the bf16 or split-FP32
+ *     M2L, and the FP64 one from p = 14 with 8-wave workgroups
+ *     (afmm_shapes.m2l_waves), the U/W/X lists, P2L, M2P and P2P run there
+ *     beside M2L as well (their VALU work shares the SIMDs with its MFMAs);
+ *   AFMM_OVERLAP_BESIDE_M2L: the latter schedule whatever the M2L shape;
+ *   AFMM_OVERLAP_IN_ORDER: the former (lists, P2L and M2P in order before M2L).
+ * In graph mode the captured P2P branch uses the context's own second stream
+ * and the lists stay in order. Results are the same bits in every mode. */
+#define AFMM_OVERLAP_OFF 0
+#define AFMM_OVERLAP_AUTO 1
+#define AFMM_OVERLAP_BESIDE_M2L 2
+#define AFMM_OVERLAP_IN_ORDER 3
+int afmm_set_overlap(afmm_ctx* ctx, int mode);
 
 /* ---- initial conditions (host) -------------------------------------------
  * makePlummerSphere (InitialConditions3D.hpp:100-152): n equal-mass bodies of

@@ -29,6 +29,8 @@ PRECISION_MIXED_BF16 = 2
 # P2P, and FP64 M2L with FP32 P2P
 PRECISION_M2L_BF16 = 3
 PRECISION_P2P_FP32 = 4
+# afmm_set_overlap modes
+OVERLAP_OFF, OVERLAP_AUTO, OVERLAP_BESIDE_M2L, OVERLAP_IN_ORDER = 0, 1, 2, 3
 
 AFMM_ERR_DEVICE = -5
 AFMM_ERR_NO_MEMORY = -6
@@ -313,10 +315,15 @@ class AdaptiveFMM3D:
         return {k: getattr(sh, k) for k, _ in Shapes._fields_}
 
     def set_overlap(self, enable=True):
-        """Two-stream overlap (default on): the upward pass beside the list
-        build and P2P beside the far-field passes. Off: one stream, in order.
-        Results are the same bits either way."""
-        self._check(self.lib.afmm_set_overlap(self.ctx, 1 if enable else 0))
+        """Two-stream overlap (afmm_set_overlap): True or OVERLAP_AUTO (the
+        default: the upward pass beside the list build, P2P beside the
+        far-field passes, and with the bf16 or split-FP32 M2L, or FP64 from
+        p = 14, the lists, P2L, M2P and P2P beside M2L), OVERLAP_BESIDE_M2L,
+        OVERLAP_IN_ORDER, or
+        False / OVERLAP_OFF (one stream, in order). Results are the same bits
+        in every mode."""
+        mode = OVERLAP_AUTO if enable is True else OVERLAP_OFF if enable is False else int(enable)
+        self._check(self.lib.afmm_set_overlap(self.ctx, mode))
 
     def stats(self):
         st = Stats()

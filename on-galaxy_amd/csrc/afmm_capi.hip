@@ -90,16 +90,22 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         AFMM_CHECK(hipEventCreateWithFlags(&c->up_done, hipEventDisableTiming));
         c->side = shared_side_stream();
     }
-    // With the bf16 M2L (outside graph mode, overlap on), M2L is enqueued on s
-    // right after its preparation (behind the V lists), and the U, W and X
-    // lists, P2L, M2P and P2P run on the second stream beside it, after the
-    // upward pass; the lists' size readback then synchronises the second
-    // stream only, while M2L keeps the GPU busy. The bf16 M2L leaves room in
-    // the register files for those waves (2 waves per SIMD at 216 VGPRs); the
-    // FP64 and split-FP32 M2L fill them (4 waves at 124-128 VGPRs), so beside
-    // them the second stream's kernels only run where M2L tiles have finished,
-    // and those forms keep the lists and the X/W passes in order on s.
-    const bool lists_beside = split && !c->use_graph && m2l_form(c->prm.precision) == kPrecMixedBf16;
+    // Beside M2L (outside graph mode, overlap on): M2L is enqueued on s right
+    // after its preparation (behind the V lists), and the U, W and X lists,
+    // P2L, M2P and P2P run on the second stream beside it, after the upward
+    // pass: their VALU work shares the SIMDs with the MFMA-bound M2L, and the
+    // lists' size readback synchronises the second stream only. That needs
+    // an M2L that leaves registers free (8-wave workgroups, 142-216 VGPRs at 2
+    // waves per SIMD; the 16-wave FP64 M2L holds 4 waves of 124 VGPRs). The
+    // automatic choice, from the 1M merger: beside for the bf16 (p = 8: 4.60
+    // -> 4.46 ms, p = 12: 15.9 -> 15.2 ms) and split-FP32 (7.75 -> 7.54 ms)
+    // forms and for FP64 from p = 14 (52 -> 48.6 ms); FP64 in order at p = 8
+    // (equal either way) and p = 12 (24.5 ms against 27.9 ms beside).
+    const bool bf16_m2l = m2l_form(c->prm.precision) == kPrecMixedBf16;
+    const bool auto_beside = m2l_form(c->prm.precision) != kPrecFP64 || (c->prm.p >= 14 && c->shapes.m2l_waves == 8);
+    const bool lists_beside = split && !c->use_graph &&
+                              (c->overlap_mode == AFMM_OVERLAP_BESIDE_M2L ||
+                               (c->overlap_mode == AFMM_OVERLAP_AUTO && auto_beside));
     if (lists_beside && !c->list_fork) {
         AFMM_CHECK(hipEventCreateWithFlags(&c->list_fork, hipEventDisableTiming));
         AFMM_CHECK(hipEventCreateWithFlags(&c->lists_done, hipEventDisableTiming));
@@ -143,7 +149,7 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         AFMM_CHECK(hipEventRecord(c->lists_done, c->side));
         // P2L and M2P (needed first, by L2L and L2P on s), then P2P
         run_far_passes(c->pool, c->side, tv, c->prm.p, phi != nullptr, c->x_done, c->w_done);
-        if (m2l_form(c->prm.precision) == kPrecMixedBf16) AFMM_CHECK(hipStreamWaitEvent(c->side, c->m2l_done, 0));
+        if (bf16_m2l) AFMM_CHECK(hipStreamWaitEvent(c->side, c->m2l_done, 0));
         run_p2p_near(c->pool, c->side, tv, c->prm.precision, c->shapes, phi != nullptr, c->ev[8], c->ev[9]);
         AFMM_CHECK(hipEventRecord(c->side_join, c->side));
     } else {
@@ -352,9 +358,12 @@ int afmm_set_graph(afmm_ctx* c, int enable) {
     return AFMM_OK;
 }
 
-int afmm_set_overlap(afmm_ctx* c, int enable) {
+int afmm_set_overlap(afmm_ctx* c, int mode) {
     if (!c) return AFMM_ERR_ARG;
-    c->split_p2p = enable != 0;
+    if (mode < AFMM_OVERLAP_OFF || mode > AFMM_OVERLAP_IN_ORDER)
+        return fail(c, AFMM_ERR_ARG, "overlap mode must be 0 (off), 1 (auto), 2 (beside M2L) or 3 (in order)");
+    c->overlap_mode = mode;
+    c->split_p2p = mode != AFMM_OVERLAP_OFF;
     return AFMM_OK;
 }
 

@@ -34,6 +34,7 @@ struct afmm_ctx {
     // far-field passes; off in graph mode and for the request server's
     // workers, which already run several evaluations side by side
     bool split_p2p = true;
+    int overlap_mode = AFMM_OVERLAP_AUTO;  // afmm_set_overlap
     hipStream_t side = nullptr;
     hipEvent_t side_fork = nullptr, side_join = nullptr, up_fork = nullptr, up_done = nullptr;
     // the U, W and X lists on the second stream beside M2L (overlap on, graph off)

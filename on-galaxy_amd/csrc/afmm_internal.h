@@ -119,7 +119,7 @@ struct TreeView {
     int* ic;             // [3B] integer coordinates at the box's level
     int* leaf_ids;       // [nleaves] BFS ids of the leaves
     int* lo_order;       // [B] boxes sorted by (level, octant); level l is [level_off[l], level_off[l+1])
-    int* m2l_order;      // [B] by (octant, level, BFS), then by V signature inside each 256-target tile
+    int* m2l_order;      // [B] boxes sorted by (octant, level): the M2L target order
     // M2L target tiles over m2l_order, cut inside octant groups: tile k is
     // ranks [m2l_tiles{256,128}[k], [k+1]), at most 256 (128) targets each
     int *m2l_tiles256, *m2l_tiles128;
@@ -259,9 +259,14 @@ struct Shapes {
 };
 // The default M2L workgroup of order p and precision mode prec: 16 waves
 // while the accumulators fit four waves per SIMD, else 8 (the bf16 form keeps
-// an FP64 total beside its FP32 accumulator, which needs 8 from p = 8).
+// an FP64 total beside its FP32 accumulator, which needs 8 from p = 8; the
+// split-FP32 form spills at 16 and takes 8 at every order). With 8 waves the
+// M2L leaves registers free on every SIMD for the passes that may run beside it.
 inline int default_m2l_waves(int p, int prec) {
-    return m2l_form(prec) == kPrecMixedBf16 ? (p <= 7 ? 16 : 8) : (p <= 8 ? 16 : 8);
+    const int form = m2l_form(prec);
+    if (form == kPrecMixedBf16) return p <= 7 ? 16 : 8;
+    if (form == kPrecMixed) return 8;
+    return p <= 8 ? 16 : 8;
 }
 void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec, const Shapes& sh);
 // P2M and M2M alone on stream s (events e0/e1 around them when non-null); a

@@ -357,33 +357,42 @@ __global__ void __launch_bounds__(kM2PWaves * 64, m2p_min_waves<P>()) k_m2p(Tree
 // group_size 1, num_groups_per_blk 4).
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(kT) k_rank_of(int nb, const int* __restrict__ order, int* __restrict__ rank) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i < nb) rank[order[i]] = i;
-}
-
-__global__ void __launch_bounds__(kT) k_fill_int(int* a, size_t n, int v) {
-    const size_t i = blockIdx.x * (size_t)kT + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-
-// srcT[offset * np + rank(t)] = source box of target t at that offset; one
-// thread per (box, V slot), so the slot reads are coalesced.
 // The offset slots of afmm_math.h for device code (srcT rows, presence and
 // operators are all kept in slot order).
 __constant__ M2LSlots kM2LSlots = make_m2l_slots();
 
-__global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __restrict__ rank, const int* __restrict__ vl,
-                                               const int* __restrict__ nv, const int* __restrict__ ic,
-                                               int* __restrict__ srcT) {
-    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
-    if (g >= (long long)nb * kVSlots) return;
-    const int b = (int)(g / kVSlots), e = (int)(g - (long long)b * kVSlots);
-    if (e >= nv[b]) return;
-    const int c = vl[g];
-    const int o = offset_index(ic[3 * (size_t)b] - ic[3 * (size_t)c], ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1],
-                               ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2]);
-    srcT[(size_t)kM2LSlots.slot_of[o] * np + rank[b]] = c;
+// srcT[slot * np + r] = the source box of target order[r] at the offset of
+// that slot, -1 where it has none. One block per 16 consecutive targets of
+// the M2L order: their V lists fill a slots x 16 table in LDS (initially -1),
+// which is then written out row by row, 64 contiguous bytes per offset, so
+// the whole table is written once and coalesced (no separate fill of -1).
+constexpr int kSrcTargets = 16;
+__global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __restrict__ order,
+                                               const int* __restrict__ vl, const int* __restrict__ nv,
+                                               const int* __restrict__ ic, int* __restrict__ srcT) {
+    __shared__ int tab[kNumOffsets][kSrcTargets];
+    __shared__ int box[kSrcTargets], cnt[kSrcTargets];
+    const int r0 = blockIdx.x * kSrcTargets, tid = threadIdx.x;
+    const int nt = min(kSrcTargets, nb - r0);
+    for (int e = tid; e < kNumOffsets * kSrcTargets; e += kT) (&tab[0][0])[e] = -1;
+    if (tid < nt) {
+        box[tid] = order[r0 + tid];
+        cnt[tid] = nv[box[tid]];
+    }
+    __syncthreads();
+    for (int w = tid; w < nt * kVSlots; w += kT) {
+        const int j = w / kVSlots, e = w - j * kVSlots;
+        if (e >= cnt[j]) continue;
+        const int b = box[j], c = vl[kVSlots * (size_t)b + e];
+        const int o = offset_index(ic[3 * (size_t)b] - ic[3 * (size_t)c], ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1],
+                                   ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2]);
+        tab[kM2LSlots.slot_of[o]][j] = c;
+    }
+    __syncthreads();
+    for (int e = tid; e < kNumOffsets * kSrcTargets; e += kT) {
+        const int k = e / kSrcTargets, j = e - k * kSrcTargets;
+        if (j < nt) srcT[(size_t)k * np + r0 + j] = tab[k][j];
+    }
 }
 
 // Waves per M2L workgroup (W, a kernel shape the autotuner may choose): by
@@ -1311,17 +1320,14 @@ const int* p2p_order(Pool& pool, hipStream_t s, const TreeView& t, bool reserve_
     return sb.val[radix_sort(sb, nl, 0, 1, s)];
 }
 
-// M2L preparation: target ranks in the M2L order, the transposed source table
-// srcT[offset * np + rank] (-1 where a target has no source at that offset)
-// and the per-tile offset presence table.
-void m2l_prep(hipStream_t s, const TreeView& t, int np, int ntiles, const int* tstart, int* rank, int* srcT,
+// M2L preparation: the transposed source table srcT[slot * np + rank] (rank:
+// the target's position in the M2L order; -1 where it has no source at that
+// offset) and the per-tile offset presence table.
+void m2l_prep(hipStream_t s, const TreeView& t, int np, int ntiles, const int* tstart, int* srcT,
               unsigned char* present) {
     const int nb = t.nb;
-    hipLaunchKernelGGL(k_rank_of, dim3(blocks(nb)), dim3(kT), 0, s, nb, t.m2l_order, rank);
-    hipLaunchKernelGGL(k_fill_int, dim3(blocks((long long)kNumOffsets * np)), dim3(kT), 0, s, srcT,
-                       (size_t)kNumOffsets * np, -1);
-    hipLaunchKernelGGL(k_m2l_src, dim3(blocks((long long)nb * kVSlots)), dim3(kT), 0, s, nb, np, rank, t.vl, t.nv,
-                       t.ic, srcT);
+    hipLaunchKernelGGL(k_m2l_src, dim3((nb + kSrcTargets - 1) / kSrcTargets), dim3(kT), 0, s, nb, np, t.m2l_order,
+                       t.vl, t.nv, t.ic, srcT);
     hipLaunchKernelGGL(k_m2l_plan, dim3(ntiles * kNumOffsets), dim3(64), 0, s, tstart, np, srcT, present);
 }
 
@@ -1335,10 +1341,9 @@ void m2l_prep_for(Pool& pool, hipStream_t s, const TreeView& t, int prec, const 
     if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
-    int* rank = pool.get<int>("m2l_rank", nb);
     int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
     unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
-    m2l_prep(s, t, np, ntiles, tstart, rank, srcT, present);
+    m2l_prep(s, t, np, ntiles, tstart, srcT, present);
 }
 
 // P2M into M, then M2M level by level (upward pass).
@@ -1439,7 +1444,6 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
     const int np = nb;
-    int* rank = pool.get<int>("m2l_rank", nb);
     int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
     unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
     uint4* Mbf = form == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
@@ -1473,7 +1477,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (split && !p2p_after_m2l && !side.p2p_launched) fork_p2p();
     if (!side.upward_launched) upward<P>(s, t, meta, tab, M, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     if (!side.m2l_launched) {
-        if (!side.prep_done) m2l_prep(s, t, np, ntiles, tstart, rank, srcT, present);
+        if (!side.prep_done) m2l_prep(s, t, np, ntiles, tstart, srcT, present);
         if (side.upward_launched && side.up_done) AFMM_CHECK(hipStreamWaitEvent(s, side.up_done, 0));  // M done
         m2l_launch<P>(pool, s, t, tab, prec, sh, ev ? ev[5] : nullptr, nullptr);
     }

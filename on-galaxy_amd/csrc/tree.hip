@@ -437,6 +437,10 @@ __global__ void k_fill(int* a, int n, int v) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i < n) a[i] = v;
 }
+__global__ void k_zero4(int n, int* a, int* b, int* c, int* d) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i < n) a[i] = b[i] = c[i] = d[i] = 0;
+}
 __global__ void __launch_bounds__(kT) k_count_starts(int nb, const int* seg_first, const int* level,
                                                     int* cnt, int* minlev) {
     const int b = blockIdx.x * kT + threadIdx.x;
@@ -490,45 +494,23 @@ __global__ void __launch_bounds__(kT) k_m2l_keys(int nb, const int* level, const
     klo[b] = 0;
     val[b] = b;
 }
-// Inside each 256-target M2L tile, a stable sort by V signature (vsig): the
-// 16 targets of a column block then mostly have sources at the same offsets
-// (on a 1M-body merger 13% fewer (block, offset) items than in BFS order),
-// while the tile keeps the same targets, and so the same source boxes for the
-// L2 to reuse from one offset to the next (a global sort by signature cuts
-// more items but scatters a tile's sources and measured slower). Any order
-// gives the same sums. One block per tile.
-__global__ void __launch_bounds__(256) k_m2l_tile_sort(const int* __restrict__ tstart, const int* __restrict__ vsig,
-                                                      int* __restrict__ order) {
-    __shared__ int box[256], key[256];
-    const int a = tstart[blockIdx.x], n = tstart[blockIdx.x + 1] - a;  // n <= 256
-    const int i = threadIdx.x;
-    if (i < n) {
-        box[i] = order[a + i];
-        key[i] = vsig[box[i]];
-    }
-    __syncthreads();
-    if (i < n) {
-        const int k = key[i];
-        int r = 0;
-        for (int j = 0; j < n; ++j) r += key[j] < k || (key[j] == k && j < i);
-        order[a + r] = box[i];
-    }
-}
-// Boxes per octant (one block of 1024 threads; integer sums, any order).
-constexpr int kOctCountThreads = 1024;
+// Boxes per octant, added into cnt[0..8) (zeroed before): block sums in LDS,
+// then eight integer atomics per block (order-free).
+constexpr int kOctCountThreads = 256;
+constexpr int kOctCountPer = 16;  // boxes per thread
 __global__ void __launch_bounds__(kOctCountThreads) k_octant_count(int nb, const int* octant, int* cnt) {
     __shared__ int c[8];
     if (threadIdx.x < 8) c[threadIdx.x] = 0;
     __syncthreads();
     int my[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b0 = 4 * (int)threadIdx.x; b0 < nb; b0 += 4 * kOctCountThreads) {
+    const int b0 = blockIdx.x * kOctCountThreads * kOctCountPer + (int)threadIdx.x;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (b0 + u < nb) {
-                const int o = octant[b0 + u];
+    for (int u = 0; u < kOctCountPer; ++u) {
+        const int b = b0 + u * kOctCountThreads;
+        if (b < nb) {
+            const int o = octant[b];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) my[q] += (o == q);
-            }
+            for (int q = 0; q < 8; ++q) my[q] += (o == q);
         }
     }
 #pragma unroll
@@ -538,7 +520,7 @@ __global__ void __launch_bounds__(kOctCountThreads) k_octant_count(int nb, const
         if ((threadIdx.x & 63) == 0 && v) atomicAdd(&c[q], v);
     }
     __syncthreads();
-    if (threadIdx.x < 8) cnt[threadIdx.x] = c[threadIdx.x];
+    if (threadIdx.x < 8 && c[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], c[threadIdx.x]);
 }
 // M2L tile starts over m2l_order (cnt = boxes per octant): each octant group
 // split into the fewest near-equal tiles of at most T targets. The host
@@ -579,6 +561,7 @@ __global__ void __launch_bounds__(kT) k_nonleaf_ids(int nb, const int* nchild, c
 __global__ void k_gather_at(const int* a, const int* idx, int n, int total, int* out, const int* extra, int nextra) {
     const int i = threadIdx.x;
     if (i < n) out[i] = idx[i] < total ? a[idx[i]] : -1;
+    if (i < 8) out[n + i] = 0;  // the octant counts, summed next
     if (i < nextra) out[n + 8 + i] = extra[i];
 }
 
@@ -694,21 +677,10 @@ __device__ __forceinline__ bool adjacent(const double* cfp, const double* half, 
            fabs(cfp[3 * (size_t)a + 2] - cfp[3 * (size_t)b + 2]) <= reach;
 }
 
-// Direction of colleague q of box p (same level) as 9 (dx+1) + 3 (dy+1) + (dz+1)
-// in 0..26 (13 = p itself); steps are clamped to [-1, 1] (only a grouping key).
-__device__ __forceinline__ int neighbour_block(const int* __restrict__ ic, int p, int q) {
-    int d = 0;
-    for (int a = 0; a < 3; ++a) d = 3 * d + (min(1, max(-1, ic[3 * (size_t)q + a] - ic[3 * (size_t)p + a])) + 1);
-    return d;
-}
-
 // Colleagues and V of every box of one level (AdaptiveOctree.hpp:143-161):
 // children of the parent's colleagues, colleague order then octant order.
 // One wave per box: lane k takes candidate k = 8*slot + octant (<= 216), and
 // ballots keep the emitted order (= ascending id = Morton order in a level).
-// vsig[b] has bit 26 - d set when some V entry of b is a child of the
-// parent's colleague in direction d (the M2L grouping key).
-//
 // The same pass carries, level by level, the two upward-looking sets the U,
 // W and X lists are built from (AdaptiveOctree.hpp:163-189):
 //  * A(b) = the leaves of b's level or coarser that are adjacent to b: a leaf
@@ -731,7 +703,7 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
                                               const double* __restrict__ half, const int* __restrict__ ic,
                                               const int* __restrict__ refid,
                                               int* __restrict__ coll, int* __restrict__ ncoll, int* __restrict__ vl,
-                                              int* __restrict__ nv, int* __restrict__ vsig, int* __restrict__ aset,
+                                              int* __restrict__ nv, int* __restrict__ aset,
                                               int* __restrict__ acnt, int* __restrict__ xset, int* __restrict__ xcnt,
                                               int* __restrict__ overflow) {
     __shared__ int xbox[kCollSlots], xkey[kCollSlots];
@@ -743,7 +715,6 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
             coll[kCollSlots * (size_t)b] = b;
             ncoll[b] = 1;
             nv[b] = 0;
-            vsig[b] = 0;
             const bool leaf = nchild[b] == 0;
             if (leaf) aset[kCollSlots * (size_t)b] = b;
             acnt[b] = leaf ? 1 : 0;
@@ -754,42 +725,37 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
     const double tol = roots[bsys[b]].tol;
     const int npc = ncoll[par];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    // A(parent) -> A(b) or X(b)
-    int ka = 0, kx = 0;
-    {
-        const int npa = acnt[par];
-        int c = -1;
-        bool adj = false;
-        if (lane < npa) {
-            c = aset[kCollSlots * (size_t)par + lane];
-            adj = adjacent(cfp, half, tol, c, b);
-        }
-        const uint64_t ba = __ballot(c >= 0 && adj), bx = __ballot(c >= 0 && !adj);
-        if (c >= 0 && adj) aset[kCollSlots * (size_t)b + __popcll(ba & lt)] = c;  // < npa <= 27
-        if (c >= 0 && !adj) {
-            xbox[__popcll(bx & lt)] = c;
-            xkey[__popcll(bx & lt)] = refid[c];
-        }
-        ka = __popcll(ba);
-        kx = __popcll(bx);
+    // A(parent) -> A(b) or X(b): entry `lane` of A(parent), read ahead of the
+    // colleague loop (its results are placed after it)
+    int pa = -1, pa_ref = 0;
+    bool pa_adj = false;
+    if (lane < acnt[par]) {
+        pa = aset[kCollSlots * (size_t)par + lane];
+        pa_adj = adjacent(cfp, half, tol, pa, b);
+        pa_ref = refid[pa];
     }
     int kc = 0, kv = 0;
-    unsigned sig = 0;
+    const uint64_t ba = __ballot(pa >= 0 && pa_adj), bx = __ballot(pa >= 0 && !pa_adj);
+    if (pa >= 0 && pa_adj) aset[kCollSlots * (size_t)b + __popcll(ba & lt)] = pa;  // < acnt[par] <= 27
+    if (pa >= 0 && !pa_adj) {
+        xbox[__popcll(bx & lt)] = pa;
+        xkey[__popcll(bx & lt)] = pa_ref;
+    }
+    int ka = __popcll(ba);
+    const int kx = __popcll(bx);
     for (int base = 0; base < 8 * npc; base += 64) {
         const int k = base + lane;
         const int slot = k >> 3, o = k & 7;
         int c = -1;
         bool adj = false;
-        int pc = -1;
         if (slot < npc) {
-            pc = coll[kCollSlots * (size_t)par + slot];
+            const int pc = coll[kCollSlots * (size_t)par + slot];
             const unsigned mk = mask[pc];
             if ((mk >> o) & 1u) {
                 c = child_first[pc] + __popc(mk & ((1u << o) - 1u));
                 adj = adjacent(cfp, half, tol, b, c);
             }
         }
-        if (c >= 0 && !adj) sig |= 1u << (26 - neighbour_block(ic, par, pc));
         const bool leafc = c >= 0 && adj && nchild[c] == 0;
         const uint64_t bc = __ballot(c >= 0 && adj), bv = __ballot(c >= 0 && !adj), bl = __ballot(leafc);
         if (c >= 0 && adj) {
@@ -813,7 +779,6 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
         kv += __popcll(bv);
         ka += __popcll(bl);
     }
-    for (int m = 32; m >= 1; m >>= 1) sig |= __shfl_xor(sig, m, 64);
     // X(b) ranked by reference id (ids are distinct)
     __syncthreads();
     if (lane < kx) {
@@ -825,7 +790,6 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
     if (lane == 0) {
         ncoll[b] = kc < kCollSlots ? kc : kCollSlots;
         nv[b] = kv < kVSlots ? kv : kVSlots;
-        vsig[b] = (int)sig;
         acnt[b] = ka < kCollSlots ? ka : kCollSlots;
         xcnt[b] = kx;
     }
@@ -838,31 +802,31 @@ __device__ __forceinline__ int wave_total(int v) {
 }
 
 // ---- U, W and X from the A and X sets -------------------------------------------
-// List sizes, one thread per box: |U(b)| = |A(b)| plus one for every finer
-// leaf that has b in its A set; |W(c)| = the boxes whose X holds c. Integer
-// atomics: the counts do not depend on the order.
+// List sizes, one thread per (box, slot) of the A and X sets: |U(b)| =
+// |A(b)| plus one for every finer leaf that has b in its A set; |W(c)| = the
+// boxes whose X holds c. Integer atomics: the counts do not depend on the order.
 __global__ void __launch_bounds__(kT) k_list_count(int nb, const int* __restrict__ nchild,
                                                   const int* __restrict__ level, const int* __restrict__ leaf_scan,
                                                   const int* __restrict__ aset, const int* __restrict__ acnt,
                                                   const int* __restrict__ xset, const int* __restrict__ xcnt,
                                                   int* __restrict__ ucnt, int* __restrict__ wcnt) {
-    const int b = blockIdx.x * kT + threadIdx.x;
-    if (b >= nb) return;
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= (long long)nb * kCollSlots) return;
+    const int b = (int)(g / kCollSlots), k = (int)(g - (long long)b * kCollSlots);
     if (nchild[b] == 0) {
-        const int na = acnt[b], lb = level[b];
-        atomicAdd(&ucnt[leaf_scan[b]], na);
-        for (int k = 0; k < na; ++k) {
-            const int c = aset[kCollSlots * (size_t)b + k];
-            if (level[c] < lb) atomicAdd(&ucnt[leaf_scan[c]], 1);
+        const int na = acnt[b];
+        if (k == 0) atomicAdd(&ucnt[leaf_scan[b]], na);
+        if (k < na) {
+            const int c = aset[g];
+            if (level[c] < level[b]) atomicAdd(&ucnt[leaf_scan[c]], 1);
         }
     }
-    const int nx = xcnt[b];
-    for (int k = 0; k < nx; ++k) atomicAdd(&wcnt[leaf_scan[xset[kCollSlots * (size_t)b + k]]], 1);
+    if (k < xcnt[b]) atomicAdd(&wcnt[leaf_scan[xset[g]]], 1);
 }
 
-// Fills the lists, one thread per box: U(b) starts with A(b); the transposed
-// entries (finer U, W) take slots from per-list cursors (zeroed before), in
-// an order k_list_sort then fixes. X(b) is copied in its ranked order.
+// Fills the lists, one thread per (box, slot): U(b) starts with A(b); the
+// transposed entries (finer U, W) take slots from per-list cursors (zeroed
+// before), in an order k_list_sort then fixes. X(b) keeps its ranked order.
 __global__ void __launch_bounds__(kT) k_list_fill(int nb, const int* __restrict__ nchild,
                                                  const int* __restrict__ level, const int* __restrict__ leaf_scan,
                                                  const int* __restrict__ aset, const int* __restrict__ acnt,
@@ -871,25 +835,20 @@ __global__ void __launch_bounds__(kT) k_list_fill(int nb, const int* __restrict_
                                                  const int* __restrict__ xoff, int* __restrict__ ucur,
                                                  int* __restrict__ wcur, int* __restrict__ uidx,
                                                  int* __restrict__ widx, int* __restrict__ xidx) {
-    const int b = blockIdx.x * kT + threadIdx.x;
-    if (b >= nb) return;
-    if (nchild[b] == 0) {
-        const int na = acnt[b], lb = level[b];
-        int* ud = uidx + uoff[leaf_scan[b]];
-        for (int k = 0; k < na; ++k) {
-            const int c = aset[kCollSlots * (size_t)b + k];
-            ud[k] = c;
-            if (level[c] < lb) {
-                const int lc = leaf_scan[c];
-                uidx[uoff[lc] + acnt[c] + atomicAdd(&ucur[lc], 1)] = b;
-            }
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= (long long)nb * kCollSlots) return;
+    const int b = (int)(g / kCollSlots), k = (int)(g - (long long)b * kCollSlots);
+    if (nchild[b] == 0 && k < acnt[b]) {
+        const int c = aset[g];
+        uidx[uoff[leaf_scan[b]] + k] = c;
+        if (level[c] < level[b]) {
+            const int lc = leaf_scan[c];
+            uidx[uoff[lc] + acnt[c] + atomicAdd(&ucur[lc], 1)] = b;
         }
     }
-    const int nx = xcnt[b];
-    int* xd = xidx + xoff[b];
-    for (int k = 0; k < nx; ++k) {
-        const int c = xset[kCollSlots * (size_t)b + k];
-        xd[k] = c;
+    if (k < xcnt[b]) {
+        const int c = xset[g];
+        xidx[xoff[b] + k] = c;
         const int lc = leaf_scan[c];
         widx[woff[lc] + atomicAdd(&wcur[lc], 1)] = b;
     }
@@ -1235,7 +1194,8 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     }
     int* d_lvl_out = pool.get<int>("lvl_out", nlev + 8 + 2);
     hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_lvl_idx, nlev, nb, d_lvl_out, d_total, 2);
-    hipLaunchKernelGGL(k_octant_count, dim3(1), dim3(kOctCountThreads), 0, s, nb, tv.octant, d_lvl_out + nlev);
+    hipLaunchKernelGGL(k_octant_count, dim3(blocks(nb, kOctCountThreads * kOctCountPer)), dim3(kOctCountThreads), 0,
+                       s, nb, tv.octant, d_lvl_out + nlev);
     std::vector<int> got(nlev + 8 + 2);
     AFMM_CHECK(hipMemcpyAsync(got.data(), d_lvl_out, sizeof(int) * (nlev + 8 + 2), hipMemcpyDeviceToHost, s));
     AFMM_CHECK(hipStreamSynchronize(s));
@@ -1338,7 +1298,6 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     tv.vl = pool.get<int>("vl", kVSlots * (size_t)nb);
     tv.nv = pool.get<int>("nv", nb);
     int* overflow = pool.get<int>("overflow", 1);
-    int* vsig = pool.get<int>("vsig", nb);
     int* aset = pool.get<int>("aset", kCollSlots * (size_t)nb);
     int* acnt = pool.get<int>("acnt", nb);
     int* xset = pool.get<int>("xset", kCollSlots * (size_t)nb);
@@ -1348,13 +1307,8 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         const int off = meta.level_off[lv], cnt = meta.level_off[lv + 1] - off;
         hipLaunchKernelGGL(k_coll_v, dim3(cnt), dim3(64), 0, s, off, d_roots, tv.bsys, tv.parent, tv.child_first,
                            tv.nchild, tv.child_mask, tv.cfp, tv.half, tv.ic, tv.refid, tv.coll, tv.ncoll, tv.vl,
-                           tv.nv, vsig, aset, acnt, xset, xcnt, overflow);
+                           tv.nv, aset, acnt, xset, xcnt, overflow);
     }
-    // (the 128-target tiles cut the same octant groups, so each holds targets
-    // of at most two neighbouring 256-target tiles)
-    if (tv.m2l_ntiles256 > 0)
-        hipLaunchKernelGGL(k_m2l_tile_sort, dim3(tv.m2l_ntiles256), dim3(256), 0, s, tv.m2l_tiles256, vsig,
-                           tv.m2l_order);
     // (the overflow flag is read back with the list sizes below; list counts
     // are capped at their slots, so the kernels in between stay in bounds)
     // (the list stream waits for the colleague lists before on_vlists enqueues
@@ -1376,8 +1330,8 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     tv.u_off = pool.get<int>("u_off", nl + 1);
     tv.w_off = pool.get<int>("w_off", nl + 1);
     tv.x_off = pool.get<int>("x_off", nb + 1);
-    for (int* a : {ucnt, wcnt, ucur, wcur}) hipLaunchKernelGGL(k_fill, dim3(blocks(nl)), dim3(kT), 0, s, a, nl, 0);
-    hipLaunchKernelGGL(k_list_count, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.level, fscan, aset, acnt,
+    hipLaunchKernelGGL(k_zero4, dim3(blocks(nl)), dim3(kT), 0, s, nl, ucnt, wcnt, ucur, wcur);
+    hipLaunchKernelGGL(k_list_count, dim3(blocks((long long)kCollSlots * nb)), dim3(kT), 0, s, nb, tv.nchild, tv.level, fscan, aset, acnt,
                        xset, xcnt, ucnt, wcnt);
     scan_exclusive(ucnt, tv.u_off, nl, tv.u_off + nl, scan_tmp, s);
     scan_exclusive(wcnt, tv.w_off, nl, tv.w_off + nl, scan_tmp + scan_tmp_ints(scan_cap), s);
@@ -1398,7 +1352,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     tv.u_idx = pool.get<int>("u_idx", tv.n_u);
     tv.w_idx = pool.get<int>("w_idx", tv.n_w);
     tv.x_idx = pool.get<int>("x_idx", tv.n_x);
-    hipLaunchKernelGGL(k_list_fill, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.level, fscan, aset, acnt,
+    hipLaunchKernelGGL(k_list_fill, dim3(blocks((long long)kCollSlots * nb)), dim3(kT), 0, s, nb, tv.nchild, tv.level, fscan, aset, acnt,
                        xset, xcnt, tv.u_off, tv.w_off, tv.x_off, ucur, wcur, tv.u_idx, tv.w_idx, tv.x_idx);
     hipLaunchKernelGGL(k_list_sort, dim3(nl), dim3(64), 0, s, tv.u_off, tv.w_off, tv.seg_first, tv.seg_count,
                        tv.u_idx, tv.w_idx, usrc);

@@ -12,8 +12,9 @@ evaluation that sizes its buffers.
 tune_leaf_cap() searches the capacity alone (default shapes). tune() searches
 the capacity first, then each shape in turn at the best values found so far
 (coordinate descent; every shape leaves the tree, the lists and the accuracy
-unchanged). Results are cached per (p, precision, size bucket, systems) for
-the life of the process.
+unchanged), then the two-stream schedule (afmm_set_overlap: the list passes
+beside M2L or in order before it; the same bits either way). Results are
+cached per (p, precision, size bucket, systems) for the life of the process.
 """
 import math
 
@@ -24,10 +25,12 @@ SHAPES = (("m2l_waves", (16, 8)), ("p2p_schedule", (2, 1)), ("p2p_pair_bodies", 
 _cache = {}
 
 
-def _time(afmm, cap, shapes, sys_off, pos_ptr, mass_ptr, acc_ptr, p, precision, G, eps, reps):
+def _time(afmm, cap, shapes, sys_off, pos_ptr, mass_ptr, acc_ptr, p, precision, G, eps, reps, overlap=None):
     s = afmm.AdaptiveFMM3D(p=p, threshold=cap, precision=precision)
     try:
         s.set_shapes(**shapes)
+        if overlap is not None:
+            s.set_overlap(overlap)
         s.compute_batch_device(sys_off, pos_ptr, mass_ptr, acc_ptr, G=G, eps=eps)
         best = float("inf")
         for _ in range(reps):
@@ -54,10 +57,11 @@ def tune_leaf_cap(afmm, sys_off, pos_ptr, mass_ptr, acc_ptr, p, precision, G=Non
 
 
 def tune(afmm, sys_off, pos_ptr, mass_ptr, acc_ptr, p, precision, G=None, eps=None, caps=CAPS, reps=1):
-    """Leaf capacity, then kernel shapes, by coordinate descent. Returns
-    (config, log): config = {"leaf_cap": c, "shapes": {...}} to pass to
-    AdaptiveFMM3D(threshold=c) and set_shapes(**shapes); log holds every
-    timing (ms) in search order."""
+    """Leaf capacity, then kernel shapes, by coordinate descent, then the
+    overlap mode. Returns (config, log): config = {"leaf_cap": c, "shapes":
+    {...}, "overlap": mode} to pass to AdaptiveFMM3D(threshold=c),
+    set_shapes(**shapes) and set_overlap(mode); log holds every timing (ms)
+    in search order."""
     n = int(sys_off[-1])
     key = ("full", p, precision, int(round(math.log2(max(n, 1)))), len(sys_off) - 1)
     if key in _cache:
@@ -81,6 +85,13 @@ def tune(afmm, sys_off, pos_ptr, mass_ptr, acc_ptr, p, precision, G=None, eps=No
     probe.set_shapes(**best)
     resolved = probe.shapes()
     probe.close()
-    out = ({"leaf_cap": cap, "shapes": resolved, "ms": best_ms}, log)
+    overlap = afmm.OVERLAP_AUTO
+    log["overlap"] = []
+    for mode in (afmm.OVERLAP_BESIDE_M2L, afmm.OVERLAP_IN_ORDER):
+        ms = _time(afmm, cap, best, sys_off, pos_ptr, mass_ptr, acc_ptr, p, precision, G, eps, reps, mode)
+        log["overlap"].append({"overlap": mode, "ms": ms})
+        if ms < best_ms:
+            overlap, best_ms = mode, ms
+    out = ({"leaf_cap": cap, "shapes": resolved, "overlap": overlap, "ms": best_ms}, log)
     _cache[key] = out
     return out

@@ -100,10 +100,33 @@ def test_autotuner_returns_accepted_shapes(afmm):
     d_pos, d_mass, d_acc = afmm.DeviceArray(None, pos), afmm.DeviceArray(None, m), afmm.DeviceArray(pos.shape)
     cfg, log = tune.tune(afmm, off, d_pos.ptr, d_mass.ptr, d_acc.ptr, 6, afmm.PRECISION_FP64, caps=(32, 64))
     print(f"autotuned: {cfg}", flush=True)
-    assert cfg["leaf_cap"] in (32, 64) and len(log["shapes"]) >= 6
+    assert cfg["leaf_cap"] in (32, 64) and len(log["shapes"]) >= 6 and len(log["overlap"]) == 2
     s = afmm.AdaptiveFMM3D(p=6, threshold=cfg["leaf_cap"])
     s.set_shapes(**cfg["shapes"])
+    s.set_overlap(cfg["overlap"])
     assert s.shapes() == cfg["shapes"]
+    s.close()
+
+
+@pytest.mark.parametrize("prec_name,p", [("fp64", 8), ("fp64", 10), ("bf16split", 8), ("split", 6)])
+def test_overlap_modes_keep_results(afmm, prec_name, p):
+    """Every schedule (one stream; two streams with the list passes beside M2L
+    or in order before it; the automatic choice) gives the same bits, with
+    16- and 8-wave M2L workgroups."""
+    prec = {"fp64": afmm.PRECISION_FP64, "bf16split": afmm.PRECISION_MIXED_BF16,
+            "split": afmm.PRECISION_MIXED}[prec_name]
+    pos, m = bodies(30000, seed=8)
+    s = afmm.AdaptiveFMM3D(G=1.0, eps=0.002, p=p, threshold=48, precision=prec)
+    a0 = s.computeAccelerations(pos, m)
+    for waves in sorted({s.shapes()["m2l_waves"], 8}):
+        s.set_shapes(m2l_waves=waves)
+        for mode in (afmm.OVERLAP_OFF, afmm.OVERLAP_BESIDE_M2L, afmm.OVERLAP_IN_ORDER, afmm.OVERLAP_AUTO):
+            s.set_overlap(mode)
+            print(f"  {prec_name} p={p} waves={waves} overlap={mode}", flush=True)
+            assert np.array_equal(s.computeAccelerations(pos, m), a0), (waves, mode)
+    for bad in (-1, 4):
+        with pytest.raises(afmm.AfmmError):
+            s.set_overlap(bad)
     s.close()
 
 
