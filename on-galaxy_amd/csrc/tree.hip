@@ -725,16 +725,43 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
     const double tol = roots[bsys[b]].tol;
     const int npc = ncoll[par];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    // A(parent) -> A(b) or X(b): entry `lane` of A(parent), read ahead of the
-    // colleague loop (its results are placed after it)
+    // All candidates' loads first (lane k of round r takes candidate
+    // 64 r + k = 8 slot + octant, at most 8 * 27 = 216), so that the four
+    // rounds' dependent load chains overlap; the ballots then place them in
+    // candidate order. Also entry `lane` of A(parent) -> A(b) or X(b).
+    constexpr int R = (8 * kCollSlots + 63) / 64;
+    int cand[R];
+    bool cadj[R], cleaf[R], cfar[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = 64 * r + lane;
+        const int slot = k >> 3, o = k & 7;
+        cand[r] = -1;
+        if (slot < npc) {
+            const int pc = coll[kCollSlots * (size_t)par + slot];
+            const unsigned mk = mask[pc];
+            if ((mk >> o) & 1u) cand[r] = child_first[pc] + __popc(mk & ((1u << o) - 1u));
+        }
+    }
     int pa = -1, pa_ref = 0;
     bool pa_adj = false;
-    if (lane < acnt[par]) {
-        pa = aset[kCollSlots * (size_t)par + lane];
+    if (lane < acnt[par]) pa = aset[kCollSlots * (size_t)par + lane];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int c = cand[r];
+        cadj[r] = cleaf[r] = cfar[r] = false;
+        if (c >= 0) {
+            cadj[r] = adjacent(cfp, half, tol, b, c);
+            cleaf[r] = nchild[c] == 0;
+            const int dx = ic[3 * (size_t)b] - ic[3 * (size_t)c], dy = ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1],
+                      dz = ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2];
+            cfar[r] = dx < -3 || dx > 3 || dy < -3 || dy > 3 || dz < -3 || dz > 3;
+        }
+    }
+    if (pa >= 0) {
         pa_adj = adjacent(cfp, half, tol, pa, b);
         pa_ref = refid[pa];
     }
-    int kc = 0, kv = 0;
     const uint64_t ba = __ballot(pa >= 0 && pa_adj), bx = __ballot(pa >= 0 && !pa_adj);
     if (pa >= 0 && pa_adj) aset[kCollSlots * (size_t)b + __popcll(ba & lt)] = pa;  // < acnt[par] <= 27
     if (pa >= 0 && !pa_adj) {
@@ -743,20 +770,11 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
     }
     int ka = __popcll(ba);
     const int kx = __popcll(bx);
-    for (int base = 0; base < 8 * npc; base += 64) {
-        const int k = base + lane;
-        const int slot = k >> 3, o = k & 7;
-        int c = -1;
-        bool adj = false;
-        if (slot < npc) {
-            const int pc = coll[kCollSlots * (size_t)par + slot];
-            const unsigned mk = mask[pc];
-            if ((mk >> o) & 1u) {
-                c = child_first[pc] + __popc(mk & ((1u << o) - 1u));
-                adj = adjacent(cfp, half, tol, b, c);
-            }
-        }
-        const bool leafc = c >= 0 && adj && nchild[c] == 0;
+    int kc = 0, kv = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int c = cand[r];
+        const bool adj = cadj[r], leafc = c >= 0 && adj && cleaf[r];
         const uint64_t bc = __ballot(c >= 0 && adj), bv = __ballot(c >= 0 && !adj), bl = __ballot(leafc);
         if (c >= 0 && adj) {
             const int pos = kc + __popcll(bc & lt);
@@ -766,9 +784,7 @@ __global__ void __launch_bounds__(64) k_coll_v(int off, const RootInfo* __restri
             const int pos = kv + __popcll(bv & lt);
             if (pos < kVSlots) vl[kVSlots * (size_t)b + pos] = c;
             else atomicOr(overflow, 1);
-            const int dx = ic[3 * (size_t)b] - ic[3 * (size_t)c], dy = ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1],
-                      dz = ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2];
-            if (dx < -3 || dx > 3 || dy < -3 || dy > 3 || dz < -3 || dz > 3) atomicOr(overflow, 2);
+            if (cfar[r]) atomicOr(overflow, 2);
         }
         if (leafc) {
             const int pos = ka + __popcll(bl & lt);
