@@ -510,6 +510,11 @@ def main():
     progress("FP64 timing, one stream")
     seq_s, seq_avg, _, seq_solver = timed(afmm.PRECISION_FP64, steps=max(3, args.steps // 2), overlap=False)
     seq_solver.close()
+    # the same launch with nothing beside it: in the timed (two-stream) run P2P
+    # starts beside M2L and the two share the CUs for part of M2L's duration
+    alone = m2l_flops / (seq_avg["m2l"] * 1e-3) / 1e12
+    roofline["alone"] = {"ms": seq_avg["m2l"], "achieved": alone, "frac": alone / PEAK_FP64_TFLOPS,
+                         "note": "k_m2l_mfma in the one-stream run (afmm_set_overlap 0), no other kernel beside it"}
 
     # PCIe-inclusive drop-in semantics (host arrays in and out), reported aside:
     # median of 5 after a warm-up
