@@ -301,9 +301,11 @@ int afmm_set_graph(afmm_ctx* ctx, int enable);
  * schedule also leave the results' bits unchanged, while the P2P lane grouping
  * changes the order of each near-field sum (same accuracy, different last bits). */
 typedef struct {
-    int32_t m2l_waves;        /* waves per M2L workgroup, 16 targets each: 8 or 16 (default 16 with
-                                 the FP64 M2L at p <= 8 and the bf16 M2L at p <= 7, else 8; 16
-                                 needs p <= 8 or the bf16 M2L; split FP32 defaults to 8) */
+    int32_t m2l_waves;        /* waves per M2L workgroup: 8 or 16. One wave per 16 targets, except
+                                 16 waves of the FP64 M2L above p = 8 and of the split-FP32 M2L,
+                                 where two waves share 16 targets (half the row blocks each).
+                                 Default 16 for FP64 at p <= 8 and p >= 14 and for bf16 at p <= 7,
+                                 else 8 */
     int32_t p2p_pair_bodies;  /* leaves with more bodies give each P2P lane two targets (default 32) */
     int32_t p2p_min_group;    /* fewest lanes sharing one target's sources: 1, 2, 4, ..., 64 (default 8) */
     int32_t p2p_schedule;     /* 1: persistent workgroups over the leaves sorted by work bin, largest
@@ -321,9 +323,9 @@ int afmm_get_shapes(const afmm_ctx* ctx, afmm_shapes* out);
  *   AFMM_OVERLAP_AUTO (the default): a second stream, shared by the process's
  *     contexts on that device, runs the upward pass beside the interaction-list
  *     build and P2P beside the far-field passes; with the bf16 or split-FP32
- *     M2L, and the FP64 one from p = 14 with 8-wave workgroups
- *     (afmm_shapes.m2l_waves), the U/W/X lists, P2L, M2P and P2P run there
- *     beside M2L as well (their VALU work shares the SIMDs with its MFMAs);
+ *     M2L, and the FP64 one from p = 14, the U/W/X lists, P2L, M2P and P2P
+ *     run there beside M2L as well (their VALU work shares the SIMDs with its
+ *     MFMAs);
  *   AFMM_OVERLAP_BESIDE_M2L: the latter schedule whatever the M2L shape;
  *   AFMM_OVERLAP_IN_ORDER: the former (lists, P2L and M2P in order before M2L).
  * In graph mode the captured P2P branch uses the context's own second stream

@@ -94,15 +94,14 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     // after its preparation (behind the V lists), and the U, W and X lists,
     // P2L, M2P and P2P run on the second stream beside it, after the upward
     // pass: their VALU work shares the SIMDs with the MFMA-bound M2L, and the
-    // lists' size readback synchronises the second stream only. That needs
-    // an M2L that leaves registers free (8-wave workgroups, 142-216 VGPRs at 2
-    // waves per SIMD; the 16-wave FP64 M2L holds 4 waves of 124 VGPRs). The
+    // lists' size readback synchronises the second stream only. The
     // automatic choice, from the 1M merger: beside for the bf16 (p = 8: 4.60
     // -> 4.46 ms, p = 12: 15.9 -> 15.2 ms) and split-FP32 (7.75 -> 7.54 ms)
-    // forms and for FP64 from p = 14 (52 -> 48.6 ms); FP64 in order at p = 8
-    // (equal either way) and p = 12 (24.5 ms against 27.9 ms beside).
+    // forms and for FP64 from p = 14 (16-wave row-split M2L: 44.8 -> 41.9 ms
+    // at p = 14, 78.8 -> 73.8 ms at p = 16); FP64 in order at p = 8 (equal
+    // either way) and p = 12 (24.4 ms against 28.2 ms beside).
     const bool bf16_m2l = m2l_form(c->prm.precision) == kPrecMixedBf16;
-    const bool auto_beside = m2l_form(c->prm.precision) != kPrecFP64 || (c->prm.p >= 14 && c->shapes.m2l_waves == 8);
+    const bool auto_beside = m2l_form(c->prm.precision) != kPrecFP64 || c->prm.p >= 14;
     const bool lists_beside = split && !c->use_graph &&
                               (c->overlap_mode == AFMM_OVERLAP_BESIDE_M2L ||
                                (c->overlap_mode == AFMM_OVERLAP_AUTO && auto_beside));
@@ -524,9 +523,7 @@ int afmm_set_shapes(afmm_ctx* c, const afmm_shapes* sh) {
     v.m2l_waves = default_m2l_waves(c->prm.p, c->prm.precision);
     if (sh) {
         if (sh->m2l_waves) {
-            const bool w16 = c->prm.p <= 8 || m2l_form(c->prm.precision) == kPrecMixedBf16;
-            if (sh->m2l_waves != 8 && !(sh->m2l_waves == 16 && w16))
-                return fail(c, AFMM_ERR_ARG, "m2l_waves must be 8, or 16 at p <= 8 or with bf16 M2L");
+            if (sh->m2l_waves != 8 && sh->m2l_waves != 16) return fail(c, AFMM_ERR_ARG, "m2l_waves must be 8 or 16");
             v.m2l_waves = sh->m2l_waves;
         }
         if (sh->p2p_pair_bodies < 0) return fail(c, AFMM_ERR_ARG, "p2p_pair_bodies must be >= 0");

@@ -257,16 +257,18 @@ struct Shapes {
     int p2p_min_group = 8;
     int p2p_schedule = 0;
 };
-// The default M2L workgroup of order p and precision mode prec: 16 waves
-// while the accumulators fit four waves per SIMD, else 8 (the bf16 form keeps
-// an FP64 total beside its FP32 accumulator, which needs 8 from p = 8; the
-// split-FP32 form spills at 16 and takes 8 at every order). With 8 waves the
-// M2L leaves registers free on every SIMD for the passes that may run beside it.
+// The default M2L workgroup of order p and precision mode prec, from
+// measurements on the 1M merger: FP64 16 waves up to p = 8 (one wave per
+// 16-target block) and from p = 14 (two waves per block, half the row blocks
+// each: p = 14 48.8 -> 41.9 ms, p = 16 78.7 -> 73.8 ms per evaluation, where
+// 8 waves spilled), 8 waves at p = 9..13 (p = 10 and 12 8% and 1% faster than
+// the 16-wave split); split FP32 8 waves (7.6 against 8.5 ms at p = 8); the
+// bf16 form 16 waves up to p = 7, 8 above (its FP64 fold totals).
 inline int default_m2l_waves(int p, int prec) {
     const int form = m2l_form(prec);
     if (form == kPrecMixedBf16) return p <= 7 ? 16 : 8;
     if (form == kPrecMixed) return 8;
-    return p <= 8 ? 16 : 8;
+    return p <= 8 || p >= 14 ? 16 : 8;
 }
 void run_m2l_prep(Pool& pool, hipStream_t s, const TreeView& tv, int p, int prec, const Shapes& sh);
 // P2M and M2M alone on stream s (events e0/e1 around them when non-null); a

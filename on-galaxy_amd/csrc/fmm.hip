@@ -420,6 +420,19 @@ __global__ void __launch_bounds__(64) k_m2l_plan(const int* __restrict__ tstart,
 
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
+// Waves per 16-target column block of the FP64 / split-FP32 M2L workgroup:
+// with 16 waves, two share each block (half the row blocks each) where one
+// wave's accumulators would not fit four waves per SIMD (above p = 8, and
+// the split-FP32 form with its FP64 fold totals).
+constexpr int m2l_row_split(int p, bool mix, int waves) { return waves == 16 && (p > 8 || mix) ? 2 : 1; }
+// Targets per M2L workgroup (tile) of order p, precision mode prec and waves
+// per workgroup (the bf16 kernel has no row split).
+inline int m2l_tile_targets(int p, int prec, int waves) {
+    const int form = m2l_form(prec);
+    const int rs = form == kPrecMixedBf16 ? 1 : m2l_row_split(p, form == kPrecMixed, waves);
+    return waves / rs * 16 * kM2LCb;
+}
+
 // Operator fragment and accumulator types. FP64: v_mfma_f64_16x16x4f64 on
 // double operators. Split FP32: each operand x is carried as hi = fl32(x),
 // lo = fl32(x - hi), and a 16x16x4 product is three v_mfma_f32_16x16x4f32
@@ -450,7 +463,12 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
     constexpr int S = nterms(P);
     constexpr int RB = m2l_mrb(S), TR = m2l_tail(S), NKC = m2l_mnkc(S), KC = m2l_mkc(S), CH = m2l_mchunk(S);
     constexpr int CB = kM2LCb;
-    constexpr int TILE = W * 16 * CB;
+    // RS waves share each 16-target column block, each taking RBH of the RB
+    // row blocks (the tail rows go to the last of them): RS = 2 when a whole
+    // column block's accumulators would not fit four waves per SIMD.
+    constexpr int RS = m2l_row_split(P, MIX, W);
+    constexpr int RBH = (RB + RS - 1) / RS;
+    constexpr int TILE = W / RS * 16 * CB;
     constexpr int NT = W * 64;
     constexpr int PER = (CH + NT - 1) / NT;
     // WHOLE: both buffers hold a whole operator (all NKC chunks), so the
@@ -460,6 +478,8 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
     __shared__ AT abuf[2][BUFN];
     __shared__ unsigned char present[kNumOffsets + 1];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int grp = wave / RS, rb0 = (wave % RS) * RBH;  // column block, first row block
+    const bool tail = wave % RS == RS - 1;                  // this wave takes the tail rows
     const int j = lane & 15, kq = lane >> 4;
     const int tile = xcd_item();
     if (tile >= ntiles) return;
@@ -468,25 +488,25 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
     bool valid[CB];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-        rank[cb] = base + (wave * CB + cb) * 16 + j;
+        rank[cb] = base + (grp * CB + cb) * 16 + j;
         valid[cb] = rank[cb] < end;
     }
-    Acc acc[CB][RB];
+    Acc acc[CB][RBH];
     // MIX: each offset's products are summed in FP32 from zero and folded into
     // FP64 totals when the offset is done, so FP32 rounding never compounds
     // across the up to 189 offsets of a target.
-    dbl4 accd[MIX ? CB : 1][MIX ? RB : 1];
+    dbl4 accd[MIX ? CB : 1][MIX ? RBH : 1];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
+        for (int rb = 0; rb < RBH; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[cb][rb][r] = 0;
     if constexpr (MIX)
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
+            for (int rb = 0; rb < RBH; ++rb)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) accd[cb][rb][r] = 0.0;
     (void)TILE;
@@ -581,8 +601,9 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
 #pragma unroll
                     for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
-                        for (int rb = 0; rb < RB; ++rb) {
-                            const float2 a = ab[(rb * KC + kk) * 64];
+                        for (int rb = 0; rb < RBH; ++rb) {
+                            if (RS > 1 && rb0 + rb >= RB) continue;  // (the last wave's short share)
+                            const float2 a = ab[((rb0 + rb) * KC + kk) * 64];
 #pragma unroll
                             for (int cb = 0; cb < CB; ++cb) {
                                 acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bh[cb][kk], acc[cb][rb], 0, 0, 0);
@@ -594,15 +615,16 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
 #pragma unroll
                     for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
-                        for (int rb = 0; rb < RB; ++rb) {
-                            const double a = ab[(rb * KC + kk) * 64];
+                        for (int rb = 0; rb < RBH; ++rb) {
+                            if (RS > 1 && rb0 + rb >= RB) continue;
+                            const double a = ab[((rb0 + rb) * KC + kk) * 64];
 #pragma unroll
                             for (int cb = 0; cb < CB; ++cb)
                                 acc[cb][rb] =
                                     __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[cb][kk], acc[cb][rb], 0, 0, 0);
                         }
                 }
-                if constexpr (TR > 0) {
+                if (TR > 0 && tail) {
                     const AT* tb = abuf[buf] + (WHOLE ? c * CH : 0) + RB * KC * 64 + kq;
 #pragma unroll
                     for (int kk = 0; kk < KC; ++kk)
@@ -618,7 +640,7 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
 #pragma unroll
                         for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
-                            for (int rb = 0; rb < RB; ++rb)
+                            for (int rb = 0; rb < RBH; ++rb)
 #pragma unroll
                                 for (int r = 0; r < 4; ++r) {
                                     accd[cb][rb][r] += (double)acc[cb][rb][r];
@@ -667,18 +689,19 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
         if (!valid[cb]) continue;
         const int t = order[rank[cb]];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
+        for (int rb = 0; rb < RBH; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 // D(i, j) of lane j + 16 kq, register r: i = kq + 4r (f64), 4kq + r (f32)
-                const int row = MIX ? 16 * rb + 4 * kq + r : 16 * rb + kq + 4 * r;
-                if (row < S) {
+                const int R0 = 16 * (rb0 + rb);
+                const int row = MIX ? R0 + 4 * kq + r : R0 + kq + 4 * r;
+                if (rb0 + rb < RB && row < S) {
                     if constexpr (MIX) L[(size_t)t * S + row] = accd[cb][rb][r];
                     else L[(size_t)t * S + row] = acc[cb][rb][r];
                 }
             }
         if constexpr (TR > 0)
-            if (kq == 0)
+            if (tail && kq == 0)
 #pragma unroll
                 for (int tr = 0; tr < TR; ++tr) L[(size_t)t * S + 16 * RB + tr] = tacc[cb][tr];
     }
@@ -1337,7 +1360,7 @@ template <int P>
 void m2l_prep_for(Pool& pool, hipStream_t s, const TreeView& t, int prec, const Shapes& sh) {
     const int nb = t.nb, np = nb;
     (void)prec;
-    const int TILE = sh.m2l_waves * 16 * kM2LCb;
+    const int TILE = m2l_tile_targets(P, prec, sh.m2l_waves);
     if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
@@ -1373,7 +1396,7 @@ void m2l_launch(Pool& pool, hipStream_t s, const TreeView& t, const FmmTables& t
     const int nb = t.nb, np = nb;
     double* M = pool.get<double>("Mexp", (size_t)nb * S);
     double* L = pool.get<double>("Lexp", (size_t)nb * S);
-    const int TILE = sh.m2l_waves * 16 * kM2LCb;
+    const int TILE = m2l_tile_targets(P, prec, sh.m2l_waves);
     if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
@@ -1392,16 +1415,12 @@ void m2l_launch(Pool& pool, hipStream_t s, const TreeView& t, const FmmTables& t
         if (form == kPrecMixedBf16)
             hipLaunchKernelGGL((k_m2l_bf16<P, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT, present,
                                tab.m2l_bf, Mbf, L);
-        else if constexpr (W == 8 || P <= 8) {
-            if (form == kPrecMixed)
-                hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
-                                   present, tab.m2l_hl, M, L);
-            else
-                hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
-                                   present, tab.m2l_sw, M, L);
-        } else {
-            throw std::runtime_error("16-wave M2L workgroups need p <= 8 or the bf16 M2L");
-        }
+        else if (form == kPrecMixed)
+            hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                               present, tab.m2l_hl, M, L);
+        else
+            hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                               present, tab.m2l_sw, M, L);
     };
     if (sh.m2l_waves == 16) launch_m2l(std::integral_constant<int, 16>{});
     else launch_m2l(std::integral_constant<int, 8>{});
@@ -1439,7 +1458,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     double* far = pool.get<double>("far", 3 * (size_t)t.n);
     double* fpot = phi ? pool.get<double>("farpot", t.n) : nullptr;
     // M2L targets in (octant, level) order, tiles cut inside octant groups
-    const int TILE = sh.m2l_waves * 16 * kM2LCb;
+    const int TILE = m2l_tile_targets(P, prec, sh.m2l_waves);
     if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;

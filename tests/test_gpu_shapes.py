@@ -78,15 +78,36 @@ def test_batch_equals_alone_under_shapes(afmm):
 
 def test_invalid_shapes_rejected(afmm):
     s = afmm.AdaptiveFMM3D(p=10)
-    for sh in ({"m2l_waves": 12}, {"m2l_waves": 16}, {"p2p_min_group": 3}, {"p2p_min_group": 128},
+    for sh in ({"m2l_waves": 12}, {"m2l_waves": 4}, {"p2p_min_group": 3}, {"p2p_min_group": 128},
                {"p2p_schedule": 3}, {"p2p_pair_bodies": -1}):
         with pytest.raises(afmm.AfmmError):
             s.set_shapes(**sh)
     assert s.shapes()["m2l_waves"] == 8
+    s.set_shapes(m2l_waves=16)  # 16 waves at any order (two per 16 targets above p = 8)
     s.close()
+    for p, w in ((8, 16), (12, 8), (14, 16)):
+        s = afmm.AdaptiveFMM3D(p=p)
+        assert s.shapes()["m2l_waves"] == w, p
+        s.close()
     b = afmm.AdaptiveFMM3D(p=10, precision=afmm.PRECISION_MIXED_BF16)
     b.set_shapes(m2l_waves=16)  # the bf16 M2L runs 16 waves at any order
     b.close()
+
+
+@pytest.mark.parametrize("prec_name,p", [("fp64", 10), ("fp64", 14), ("split", 8)])
+def test_row_split_m2l_bit_identical(afmm, prec_name, p):
+    """16-wave M2L workgroups in which two waves share 16 targets (half the
+    row blocks each) give the bits of the 8-wave workgroups: every row keeps
+    its accumulation order."""
+    prec = {"fp64": afmm.PRECISION_FP64, "split": afmm.PRECISION_MIXED}[prec_name]
+    pos, m = bodies(30000, seed=9)
+    out = []
+    for w in (8, 16):
+        s = afmm.AdaptiveFMM3D(G=1.0, eps=0.002, p=p, threshold=48, precision=prec)
+        s.set_shapes(m2l_waves=w)
+        out.append(s.computeAccelerations(pos, m))
+        s.close()
+    assert np.array_equal(out[0], out[1])
 
 
 def test_autotuner_returns_accepted_shapes(afmm):
