@@ -395,11 +395,13 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
     }
 }
 
-// Waves per M2L workgroup (W, a kernel shape the autotuner may choose): by
-// default 16 at p <= 8 (4 per SIMD share the MFMA pipe and hide each other's
-// LDS and gather latency; measured 8% faster than 8 waves of 32 targets), 8
-// above (the accumulators of 11 row blocks need more registers). Each wave
-// owns one 16-target column block, so a tile holds 16 W targets.
+// Waves per M2L workgroup (W, a kernel shape the autotuner may choose): 16
+// waves (4 per SIMD share the MFMA pipe and hide each other's LDS and gather
+// latency; at p = 8 measured 8% faster than 8 waves of 32 targets) or 8. A
+// wave owns one 16-target column block, except where one block's
+// accumulators would not fit four waves per SIMD (FP64 above p = 8, split
+// FP32): there 16-wave workgroups give each block two waves (m2l_row_split).
+// The defaults per order and form are in default_m2l_waves.
 constexpr int kM2LCb = 1;  // 16-target column blocks per wave
 
 // present[wg][k] = 1 when some target of tile wg (ranks tstart[wg] ..
