@@ -326,7 +326,8 @@ int afmm_get_shapes(const afmm_ctx* ctx, afmm_shapes* out);
  *     M2L, and the FP64 one from p = 14, the U/W/X lists, P2L, M2P and P2P
  *     run there beside M2L as well (their VALU work shares the SIMDs with its
  *     MFMAs);
- *   AFMM_OVERLAP_BESIDE_M2L: the latter schedule whatever the M2L shape;
+ *   AFMM_OVERLAP_BESIDE_M2L: the latter schedule in every form, with P2P
+ *     beside M2L too (the automatic schedule starts P2P after the bf16 M2L);
  *   AFMM_OVERLAP_IN_ORDER: the former (lists, P2L and M2P in order before M2L).
  * In graph mode the captured P2P branch uses the context's own second stream
  * and the lists stay in order. Results are the same bits in every mode. */

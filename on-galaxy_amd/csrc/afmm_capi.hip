@@ -143,12 +143,15 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
         if (r.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
     if (lists_beside) {
         // the tree phase ends with the lists on the second stream; P2P follows
-        // them there (after M2L in the bf16 form, whose short M2L it slowed)
+        // them there (in the automatic schedule after the bf16 M2L, whose short
+        // M2L it slowed on the 1M merger; AFMM_OVERLAP_BESIDE_M2L keeps it
+        // beside, for near fields as large as the far field)
         AFMM_CHECK(hipEventRecord(c->ev[1], c->side));
         AFMM_CHECK(hipEventRecord(c->lists_done, c->side));
         // P2L and M2P (needed first, by L2L and L2P on s), then P2P
         run_far_passes(c->pool, c->side, tv, c->prm.p, phi != nullptr, c->x_done, c->w_done);
-        if (bf16_m2l) AFMM_CHECK(hipStreamWaitEvent(c->side, c->m2l_done, 0));
+        if (bf16_m2l && c->overlap_mode != AFMM_OVERLAP_BESIDE_M2L)
+            AFMM_CHECK(hipStreamWaitEvent(c->side, c->m2l_done, 0));
         run_p2p_near(c->pool, c->side, tv, c->prm.precision, c->shapes, phi != nullptr, c->ev[8], c->ev[9]);
         AFMM_CHECK(hipEventRecord(c->side_join, c->side));
     } else {
