@@ -357,27 +357,50 @@ __global__ void __launch_bounds__(kM2PWaves * 64, m2p_min_waves<P>()) k_m2p(Tree
 // group_size 1, num_groups_per_blk 4).
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+__global__ void __launch_bounds__(kT) k_zero_int(int* a, size_t n) {
+    const size_t i = blockIdx.x * (size_t)kT + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
 // The offset slots of afmm_math.h for device code (srcT rows, presence and
 // operators are all kept in slot order).
 __constant__ M2LSlots kM2LSlots = make_m2l_slots();
 
 // srcT[slot * np + r] = the source box of target order[r] at the offset of
-// that slot, -1 where it has none. One block per 16 consecutive targets of
-// the M2L order: their V lists fill a slots x 16 table in LDS (initially -1),
-// which is then written out row by row, 64 contiguous bytes per offset, so
-// the whole table is written once and coalesced (no separate fill of -1).
+// that slot, -1 where it has none, and present[tile][slot] = 1 when some
+// target of the tile has a source at that slot (present zeroed before). One
+// block per 16 consecutive targets of the M2L order: their V lists fill a
+// slots x 16 table in LDS (initially -1), which is then written out row by
+// row, 64 contiguous bytes per offset, so the whole table is written once and
+// coalesced (no separate fill of -1). Presence: one integer atomic per
+// (slot, run of the block's targets in one tile) with a source there
+// (order-free; a block's targets span one tile or a few small ones).
 constexpr int kSrcTargets = 16;
 __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __restrict__ order,
                                                const int* __restrict__ vl, const int* __restrict__ nv,
-                                               const int* __restrict__ ic, int* __restrict__ srcT) {
+                                               const int* __restrict__ ic, int* __restrict__ srcT,
+                                               const int* __restrict__ tstart, int ntiles,
+                                               int* __restrict__ present) {
     __shared__ int tab[kNumOffsets][kSrcTargets];
-    __shared__ int box[kSrcTargets], cnt[kSrcTargets];
+    __shared__ int box[kSrcTargets], cnt[kSrcTargets], tile_of[kSrcTargets];
     const int r0 = blockIdx.x * kSrcTargets, tid = threadIdx.x;
     const int nt = min(kSrcTargets, nb - r0);
     for (int e = tid; e < kNumOffsets * kSrcTargets; e += kT) (&tab[0][0])[e] = -1;
     if (tid < nt) {
         box[tid] = order[r0 + tid];
         cnt[tid] = nv[box[tid]];
+    }
+    if (tid == 0) {  // the tile of each target: the last k with tstart[k] <= rank
+        int lo = 0, hi = ntiles - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tstart[mid] <= r0) lo = mid;
+            else hi = mid - 1;
+        }
+        for (int j = 0; j < nt; ++j) {
+            while (lo + 1 < ntiles && tstart[lo + 1] <= r0 + j) ++lo;
+            tile_of[j] = lo;
+        }
     }
     __syncthreads();
     for (int w = tid; w < nt * kVSlots; w += kT) {
@@ -393,6 +416,16 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
         const int k = e / kSrcTargets, j = e - k * kSrcTargets;
         if (j < nt) srcT[(size_t)k * np + r0 + j] = tab[k][j];
     }
+    for (int k = tid; k < kNumOffsets; k += kT) {
+        bool any = false;
+        for (int j = 0; j < nt; ++j) {
+            any |= tab[k][j] >= 0;
+            if (j + 1 == nt || tile_of[j + 1] != tile_of[j]) {  // end of a run in one tile
+                if (any) atomicOr(&present[(size_t)tile_of[j] * kNumOffsets + k], 1);
+                any = false;
+            }
+        }
+    }
 }
 
 // Waves per M2L workgroup (W, a kernel shape the autotuner may choose): 16
@@ -403,22 +436,6 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
 // FP32): there 16-wave workgroups give each block two waves (m2l_row_split).
 // The defaults per order and form are in default_m2l_waves.
 constexpr int kM2LCb = 1;  // 16-target column blocks per wave
-
-// present[wg][k] = 1 when some target of tile wg (ranks tstart[wg] ..
-// tstart[wg+1]) has a source at the offset of slot k (one wave per (tile, slot)).
-__global__ void __launch_bounds__(64) k_m2l_plan(const int* __restrict__ tstart, int np,
-                                                const int* __restrict__ srcT, unsigned char* __restrict__ present) {
-    const int wg = blockIdx.x / kNumOffsets, o = blockIdx.x % kNumOffsets;  // o: a slot
-    const int off = kM2LSlots.offset_of[o];
-    const int dx = off / 49 - 3, dy = (off / 7) % 7 - 3, dz = off % 7 - 3;
-    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
-    bool any = false;
-    if (!near)
-        for (int r = tstart[wg] + (int)threadIdx.x; r < tstart[wg + 1]; r += 64)
-            if (srcT[(size_t)o * np + r] >= 0) any = true;
-    const bool f = __ballot(any) != 0ull;
-    if (threadIdx.x == 0) present[(size_t)wg * kNumOffsets + o] = f ? 1 : 0;
-}
 
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
@@ -457,7 +474,7 @@ template <int P, bool MIX, int W>
 __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const int* __restrict__ tstart,
                                                            const int* __restrict__ order,
                                                            const int* __restrict__ srcT,
-                                                           const unsigned char* __restrict__ present_g,
+                                                           const int* __restrict__ present_g,
                                                            const typename M2LTypes<MIX>::A* __restrict__ Asw,
                                                            const double* __restrict__ M, double* __restrict__ L) {
     using AT = typename M2LTypes<MIX>::A;
@@ -763,7 +780,7 @@ template <int P, int W>
 __global__ void __launch_bounds__(W * 64) k_m2l_bf16(int np, int ntiles, const int* __restrict__ tstart,
                                                            const int* __restrict__ order,
                                                            const int* __restrict__ srcT,
-                                                           const unsigned char* __restrict__ present_g,
+                                                           const int* __restrict__ present_g,
                                                            const uint4* __restrict__ Abf,
                                                            const uint4* __restrict__ Mbf, double* __restrict__ L) {
     constexpr int S = nterms(P);
@@ -1348,12 +1365,12 @@ const int* p2p_order(Pool& pool, hipStream_t s, const TreeView& t, bool reserve_
 // M2L preparation: the transposed source table srcT[slot * np + rank] (rank:
 // the target's position in the M2L order; -1 where it has no source at that
 // offset) and the per-tile offset presence table.
-void m2l_prep(hipStream_t s, const TreeView& t, int np, int ntiles, const int* tstart, int* srcT,
-              unsigned char* present) {
+void m2l_prep(hipStream_t s, const TreeView& t, int np, int ntiles, const int* tstart, int* srcT, int* present) {
     const int nb = t.nb;
+    hipLaunchKernelGGL(k_zero_int, dim3(blocks((long long)ntiles * kNumOffsets)), dim3(kT), 0, s, present,
+                       (size_t)ntiles * kNumOffsets);
     hipLaunchKernelGGL(k_m2l_src, dim3((nb + kSrcTargets - 1) / kSrcTargets), dim3(kT), 0, s, nb, np, t.m2l_order,
-                       t.vl, t.nv, t.ic, srcT);
-    hipLaunchKernelGGL(k_m2l_plan, dim3(ntiles * kNumOffsets), dim3(64), 0, s, tstart, np, srcT, present);
+                       t.vl, t.nv, t.ic, srcT, tstart, ntiles, present);
 }
 
 // The M2L buffers of order P and precision prec, taken from the pool under
@@ -1367,7 +1384,7 @@ void m2l_prep_for(Pool& pool, hipStream_t s, const TreeView& t, int prec, const 
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
     int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
-    unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
+    int* present = pool.get<int>("m2l_present", (size_t)ntiles * kNumOffsets);
     m2l_prep(s, t, np, ntiles, tstart, srcT, present);
 }
 
@@ -1403,7 +1420,7 @@ void m2l_launch(Pool& pool, hipStream_t s, const TreeView& t, const FmmTables& t
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
     const int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
-    const unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
+    const int* present = pool.get<int>("m2l_present", (size_t)ntiles * kNumOffsets);
     uint4* Mbf = form == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
                                         : nullptr;
     if (form == kPrecMixedBf16) {
@@ -1466,7 +1483,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
     const int np = nb;
     int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
-    unsigned char* present = pool.get<unsigned char>("m2l_present", (size_t)ntiles * kNumOffsets);
+    int* present = pool.get<int>("m2l_present", (size_t)ntiles * kNumOffsets);
     uint4* Mbf = form == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
                                         : nullptr;
     (void)Mbf;  // (sized here for the capture; m2l_launch takes it)
