@@ -189,3 +189,37 @@ def test_large_duplicate_leaf(afmm, ndup):
     ref, _ = orc.fmm(pos, m, 1.0, 0.01, 4, 32)
     mx, _ = cabi.rel_err(ref, acc)
     assert mx <= TOL
+
+
+def _sweep_bodies(kind, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "lattice":  # bodies on the planes that bound boxes
+        g = np.arange(20) / 20.0 - 0.5
+        pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    elif kind == "two_scales":  # a 1e-3 clump beside a unit cloud: deep leaves next to coarse ones
+        pos = np.concatenate([rng.uniform(-1, 1, (6000, 3)), 0.3 + 1e-3 * rng.standard_normal((6000, 3))])
+    elif kind == "plane":
+        pos = np.concatenate([rng.uniform(-1, 1, (8000, 2)), np.zeros((8000, 1))], axis=1)
+    elif kind == "shell":
+        v = rng.standard_normal((10000, 3))
+        pos = v / np.linalg.norm(v, axis=1, keepdims=True)
+    else:
+        raise ValueError(kind)
+    return np.ascontiguousarray(pos), rng.uniform(0.5, 1.5, len(pos))
+
+
+@pytest.mark.parametrize("kind,cap", [("lattice", 8), ("lattice", 64), ("two_scales", 16), ("two_scales", 200),
+                                      ("plane", 32), ("shell", 4)])
+def test_tree_and_lists_sweep_vs_oracle(afmm, kind, cap):
+    """Trees and all five lists bit-exact against the oracle on distributions
+    that stress adjacency (bodies on box faces, a plane, a shell) and the
+    U/W/X construction across levels (a tight clump beside a coarse cloud,
+    which fills X and W), at several leaf capacities."""
+    pos, m = _sweep_bodies(kind, 11)
+    print(f"sweep {kind} n={len(m)} cap={cap}", flush=True)
+    s = solver(afmm, 4, cap, 0.0)
+    s.computeAccelerations(pos, m)
+    t, tr = s.tree(), cabi.oracle().tree(pos, m, cap, 0.0)
+    for k in TREE_KEYS:
+        assert np.array_equal(t[k], tr[k]), k
+    s.close()
