@@ -10,7 +10,8 @@ HBM when timing starts. Prints one JSON line. Extra fields:
                 with HIP events on the library's stream over the timed steps
   cpu_baseline  the reference CPU adaptive FMM (oracle/_ref, single thread)
                 or, without it, the oracle restatement, on a random subsample
-                of the same bodies, scaled linearly in N to the full workload
+                of the same bodies, scaled linearly in N to the full workload;
+                all_cores: the oracle restatement on every host core (OpenMP)
   phases_ms     per-phase device time (hipEvents) averaged over timed steps
   host_io_s_per_eval  afmm_eval with host arrays in and out (PCIe-inclusive)
   mixed         the same evaluation in mixed precision (FP32 P2P with FP64
@@ -95,23 +96,46 @@ def cpu_baseline(pos, mass, args):
     what = ("AdaptiveFMM3D (reference, compiled from its sources)" if kind == "reference"
             else "oracle restatement")
     rng = np.random.default_rng(1)
-    runs = {}
+    runs, samples = {}, {}
+    orc = cabi.oracle()
+    orc.set_threads(1)
     for ns in (args.cpu_sample, 4 * args.cpu_sample):
         idx = np.sort(rng.choice(len(mass), ns, replace=False))
         sp, sm = np.ascontiguousarray(pos[idx]), np.ascontiguousarray(mass[idx])
         t0 = time.perf_counter()
-        impl.fmm(sp, sm, 1.0, args.eps, args.p, args.cap)
+        acc, _ = impl.fmm(sp, sm, 1.0, args.eps, args.p, args.cap)
         dt = time.perf_counter() - t0
         runs[ns] = {"sample_bodies": ns, "seconds": dt, "scaled_s_per_eval": dt * len(mass) / ns}
+        samples[ns] = (sp, sm, acc)
     big = runs[4 * args.cpu_sample]
     small = runs[args.cpu_sample]
+    # The same method on every host core: the oracle restatement, whose
+    # OpenMP passes split the boxes of one level (and the leaves) and keep each
+    # box's sums in the reference's order. Its one-thread time on the smaller
+    # sample calibrates it against the figure above; its all-core result on
+    # the larger sample must be the same bits.
+    sp, sm, _ = samples[args.cpu_sample]
+    t0 = time.perf_counter()
+    orc.fmm(sp, sm, 1.0, args.eps, args.p, args.cap)
+    orc1 = time.perf_counter() - t0
+    nt = orc.set_threads(0)
+    sp, sm, acc_big = samples[4 * args.cpu_sample]
+    t0 = time.perf_counter()
+    acc_all, _ = orc.fmm(sp, sm, 1.0, args.eps, args.p, args.cap)
+    tall = time.perf_counter() - t0
+    all_cores = {"kind": "port", "threads": nt, "sample_bodies": 4 * args.cpu_sample, "seconds": tall,
+                 "scaled_s_per_eval": tall * len(mass) / (4 * args.cpu_sample),
+                 "speedup_vs_1_core": big["seconds"] / tall,
+                 "oracle_1_thread_vs_1_core_figure": orc1 / small["seconds"],
+                 "same_bits_as_1_core": bool(np.array_equal(acc_all, acc_big))}
     return {"value": big["scaled_s_per_eval"], "unit": "s/eval", "cores": 1, "kind": kind,
             "host": host_cpu(),
             "sample": f"one {what} evaluation of a random {big['sample_bodies']}-body subsample of the same "
                       f"merger (p={args.p}, cap {args.cap}, eps {args.eps}) took {big['seconds']:.2f} s on 1 "
                       f"host core; value = that x N/{big['sample_bodies']}",
             "subsamples": list(runs.values()),
-            "scaling_check": big["scaled_s_per_eval"] / small["scaled_s_per_eval"]}
+            "scaling_check": big["scaled_s_per_eval"] / small["scaled_s_per_eval"],
+            "all_cores": all_cores}
 
 
 def c0_leg(afmm, args):

@@ -474,9 +474,20 @@ void adaptive_fmm(const double* pos, const double* mass, std::size_t n, double G
         M[b] = Expansion(centre(b), p);
         L[b] = Expansion(centre(b), p);
     }
-    // Upward pass (:53-71): P2M at leaves, M2M children -> parent by
-    // descending id (children always have larger ids than their parent).
-    for (int b : t.leaves) {
+    // Boxes by level: the reference visits them in descending (upward) and
+    // ascending (downward) id, which orders every child after its parent;
+    // here the boxes of one level, whose sums do not depend on each other,
+    // may run on several threads (OpenMP), each box's own sums in the
+    // reference's order, so the results are the same bits at any thread count.
+    int maxlev = 0;
+    for (const Box& bx : t.box) maxlev = std::max(maxlev, bx.level);
+    std::vector<std::vector<int>> bylev(maxlev + 1);
+    for (int b = 0; b < nb; ++b) bylev[t.box[b].level].push_back(b);
+    const int nl = (int)t.leaves.size();
+    // Upward pass (:53-71): P2M at leaves, M2M children -> parent.
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int li = 0; li < nl; ++li) {
+        const int b = t.leaves[li];
         const Box& bx = t.box[b];
         std::vector<V3> sp;
         std::vector<double> sq;
@@ -486,32 +497,43 @@ void adaptive_fmm(const double* pos, const double* mass, std::size_t n, double G
         }
         M[b] = p2m(centre(b), sp, sq, p);
     }
-    for (int b = nb - 1; b >= 0; --b) {
-        if (t.box[b].leaf) continue;
-        for (int o = 0; o < 8; ++o) {
-            const int ch = t.box[b].child[o];
-            if (ch < 0) continue;
-            const Expansion sh = m2m(M[ch], centre(b));
-            for (int s = 0; s < S; ++s) M[b].a[s] += sh.a[s];
+    for (int lv = maxlev; lv >= 0; --lv) {
+        const std::vector<int>& ids = bylev[lv];
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int k = 0; k < (int)ids.size(); ++k) {
+            const int b = ids[k];
+            if (t.box[b].leaf) continue;
+            for (int o = 0; o < 8; ++o) {
+                const int ch = t.box[b].child[o];
+                if (ch < 0) continue;
+                const Expansion sh = m2m(M[ch], centre(b));
+                for (int s = 0; s < S; ++s) M[b].a[s] += sh.a[s];
+            }
         }
     }
-    // Downward pass (:74-85) in ascending id: L2L from parent, M2L over V,
-    // P2L over X.
-    for (int b = 0; b < nb; ++b) {
-        const V3 cb = centre(b);
-        if (t.box[b].parent >= 0) {
-            const Expansion up = l2l(L[t.box[b].parent], cb);
-            for (int s = 0; s < S; ++s) L[b].a[s] += up.a[s];
+    // Downward pass (:74-85): L2L from parent, M2L over V, P2L over X.
+    for (int lv = 0; lv <= maxlev; ++lv) {
+        const std::vector<int>& ids = bylev[lv];
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int k = 0; k < (int)ids.size(); ++k) {
+            const int b = ids[k];
+            const V3 cb = centre(b);
+            if (t.box[b].parent >= 0) {
+                const Expansion up = l2l(L[t.box[b].parent], cb);
+                for (int s = 0; s < S; ++s) L[b].a[s] += up.a[s];
+            }
+            for (int c : t.V[b]) {
+                const Expansion tr = m2l(M[c], cb);
+                for (int s = 0; s < S; ++s) L[b].a[s] += tr.a[s];
+            }
+            for (int c : t.X[b]) p2l_accumulate(L[b], t, t.box[c], cb);
         }
-        for (int c : t.V[b]) {
-            const Expansion tr = m2l(M[c], cb);
-            for (int s = 0; s < S; ++s) L[b].a[s] += tr.a[s];
-        }
-        for (int c : t.X[b]) p2l_accumulate(L[b], t, t.box[c], cb);
     }
     // Leaves (:88-108): L2P + M2P over W + softened P2P over U (self skipped).
     const double eps2 = eps * eps;
-    for (int b : t.leaves) {
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int li = 0; li < nl; ++li) {
+        const int b = t.leaves[li];
         const Box& bx = t.box[b];
         for (int ti = bx.pfirst; ti < bx.pfirst + bx.pcount; ++ti) {
             const V3 zi{t.px[ti], t.py[ti], t.pz[ti]};

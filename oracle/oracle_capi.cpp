@@ -5,6 +5,7 @@
 // functions with the ref_ prefix, so the two can be compared call for call.
 // Expansions cross the boundary as interleaved complex: 2*(p+1)^2 doubles.
 // ============================================================================
+#include <omp.h>
 #include <cmath>
 #include <cstdint>
 #include <random>
@@ -145,6 +146,15 @@ Err evalErr(Pot pot, Fld fld, const std::vector<V3>& tp, const std::vector<doubl
 }  // namespace
 
 extern "C" {
+
+// Threads for the FMM passes (OpenMP over independent boxes; the results are
+// the same bits at any count). 0 restores the count the process started with
+// (OMP_NUM_THREADS, else every host core).
+int orc_set_threads(int nthreads) {
+    static const int initial = omp_get_max_threads();
+    omp_set_num_threads(nthreads > 0 ? nthreads : initial);
+    return omp_get_max_threads();
+}
 
 int orc_fmm(int64_t n, const double* pos, const double* mass, double G, double eps, int p, int cap,
             double* acc, int32_t* stats) {

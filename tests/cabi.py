@@ -95,6 +95,17 @@ class CpuFmm:
     def _f(self, name):
         return getattr(self.lib, f"{self.prefix}_{name}")
 
+    def set_threads(self, n):
+        """Threads for the oracle's FMM passes (OpenMP over independent boxes,
+        the same bits at any count); 0 = the starting count (OMP_NUM_THREADS,
+        else all host cores). The compiled
+        reference has no threads: returns 1 there."""
+        fn = getattr(self.lib, f"{self.prefix}_set_threads", None)
+        if fn is None:
+            return 1
+        fn.argtypes = [C.c_int]
+        return int(fn(int(n)))
+
     # ---- solvers ---------------------------------------------------------
     def fmm(self, pos, mass, G=1.0, eps=0.0, p=8, cap=64):
         pos = np.ascontiguousarray(pos, np.float64)

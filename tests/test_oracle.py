@@ -44,9 +44,17 @@ def test_forces_match_reference_fixture(orc, case):
     g = np.load(os.path.join(GOLD, f"force_{case}.npz"))
     eps, cap = float(g["eps"]), int(g["cap"])
     assert np.array_equal(orc.direct(g["pos"], g["mass"], 1.0, eps), g["direct"])
-    for p in g["ps"]:
-        acc, _ = orc.fmm(g["pos"], g["mass"], 1.0, eps, int(p), cap)
-        assert np.array_equal(acc, g[f"acc_p{p}"]), p
+    # One thread, then several: the OpenMP passes split independent boxes
+    # only, each box summing in the reference's order, so the bits must not
+    # depend on the thread count.
+    try:
+        for nt in (1, 4):
+            assert orc.set_threads(nt) == nt
+            for p in g["ps"]:
+                acc, _ = orc.fmm(g["pos"], g["mass"], 1.0, eps, int(p), cap)
+                assert np.array_equal(acc, g[f"acc_p{p}"]), (nt, p)
+    finally:
+        orc.set_threads(0)
 
 
 @pytest.mark.parametrize("case", FORCE_CASES)
