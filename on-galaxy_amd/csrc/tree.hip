@@ -282,141 +282,256 @@ __global__ void __launch_bounds__(kT) k_emit(int off, int cnt, int l, int next_o
     mask[b] = mk;
 }
 
-// Device-counted variants of k_split / k_emit for the level loop that never
-// waits for the host: level l holds boxes [lev[l], lev[l+1]) and the scan of
-// the child counts writes lev[l+2]. Fixed grids stride over the level; child
-// slots beyond the box arrays' capacity set *overflow and are not written.
-// Level range of lev[l] clamped to the buffers (maxcnt entries per level,
-// `capacity` boxes); a clamp can only follow an overflow, which the host sees.
-constexpr int kEmitTile = kScanThreads;  // boxes per emit block, one per thread
+// ---- topology of all levels at once -----------------------------------------------
+// The run of sorted bodies that shares body i's first l digits holds more than
+// cap bodies iff some window of cap + 1 consecutive bodies around i shares
+// them, i.e. iff M(i) = max over s in [i - cap, i] of common(s, s + cap) >= l.
+// A box splits iff it holds more than cap bodies and its level is below the
+// system's key_levels (= the eps-floor and depth rule of the split), so the
+// leaf holding i is at level d(i) = min(key_levels, M(i) + 1), and i is the
+// first body of its boxes at levels a(i) < l <= d(i), a(i) = common(i - 1, i)
+// (-1 at the first body of a system). Boxes of one level are numbered in body
+// order, which is the level-by-level order of the split (children by parent,
+// then octant): a box's id is its level's offset plus the number of bodies
+// before i that start a box of that level. The box of level l holding body
+// e - 1 ends at e iff a(e) < l; it is the last box of that level started
+// before e.
+constexpr int kTopoTile = 256;                     // bodies per block
+constexpr int kTopoWinMax = 4096 - kTopoTile;      // largest leaf capacity of this path
+constexpr int kTopoItems = (kTopoTile + kTopoWinMax) / kTopoTile;
+static_assert(kTopoTile == 4 * kWave, "four waves per tile");
 
-__device__ __forceinline__ int level_count(const int* lev, int l, int maxcnt, int capacity) {
-    const int off = lev[l];
-    return max(0, min(min(lev[l + 1] - off, maxcnt), capacity - off));
+// Levels (digits) two keys of one system share; -1 for keys of two systems.
+// Level l's digit sits at khi bits [63-3l, 66-3l) (bit 63 is always 0) for
+// l <= 21, at klo bits [3(24-l), 3(25-l)) above that; klo >> 9 is the system.
+__device__ __forceinline__ int common_levels(uint64_t ha, uint32_t la, uint64_t hb, uint32_t lb) {
+    if ((la ^ lb) >> 9) return -1;
+    const uint64_t x = ha ^ hb;
+    if (x) return (__clzll((long long)x) - 1) / 3;
+    const uint32_t y = (la ^ lb) & 0x1FFu;
+    if (y) return 21 + (__clz((int)y) - 23) / 3;
+    return kMaxDepth;
 }
 
-__global__ void __launch_bounds__(kT) k_split_dev(const int* __restrict__ lev, int l, int maxcnt, int capacity,
-                                                 int cap, const double* eps_floor, const RootInfo* roots,
-                                                 const int* bsys, const int* seg_first, const int* seg_count,
-                                                 const uint64_t* khi, const uint32_t* klo, int* bnd, int* nch,
-                                                 int* tsum) {
-    const int off = lev[l], cnt = level_count(lev, l, maxcnt, capacity);
-    for (long long gt = (long long)blockIdx.x * kT + threadIdx.x; gt < 8LL * ((cnt + 7) / 8 * 8);
-         gt += (long long)gridDim.x * kT) {
-        const int i = (int)(gt >> 3), o = (int)(gt & 7);
-        const bool live = i < cnt;
-        int f = 0, c = 0;
-        bool split = false;
-        if (live) {
-            const int b = off + i;
-            f = seg_first[b];
-            c = seg_count[b];
-            const int q = bsys[b];
-            const double half = ldexp(roots[q].half, -l);
-            split = c > cap && !(2.0 * half <= eps_floor[q]) && l < kMaxDepth;
-        }
-        int lo = f;
-        if (split && o > 0) {
-            int hi = f + c;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (level_digit(khi[mid], klo[mid], l + 1) < o) lo = mid + 1;
-                else hi = mid;
-            }
-        }
-        const int up = __shfl_down(lo, 1, 8);
-        const int end = (o == 7) ? f + c : up;
-        int nonempty = (split && end > lo) ? 1 : 0;
-#pragma unroll
-        for (int m = 1; m < 8; m <<= 1) nonempty += __shfl_xor(nonempty, m, 8);
-        // children of the wave's 8 boxes (one tile: tiles are multiples of 8
-        // boxes) into that tile's total; integer sums in any order
-        int wsum = (o == 0 && live && split) ? nonempty : 0;
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) wsum += __shfl_xor(wsum, m, 64);
-        if ((threadIdx.x & 63) == 0 && wsum) atomicAdd(&tsum[(int)((gt >> 3) / kEmitTile)], wsum);
-        if (!live) continue;
-        int* bd = bnd + 9 * (size_t)i;
-        if (split) {
-            bd[o] = lo;
-            if (o == 7) bd[8] = f + c;
-        }
-        if (o == 0) nch[i] = split ? nonempty : 0;
+// d(i) and a(i) + 1 per body, the number of boxes it starts (the reference-id
+// pass reads these), and per level the bodies of this tile that start a box.
+__global__ void __launch_bounds__(kTopoTile) k_topo_count(int n, int cap, int nlev, int ntiles,
+                                                          const RootInfo* __restrict__ roots,
+                                                          const uint64_t* __restrict__ khi,
+                                                          const uint32_t* __restrict__ klo, int* __restrict__ dlev,
+                                                          int* __restrict__ minlev, int* __restrict__ nstart,
+                                                          int* __restrict__ tcnt, int* __restrict__ ovf) {
+    __shared__ int m[kTopoTile + kTopoWinMax];
+    __shared__ int wc[kMaxDepth + 1][kTopoTile / kWave];
+    const int tid = threadIdx.x, base = blockIdx.x * kTopoTile;
+    const int lw = kTopoTile + cap;  // windows s = base - cap + x, x < lw
+    for (int x = tid; x < lw; x += kTopoTile) {
+        const int s = base - cap + x;
+        int w = -1;
+        if (s >= 0 && s + cap < n) w = common_levels(khi[s], klo[s], khi[s + cap], klo[s + cap]);
+        m[x] = w;
     }
+    // maxima over [x, x + P) by doubling, P the largest power of two <= cap + 1;
+    // a window starting past lw - P is partial and never read below
+    int P = 1;
+    while (2 * P <= cap + 1) {
+        __syncthreads();
+        int v[kTopoItems];
+#pragma unroll
+        for (int k = 0; k < kTopoItems; ++k) {
+            const int x = tid + k * kTopoTile;
+            v[k] = x < lw ? max(m[x], x + P < lw ? m[x + P] : -1) : -1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTopoItems; ++k) {
+            const int x = tid + k * kTopoTile;
+            if (x < lw) m[x] = v[k];
+        }
+        P *= 2;
+    }
+    __syncthreads();
+    const int i = base + tid;
+    int a = kMaxDepth, d = -1;  // a body past the end starts nothing
+    if (i < n) {
+        const int M = max(m[tid], m[tid + cap + 1 - P]);  // windows [tid, tid + cap]
+        const uint64_t h = khi[i];
+        const uint32_t lo = klo[i];
+        d = min(roots[lo >> 9].key_levels, M + 1);
+        a = i > 0 ? common_levels(khi[i - 1], klo[i - 1], h, lo) : -1;
+        dlev[i] = d;
+        minlev[i] = a + 1;
+        nstart[i] = max(0, d - a);
+    }
+    const int lane = tid & (kWave - 1), wv = tid / kWave;
+    for (int l = 0; l <= nlev; ++l) {
+        const uint64_t b = __ballot(a < l && l <= d);
+        if (lane == 0) wc[l][wv] = __popcll(b);
+    }
+    __syncthreads();
+    if (tid <= nlev) tcnt[(size_t)tid * ntiles + blockIdx.x] = wc[tid][0] + wc[tid][1] + wc[tid][2] + wc[tid][3];
+    if (blockIdx.x == 0 && tid == 0) *ovf = 0;
 }
 
-// Children of level l, one box per thread: each block scans its tile of
-// kEmitTile child counts (offset: the tile totals before it, summed by the
-// split), writes the children of its boxes and zeroes the next level's tile
-// totals; the block of the last tile writes lev[l + 2].
-__global__ void __launch_bounds__(kScanThreads) k_emit_dev(int* __restrict__ lev, int l, int maxcnt, int capacity,
-                                                          const int* bnd, const int* nch, const int* tsum,
-                                                          int* tsum_next, int ntiles, int* seg_first,
-                                                          int* seg_count, int* level, int* parent, int* octant,
-                                                          int* child_first, int* nchild, unsigned char* mask,
-                                                          int* bsys, int* overflow) {
+// Per level (one block each): exclusive scan of the tile counts in place, and
+// the level's total.
+__global__ void __launch_bounds__(kScanThreads) k_topo_scan(int ntiles, int* __restrict__ tcnt,
+                                                            int* __restrict__ ltot) {
     __shared__ int lds[kScanThreads / kWave];
-    const int off = lev[l], cnt = level_count(lev, l, maxcnt, capacity), next_off = lev[l + 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && cnt != lev[l + 1] - off) atomicOr(overflow, 2);
-    for (int k = blockIdx.x * kScanThreads + threadIdx.x; k < ntiles; k += gridDim.x * kScanThreads)
-        tsum_next[k] = 0;
-    const int last = cnt > 0 ? (cnt - 1) / kEmitTile : 0;
-    if ((int)blockIdx.x > last) return;
-    const int tile_base = block_prefix_of_tiles(tsum, blockIdx.x, lds);
-    const int i = blockIdx.x * kEmitTile + threadIdx.x;
-    const int kc = i < cnt ? nch[i] : 0;
-    int total;
-    const int ex = block_exclusive_scan(kc, lds, total) + tile_base;
-    if ((int)blockIdx.x == last && threadIdx.x == 0) lev[l + 2] = next_off + tile_base + total;
-    if (i < cnt) {
-        const int b = off + i;
-        nchild[b] = kc;
-        if (kc == 0) {
-            child_first[b] = -1;
-            mask[b] = 0;
-            return;
+    int* row = tcnt + (size_t)blockIdx.x * ntiles;
+    int carry = 0;
+    for (int c0 = 0; c0 < ntiles; c0 += kScanTile) {
+        int v[kScanItems], sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int t = c0 + threadIdx.x * kScanItems + k;
+            v[k] = t < ntiles ? row[t] : 0;
+            sum += v[k];
         }
-        const int* bd = bnd + 9 * (size_t)i;
-        const int first = next_off + ex;
-        child_first[b] = first;
-        unsigned char mk = 0;
-        int r = 0;
-        for (int o = 0; o < 8; ++o) {
-            if (bd[o + 1] <= bd[o]) continue;
-            const int c = first + r++;
-            mk |= (unsigned char)(1u << o);
-            if (c >= capacity) {
-                atomicOr(overflow, 1);
-                continue;
+        int total;
+        int ex = block_exclusive_scan(sum, lds, total) + carry;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int t = c0 + threadIdx.x * kScanItems + k;
+            if (t < ntiles) row[t] = ex;
+            ex += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) ltot[blockIdx.x] = carry;
+}
+
+// Level offsets (off[0..nlev+1]) and per-level wave counts of the tile's
+// starts: the shared part of the two passes below. Returns through `off` and
+// `wc` (LDS) after a barrier.
+__device__ __forceinline__ void topo_tile_setup(int nlev, const int* __restrict__ ltot, int a, int d, int* off,
+                                                int (*wc)[kTopoTile / kWave]) {
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    if (tid == 0) {
+        int acc = 0;
+        for (int l = 0; l <= nlev; ++l) {
+            off[l] = acc;
+            acc += ltot[l];
+        }
+        off[nlev + 1] = acc;
+    }
+    for (int l = 0; l <= nlev; ++l) {
+        const uint64_t b = __ballot(a < l && l <= d);
+        if (lane == 0) wc[l][wv] = __popcll(b);
+    }
+    __syncthreads();
+}
+
+// Number of bodies before this thread's body that start a box of level l
+// (this body's box id there if it starts one), given the level's ballot.
+__device__ __forceinline__ int topo_rank(int l, uint64_t b, const int* off, const int (*wc)[kTopoTile / kWave],
+                                         const int* __restrict__ tpre, int ntiles) {
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int r = off[l] + tpre[(size_t)l * ntiles + blockIdx.x] + __popcll(b & lt);
+    for (int w = 0; w < wv; ++w) r += wc[l][w];
+    return r;
+}
+
+// Every box: first body, level, octant, system, parent, first child; leaves
+// get no children. Box ids past `capacity` set *ovf and are not written.
+__global__ void __launch_bounds__(kTopoTile) k_topo_emit(int n, int nlev, int ntiles, int capacity,
+                                                         const uint64_t* __restrict__ khi,
+                                                         const uint32_t* __restrict__ klo,
+                                                         const int* __restrict__ dlev, const int* __restrict__ minlev,
+                                                         const int* __restrict__ tpre, const int* __restrict__ ltot,
+                                                         int* __restrict__ lev, int* ovf, int* seg_first, int* level,
+                                                         int* parent, int* octant, int* child_first, int* nchild,
+                                                         unsigned char* mask, int* bsys) {
+    __shared__ int off[kMaxDepth + 2];
+    __shared__ int wc[kMaxDepth + 1][kTopoTile / kWave];
+    const int e = blockIdx.x * kTopoTile + threadIdx.x;
+    int a = kMaxDepth, d = -1;
+    uint64_t h = 0;
+    uint32_t lo = 0;
+    if (e < n) {
+        d = dlev[e];
+        a = minlev[e] - 1;
+        h = khi[e];
+        lo = klo[e];
+    }
+    topo_tile_setup(nlev, ltot, a, d, off, wc);
+    if (blockIdx.x == 0 && (int)threadIdx.x <= nlev + 1) lev[threadIdx.x] = off[threadIdx.x];
+    int prev = -1, rank_prev = 0;  // this body's box one level up; its rank there
+    bool over = false;
+    for (int l = 0; l <= nlev; ++l) {
+        const bool f = a < l && l <= d;
+        const uint64_t b = __ballot(f);
+        const int id = topo_rank(l, b, off, wc, tpre, ntiles);
+        if (f) {
+            if (id < capacity) {
+                seg_first[id] = e;
+                level[id] = l;
+                octant[id] = l ? level_digit(h, lo, l) : 0;
+                bsys[id] = int(lo >> 9);
+                // the shallowest box's parent holds e - 1 too: the last box of
+                // the level above started before e
+                parent[id] = l == 0 ? -1 : (l == a + 1 ? rank_prev - 1 : prev);
+                if (l == d) {
+                    nchild[id] = 0;
+                    child_first[id] = -1;
+                    mask[id] = 0;
+                }
+            } else {
+                over = true;
             }
-            seg_first[c] = bd[o];
-            seg_count[c] = bd[o + 1] - bd[o];
-            level[c] = l + 1;
-            parent[c] = b;
-            octant[c] = o;
-            bsys[c] = bsys[b];
+            if (l > a + 1 && prev < capacity) child_first[prev] = id;
+            prev = id;
         }
-        mask[b] = mk;
+        rank_prev = id;
     }
+    if (over) atomicOr(ovf, 1);
 }
 
-// Boxes [off, off + cnt) of the last level can no longer split: no children.
-__global__ void __launch_bounds__(kT) k_fill_leaf_range(int off, int cnt, int* nchild, int* child_first,
-                                                       unsigned char* mask) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= cnt) return;
-    nchild[off + i] = 0;
-    child_first[off + i] = -1;
-    mask[off + i] = 0;
-}
-
-__global__ void k_level_init(int* lev, int nsys, int* overflow, int* tsum0, int ntiles) {
-    if (threadIdx.x == 0) {
-        lev[0] = 0;
-        lev[1] = nsys;
-        *overflow = 0;
+// Box ends, seen from the body after them: for e in [1, n] the boxes of levels
+// a(e) < l <= d(e - 1) that hold e - 1 end at e (e = n: every box still open).
+// Their body counts, child counts and child octant masks.
+__global__ void __launch_bounds__(kTopoTile) k_topo_ends(int n, int nlev, int ntiles, int capacity,
+                                                         const int* __restrict__ dlev,
+                                                         const int* __restrict__ minlev,
+                                                         const int* __restrict__ tpre, const int* __restrict__ ltot,
+                                                         const int* __restrict__ seg_first,
+                                                         const int* __restrict__ child_first,
+                                                         const int* __restrict__ octant, int* seg_count,
+                                                         int* nchild, unsigned char* mask) {
+    __shared__ int off[kMaxDepth + 2];
+    __shared__ int wc[kMaxDepth + 1][kTopoTile / kWave];
+    const int e = blockIdx.x * kTopoTile + threadIdx.x;
+    int a = kMaxDepth, d = -1, ae = kMaxDepth, dp = -1;
+    if (e < n) {
+        d = dlev[e];
+        a = minlev[e] - 1;
     }
-    for (int k = threadIdx.x; k < ntiles; k += blockDim.x) tsum0[k] = 0;
+    if (e >= 1 && e <= n) {
+        ae = e < n ? a : -1;
+        dp = dlev[e - 1];
+    }
+    topo_tile_setup(nlev, ltot, a, d, off, wc);
+    int rank_prev = 0;
+    for (int l = 0; l <= nlev + 1; ++l) {
+        int r = 0;
+        if (l <= nlev) r = topo_rank(l, __ballot(a < l && l <= d), off, wc, tpre, ntiles);
+        const int lp = l - 1;
+        if (lp >= 0 && ae < lp && lp <= dp) {
+            const int P = rank_prev - 1;  // the box of level lp holding e - 1
+            if (P < capacity) {
+                seg_count[P] = e - seg_first[P];
+                if (lp < dp) {
+                    const int cf = child_first[P];
+                    nchild[P] = r - cf;
+                    unsigned mk = 0;
+                    for (int c = cf; c < r && c < capacity; ++c) mk |= 1u << octant[c];
+                    mask[P] = (unsigned char)mk;
+                }
+            }
+        }
+        rank_prev = r;
+    }
 }
 
 // Box q of level 0 is the root of system q (bodies [sys_off[q], sys_off[q+1])).
@@ -1074,62 +1189,59 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         tv.child_mask = pool.get<unsigned char>("child_mask", c, s, keep);
         tv.bsys = pool.get<int>("bsys", c, s, keep);
     };
-    // Levels 0..L-1 may split. Run them all without host round trips when the
-    // box arrays can hold the worst case (every level split as far as the
-    // bodies allow); otherwise, or on overflow, fall back to the loop that
-    // reads each level's size back.
+    // Levels 0..L-1 may split. All levels come from the sorted keys at once
+    // (four launches, one read-back of the level offsets) when the leaf
+    // capacity fits the window of the count pass; the box arrays are sized
+    // for the worst case where that bound is modest. Otherwise, or when the
+    // boxes overflow the arrays, the loop below builds one level at a time
+    // and reads each level's size back.
     size_t bound = (size_t)nsys;
     for (int lv = 0; lv < L; ++lv) {
         const double full = (double)nsys * std::pow(8.0, lv);
         bound += 8 * (size_t)std::min<double>(full, (double)n / (cap + 1) + 1.0);
     }
-    const bool device_levels = L > 0 && bound <= (size_t)8 * n + (1u << 20) && bound < (size_t)INT_MAX / 2;
+    const bool topo_levels = L > 0 && cap <= kTopoWinMax && (size_t)(n + 1) * (L + 1) < (size_t)INT_MAX / 2;
     cap_boxes = std::max(cap_boxes, (size_t)nsys + 1024);
-    if (device_levels) cap_boxes = std::max(cap_boxes, bound);
+    if (topo_levels && bound <= (size_t)8 * n + (1u << 20) && bound < (size_t)INT_MAX / 2)
+        cap_boxes = std::max(cap_boxes, bound);
     box_arrays(cap_boxes, false);
-    hipLaunchKernelGGL(k_init_roots, dim3((nsys + 255) / 256), dim3(256), 0, s, nsys, d_sys_off, tv.seg_first,
-                       tv.seg_count, tv.level, tv.parent, tv.octant, tv.bsys);
     int* d_total = pool.get<int>("total", 4);
+    // boxes started per body and the shallowest level of them (the
+    // reference-id pass below): written by the count pass, else counted there
+    int* cnt_start = pool.get<int>("cnt_start", n + 1);
+    int* minlev = pool.get<int>("minlev", n + 1);
+    bool starts_ready = false;
     int l = 0;
     bool done = false;
-    const int* d_level_dev = nullptr;  // the level offsets on the device (device-driven loop)
-    {
-        if (device_levels) {
-            const int maxcnt = std::min<size_t>(bound, (size_t)n + nsys);
-            int* lev = pool.get<int>("level_dev", L + 3);
-            int* bnd = pool.get<int>("split_bnd", 9 * (size_t)maxcnt);
-            int* nch = pool.get<int>("split_nch", maxcnt);
-            int* ovf = lev + L + 2;
-            // child-count totals per kEmitTile boxes, two levels' worth alternating
-            const int ntiles = (maxcnt + kEmitTile - 1) / kEmitTile;
-            int* tsum = pool.get<int>("level_tile_sums", 2 * (size_t)ntiles);
-            hipLaunchKernelGGL(k_level_init, dim3(1), dim3(256), 0, s, lev, nsys, ovf, tsum, ntiles);
-            const int g8 = std::max(1, std::min(4096, (int)((8LL * maxcnt + kT - 1) / kT)));
-            for (int lv = 0; lv < L; ++lv) {
-                int* ts = tsum + (size_t)(lv & 1) * ntiles;
-                int* ts_next = tsum + (size_t)((lv + 1) & 1) * ntiles;
-                hipLaunchKernelGGL(k_split_dev, dim3(g8), dim3(kT), 0, s, lev, lv, maxcnt, (int)cap_boxes, cap,
-                                   d_eps_floor, d_roots, tv.bsys, tv.seg_first, tv.seg_count, khi, klo, bnd, nch, ts);
-                hipLaunchKernelGGL(k_emit_dev, dim3(ntiles), dim3(kScanThreads), 0, s, lev, lv, maxcnt,
-                                   (int)cap_boxes, bnd, nch, ts, ts_next, ntiles, tv.seg_first, tv.seg_count,
-                                   tv.level, tv.parent, tv.octant, tv.child_first, tv.nchild, tv.child_mask,
-                                   tv.bsys, ovf);
-            }
-            std::vector<int> hl(L + 3);
-            AFMM_CHECK(hipMemcpyAsync(hl.data(), lev, sizeof(int) * (L + 3), hipMemcpyDeviceToHost, s));
-            AFMM_CHECK(hipStreamSynchronize(s));
-            if (hl[L + 2] == 0) {
-                d_level_dev = lev;  // (not on the fallback below, which rebuilds the levels)
-                // level L boxes (if any) are leaves
-                hipLaunchKernelGGL(k_fill_leaf_range, dim3(blocks(hl[L + 1] - hl[L])), dim3(kT), 0, s, hl[L],
-                                   hl[L + 1] - hl[L], tv.nchild, tv.child_first, tv.child_mask);
-                meta.level_off.assign(hl.begin(), hl.begin() + L + 2);
-                while (meta.level_off.size() > 2 && meta.level_off[meta.level_off.size() - 1] ==
-                                                         meta.level_off[meta.level_off.size() - 2])
-                    meta.level_off.pop_back();
-                l = (int)meta.level_off.size() - 2;
-                done = true;
-            }
+    const int* d_level_dev = nullptr;  // the level offsets on the device
+    if (topo_levels) {
+        const int ntiles = (n + 1 + kTopoTile - 1) / kTopoTile;  // bodies 0..n (n: the ends of the last boxes)
+        int* lev = pool.get<int>("level_dev", L + 3);
+        int* ovf = lev + L + 2;
+        int* dl = pool.get<int>("topo_leaf_level", n + 1);
+        int* tcnt = pool.get<int>("topo_tiles", (size_t)(L + 1) * ntiles);
+        int* ltot = pool.get<int>("topo_totals", L + 1);
+        hipLaunchKernelGGL(k_topo_count, dim3(ntiles), dim3(kTopoTile), 0, s, n, cap, L, ntiles, d_roots, khi, klo,
+                           dl, minlev, cnt_start, tcnt, ovf);
+        hipLaunchKernelGGL(k_topo_scan, dim3(L + 1), dim3(kScanThreads), 0, s, ntiles, tcnt, ltot);
+        hipLaunchKernelGGL(k_topo_emit, dim3(ntiles), dim3(kTopoTile), 0, s, n, L, ntiles, (int)cap_boxes, khi, klo,
+                           dl, minlev, tcnt, ltot, lev, ovf, tv.seg_first, tv.level, tv.parent, tv.octant,
+                           tv.child_first, tv.nchild, tv.child_mask, tv.bsys);
+        hipLaunchKernelGGL(k_topo_ends, dim3(ntiles), dim3(kTopoTile), 0, s, n, L, ntiles, (int)cap_boxes, dl, minlev,
+                           tcnt, ltot, tv.seg_first, tv.child_first, tv.octant, tv.seg_count, tv.nchild,
+                           tv.child_mask);
+        std::vector<int> hl(L + 3);
+        AFMM_CHECK(hipMemcpyAsync(hl.data(), lev, sizeof(int) * (L + 3), hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
+        if (hl[L + 2] == 0) {
+            d_level_dev = lev;  // (not on the fallback below, which rebuilds the levels)
+            meta.level_off.assign(hl.begin(), hl.begin() + L + 2);
+            while (meta.level_off.size() > 2 &&
+                   meta.level_off[meta.level_off.size() - 1] == meta.level_off[meta.level_off.size() - 2])
+                meta.level_off.pop_back();
+            l = (int)meta.level_off.size() - 2;
+            done = true;
+            starts_ready = true;
         }
     }
     if (!done) {
@@ -1171,17 +1283,17 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     }
 
     // ---- reference box ids
-    int* cnt_start = pool.get<int>("cnt_start", n + 1);
-    int* minlev = pool.get<int>("minlev", n + 1);
     int* start_scan = pool.get<int>("start_scan", n + 1);
     int* pre = pool.get<int>("pre", nb);
     int* nch_pre = pool.get<int>("nch_pre", nb);
     int* base_pre = pool.get<int>("base_pre", nb);
     tv.refid = pool.get<int>("refid", nb);
-    hipLaunchKernelGGL(k_fill, dim3(blocks(n)), dim3(kT), 0, s, cnt_start, n, 0);
-    hipLaunchKernelGGL(k_fill, dim3(blocks(n)), dim3(kT), 0, s, minlev, n, INT_MAX);
-    hipLaunchKernelGGL(k_count_starts, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.seg_first, tv.level, cnt_start,
-                       minlev);
+    if (!starts_ready) {
+        hipLaunchKernelGGL(k_fill, dim3(blocks(n)), dim3(kT), 0, s, cnt_start, n, 0);
+        hipLaunchKernelGGL(k_fill, dim3(blocks(n)), dim3(kT), 0, s, minlev, n, INT_MAX);
+        hipLaunchKernelGGL(k_count_starts, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.seg_first, tv.level, cnt_start,
+                           minlev);
+    }
     scan_exclusive(cnt_start, start_scan, n, nullptr, scan_tmp, s);
     hipLaunchKernelGGL(k_preorder, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.seg_first, tv.level, start_scan,
                        minlev, tv.nchild, pre, nch_pre);
