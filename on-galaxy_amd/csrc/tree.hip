@@ -69,14 +69,30 @@ __global__ void __launch_bounds__(kT) k_bbox_partial(const double* pos, const do
     }
     int bad = 0;
     const int i0 = cbeg[blockIdx.x], i1 = cend[blockIdx.x];
-    for (int i = i0 + threadIdx.x; i < i1; i += kT) {
+    // all loads of the chunk first (chunks hold at most kBBoxChunk bodies),
+    // then the comparisons: (value, index) order makes the result independent
+    // of the order in which they are taken
+    constexpr int kPer = kBBoxChunk / kT;
+    double v[kPer][3], mv[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = i0 + threadIdx.x + k * kT;
+        const bool in = i < i1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[k][a] = in ? pos[3 * (size_t)i + a] : 0.0;
+        mv[k] = in ? mass[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = i0 + threadIdx.x + k * kT;
+        if (i >= i1) continue;
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const double v = pos[3 * (size_t)i + a];
-            bad += !isfinite(v);
-            take_lo(e[a], Ext{v, i});
-            take_hi(e[3 + a], Ext{v, i});
+            bad += !isfinite(v[k][a]);
+            take_lo(e[a], Ext{v[k][a], i});
+            take_hi(e[3 + a], Ext{v[k][a], i});
         }
-        bad += !isfinite(mass[i]);
+        bad += !isfinite(mv[k]);
     }
     __shared__ double sv[6][kT];
     __shared__ int si[6][kT];
