@@ -45,13 +45,32 @@ struct HipError : std::runtime_error {
             throw ::afmm::HipError(e_, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Grow-only named device allocations.
+// Grow-only named device allocations (and a few pinned host ones, for small
+// read-backs that must not block the host when they are enqueued).
 class Pool {
 public:
     ~Pool() { release(); }
     void release() {
         for (auto& kv : slots_) (void)hipFree(kv.second.p);
         slots_.clear();
+        for (auto& kv : hslots_) (void)hipHostFree(kv.second.p);
+        hslots_.clear();
+    }
+    // Pinned host buffer of at least `count` elements (contents undefined
+    // after growth). The caller makes sure no copy into it is still pending.
+    template <class T>
+    T* host(const std::string& name, size_t count) {
+        const size_t bytes = (count ? count : 1) * sizeof(T);
+        Slot& sl = hslots_[name];
+        if (sl.bytes < bytes) {
+            void* np = nullptr;
+            hipError_t e = hipHostMalloc(&np, bytes, hipHostMallocDefault);
+            if (e != hipSuccess) throw HipError(e, "hipHostMalloc(" + name + ")");
+            if (sl.p) AFMM_CHECK(hipHostFree(sl.p));
+            sl.p = np;
+            sl.bytes = bytes;
+        }
+        return static_cast<T*>(sl.p);
     }
     // Buffer of at least `count` elements; contents are undefined after growth
     // unless keep is set (then the old prefix is copied on `s`).
@@ -86,6 +105,7 @@ private:
         size_t bytes = 0;
     };
     std::map<std::string, Slot> slots_;
+    std::map<std::string, Slot> hslots_;
 };
 
 // Root cube and sizing, computed on the device from the bodies.
@@ -184,12 +204,15 @@ int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, 
 // (and its event ls_ready), the U, W and X lists are built on stream ls after
 // it waits for the colleague lists on s (recorded before on_vlists runs; their
 // size readback then synchronises ls only, so work that on_vlists enqueues on s
-// keeps the GPU busy meanwhile).
+// keeps the GPU busy meanwhile). levels_hint (optional, in and out): the key
+// depth of the previous build, taken as a guess that saves the root cubes'
+// own round trip (verified, and the sort redone if too shallow); set to this
+// build's depth.
 using TreeHook = std::function<void(const TreeView&, const TreeMeta&)>;
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
                   std::vector<RootInfo>& roots, const TreeHook& on_core = {}, const TreeHook& on_vlists = {},
-                  hipStream_t ls = nullptr, hipEvent_t ls_ready = nullptr);
+                  hipStream_t ls = nullptr, hipEvent_t ls_ready = nullptr, int* levels_hint = nullptr);
 
 // ---- fmm.hip -----------------------------------------------------------------
 struct FmmTimes {

@@ -316,6 +316,10 @@ constexpr int kTopoTile = 256;                     // bodies per block
 constexpr int kTopoWinMax = 4096 - kTopoTile;      // largest leaf capacity of this path
 constexpr int kTopoItems = (kTopoTile + kTopoWinMax) / kTopoTile;
 static_assert(kTopoTile == 4 * kWave, "four waves per tile");
+// Counts the passes gather for the host with the level offsets: leaves per
+// level [0, kMaxDepth], boxes per octant, the largest leaf's body count.
+constexpr int kTopoLeafCnt = 0, kTopoOctCnt = kMaxDepth + 1, kTopoMaxLeaf = kTopoOctCnt + 8;
+constexpr int kTopoExtra = kTopoMaxLeaf + 1;
 
 // Levels (digits) two keys of one system share; -1 for keys of two systems.
 // Level l's digit sits at khi bits [63-3l, 66-3l) (bit 63 is always 0) for
@@ -373,7 +377,9 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_count(int n, int cap, int nl
         const int M = max(m[tid], m[tid + cap + 1 - P]);  // windows [tid, tid + cap]
         const uint64_t h = khi[i];
         const uint32_t lo = klo[i];
-        d = min(roots[lo >> 9].key_levels, M + 1);
+        // (a guessed depth nlev below the system's own stops every body there:
+        // that build is only checked and then redone, but it stays consistent)
+        d = min(min(roots[lo >> 9].key_levels, nlev), M + 1);
         a = i > 0 ? common_levels(khi[i - 1], klo[i - 1], h, lo) : -1;
         dlev[i] = d;
         minlev[i] = a + 1;
@@ -458,10 +464,12 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_emit(int n, int nlev, int nt
                                                          const int* __restrict__ tpre, const int* __restrict__ ltot,
                                                          int* __restrict__ lev, int* ovf, int* seg_first, int* level,
                                                          int* parent, int* octant, int* child_first, int* nchild,
-                                                         unsigned char* mask, int* bsys) {
+                                                         unsigned char* mask, int* bsys, int* __restrict__ part) {
     __shared__ int off[kMaxDepth + 2];
     __shared__ int wc[kMaxDepth + 1][kTopoTile / kWave];
+    __shared__ int cnt[kTopoMaxLeaf];  // leaves per level, then boxes per octant
     const int e = blockIdx.x * kTopoTile + threadIdx.x;
+    if (threadIdx.x < kTopoMaxLeaf) cnt[threadIdx.x] = 0;  // (read after the barriers below)
     int a = kMaxDepth, d = -1;
     uint64_t h = 0;
     uint32_t lo = 0;
@@ -481,10 +489,13 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_emit(int n, int nlev, int nt
         const int id = topo_rank(l, b, off, wc, tpre, ntiles);
         if (f) {
             if (id < capacity) {
+                const int oc = l ? level_digit(h, lo, l) : 0;
                 seg_first[id] = e;
                 level[id] = l;
-                octant[id] = l ? level_digit(h, lo, l) : 0;
+                octant[id] = oc;
                 bsys[id] = int(lo >> 9);
+                atomicAdd(&cnt[kTopoOctCnt + oc], 1);
+                if (l == d) atomicAdd(&cnt[kTopoLeafCnt + l], 1);
                 // the shallowest box's parent holds e - 1 too: the last box of
                 // the level above started before e
                 parent[id] = l == 0 ? -1 : (l == a + 1 ? rank_prev - 1 : prev);
@@ -502,6 +513,8 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_emit(int n, int nlev, int nt
         rank_prev = id;
     }
     if (over) atomicOr(ovf, 1);
+    __syncthreads();
+    if (threadIdx.x < kTopoMaxLeaf) part[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
 // Box ends, seen from the body after them: for e in [1, n] the boxes of levels
@@ -514,7 +527,7 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_ends(int n, int nlev, int nt
                                                          const int* __restrict__ seg_first,
                                                          const int* __restrict__ child_first,
                                                          const int* __restrict__ octant, int* seg_count,
-                                                         int* nchild, unsigned char* mask) {
+                                                         int* nchild, unsigned char* mask, int* __restrict__ part) {
     __shared__ int off[kMaxDepth + 2];
     __shared__ int wc[kMaxDepth + 1][kTopoTile / kWave];
     const int e = blockIdx.x * kTopoTile + threadIdx.x;
@@ -528,7 +541,7 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_ends(int n, int nlev, int nt
         dp = dlev[e - 1];
     }
     topo_tile_setup(nlev, ltot, a, d, off, wc);
-    int rank_prev = 0;
+    int rank_prev = 0, big = 0;
     for (int l = 0; l <= nlev + 1; ++l) {
         int r = 0;
         if (l <= nlev) r = topo_rank(l, __ballot(a < l && l <= d), off, wc, tpre, ntiles);
@@ -536,8 +549,10 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_ends(int n, int nlev, int nt
         if (lp >= 0 && ae < lp && lp <= dp) {
             const int P = rank_prev - 1;  // the box of level lp holding e - 1
             if (P < capacity) {
-                seg_count[P] = e - seg_first[P];
-                if (lp < dp) {
+                const int c = e - seg_first[P];
+                seg_count[P] = c;
+                if (lp == dp) big = max(big, c);  // a leaf
+                if (lp < dp && child_first[P] >= 0) {
                     const int cf = child_first[P];
                     nchild[P] = r - cf;
                     unsigned mk = 0;
@@ -547,6 +562,39 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_ends(int n, int nlev, int nt
             }
         }
         rank_prev = r;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) big = max(big, __shfl_xor(big, m, 64));
+    __shared__ int wbig[kTopoTile / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) wbig[threadIdx.x / kWave] = big;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[(size_t)kTopoMaxLeaf * ntiles + blockIdx.x] = max(max(wbig[0], wbig[1]), max(wbig[2], wbig[3]));
+}
+
+// The counts of the topology passes from their per-block partials (one block
+// per count: sums, the largest leaf a maximum) into extra[0, kTopoExtra).
+__global__ void __launch_bounds__(kScanThreads) k_topo_sums(int ntiles, const int* __restrict__ part,
+                                                            int* __restrict__ extra) {
+    __shared__ int lds[kScanThreads / kWave];
+    const int k = blockIdx.x;
+    const int* row = part + (size_t)k * ntiles;
+    int v = 0;
+    if (k == kTopoMaxLeaf) {
+        for (int t = threadIdx.x; t < ntiles; t += kScanThreads) v = max(v, row[t]);
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+        if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kScanThreads / kWave; ++w) v = max(v, lds[w]);
+            extra[k] = v;
+        }
+    } else {
+        for (int t = threadIdx.x; t < ntiles; t += kScanThreads) v += row[t];
+        int total;
+        block_exclusive_scan(v, lds, total);
+        if (threadIdx.x == 0) extra[k] = total;
     }
 }
 
@@ -697,7 +745,8 @@ __global__ void k_gather_at(const int* a, const int* idx, int n, int total, int*
 }
 
 // ---- leaves and the in-leaf input order ------------------------------------------
-// Leaf flags, and the largest leaf's body count into *maxleaf (zeroed before).
+// Leaf flags, and the largest leaf's body count into *maxleaf (zeroed before;
+// null: not wanted).
 __global__ void __launch_bounds__(kT) k_leaf_flags(int nb, const int* nchild, const int* seg_count, int* flag,
                                                   int* maxleaf) {
     const int b = blockIdx.x * kT + threadIdx.x;
@@ -707,6 +756,7 @@ __global__ void __launch_bounds__(kT) k_leaf_flags(int nb, const int* nchild, co
         flag[b] = leaf;
         if (leaf) c = seg_count[b];
     }
+    if (!maxleaf) return;
     for (int m = 32; m >= 1; m >>= 1) c = max(c, __shfl_xor(c, m, 64));
     if ((threadIdx.x & 63) == 0 && c > 0) atomicMax(maxleaf, c);
 }
@@ -1123,7 +1173,7 @@ int read_int(const int* d, hipStream_t s) {
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
                   std::vector<RootInfo>& roots, const TreeHook& on_core, const TreeHook& on_vlists,
-                  hipStream_t ls, hipEvent_t ls_ready) {
+                  hipStream_t ls, hipEvent_t ls_ready, int* levels_hint) {
     const int n = h_sys_off[nsys];
     // ---- per-system root cubes: chunks of at most kBBoxChunk bodies of one system
     std::vector<int> hb;  // [cbeg | cend | cfirst(nsys+1) | sys_off(nsys+1)] as one upload
@@ -1155,15 +1205,36 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     RootInfo* d_roots = pool.get<RootInfo>("roots", nsys);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nch), dim3(kT), 0, s, d_pos, d_mass, d_cbeg, d_cend, pv, pi, pbad);
     hipLaunchKernelGGL(k_bbox_final, dim3(nsys), dim3(kT), 0, s, pv, pi, pbad, nch, d_cfirst, d_eps_floor, d_roots);
-    roots.resize(nsys);
-    AFMM_CHECK(hipMemcpyAsync(roots.data(), d_roots, sizeof(RootInfo) * nsys, hipMemcpyDeviceToHost, s));
-    AFMM_CHECK(hipStreamSynchronize(s));  // also retires the uploads of hb
-    for (const RootInfo& r : roots)
-        if (r.nonfinite) return;
+    // (into pinned memory: a copy to pageable memory would wait for the GPU)
+    RootInfo* h_roots = pool.host<RootInfo>("h_roots", nsys);
+    AFMM_CHECK(hipMemcpyAsync(h_roots, d_roots, sizeof(RootInfo) * nsys, hipMemcpyDeviceToHost, s));
+    // The key depth L (levels that may split, the deepest over the systems)
+    // sets the number of sort passes and of topology levels. With the
+    // previous evaluation's depth as a guess, the root cubes come back with
+    // the level offsets (one round trip fewer); a deeper actual depth sorts
+    // and builds again, a shallower one gives the same tree (the extra key
+    // digits are zero and every system stops at its own depth).
+    const int hint = levels_hint ? *levels_hint : -1;
+    bool guess = hint > 0 && cap <= kTopoWinMax && (size_t)(n + 1) * (hint + 1) < (size_t)INT_MAX / 2;
+    auto roots_back = [&]() {  // after a synchronisation: the depth, or -1 for a NaN/Inf input
+        roots.assign(h_roots, h_roots + nsys);
+        int d = 0;
+        for (const RootInfo& r : roots) {
+            if (r.nonfinite) return -1;
+            d = std::max(d, r.key_levels);
+        }
+        if (levels_hint) *levels_hint = d;
+        return d;
+    };
+    int L = hint;
+    if (!guess) {
+        AFMM_CHECK(hipStreamSynchronize(s));  // also retires the uploads of hb
+        L = roots_back();
+        if (L < 0) return;
+    }
     tv.n = n;
     tv.nsys = nsys;
     tv.roots = d_roots;
-    tv.tol = roots[0].tol;
 
     // ---- pass 1: sort bodies by octant path
     const int nsb = sort_blocks(n);
@@ -1177,20 +1248,24 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     int scan_cap = std::max(n, 512 * nsb) + 1;
     int* scan_tmp = pool.get<int>("scan_tmp", 2 * scan_tmp_ints(scan_cap) + 16);
     sb.scan_tmp = scan_tmp;
-    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kT), 0, s, d_pos, n, nsys, d_sys_off, d_roots, sb.khi[0],
-                       sb.klo[0], sb.val[0]);
-    int L = 0;
-    for (const RootInfo& r : roots) L = std::max(L, r.key_levels);
-    const int ndig = (L + 2) / 3;
-    // system digits (klo bits 9..26) after the path digits: sorted by (system, path)
-    const int sdig = nsys <= 1 ? 0 : (nsys <= 512 ? 1 : 2);
-    int r1 = radix_sort(sb, n, 8 - ndig, ndig, s);
-    if (sdig > 0) r1 = radix_sort(sb, n, 8, sdig, s, r1);
-    // the pass-1 keys and permutation stay where the sort left them (nothing
+    // the pass-1 keys and permutation stay where the sort leaves them (nothing
     // below sorts with these buffers again)
-    const uint64_t* khi = sb.khi[r1];
-    const uint32_t* klo = sb.klo[r1];
-    const int* perm1 = sb.val[r1];
+    const uint64_t* khi = nullptr;
+    const uint32_t* klo = nullptr;
+    const int* perm1 = nullptr;
+    auto sort_bodies = [&]() {
+        hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kT), 0, s, d_pos, n, nsys, d_sys_off, d_roots, sb.khi[0],
+                           sb.klo[0], sb.val[0]);
+        const int ndig = (L + 2) / 3;
+        // system digits (klo bits 9..26) after the path digits: sorted by (system, path)
+        const int sdig = nsys <= 1 ? 0 : (nsys <= 512 ? 1 : 2);
+        int r1 = radix_sort(sb, n, 8 - ndig, ndig, s);
+        if (sdig > 0) r1 = radix_sort(sb, n, 8, sdig, s, r1);
+        khi = sb.khi[r1];
+        klo = sb.klo[r1];
+        perm1 = sb.val[r1];
+    };
+    sort_bodies();
 
     // ---- topology, level by level
     size_t cap_boxes = std::max<size_t>(1024, (size_t)n / 4 + 64);
@@ -1205,12 +1280,23 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         tv.child_mask = pool.get<unsigned char>("child_mask", c, s, keep);
         tv.bsys = pool.get<int>("bsys", c, s, keep);
     };
+    int* d_total = pool.get<int>("total", 4);
+    // boxes started per body and the shallowest level of them (the
+    // reference-id pass below): written by the count pass, else counted there
+    int* cnt_start = pool.get<int>("cnt_start", n + 1);
+    int* minlev = pool.get<int>("minlev", n + 1);
+    bool starts_ready = false;
+    int l = 0;
+    bool done = false;
+    std::vector<int> topo_extra;       // leaves per level, boxes per octant, largest leaf
+    const int* d_oct_cnt = nullptr;    // boxes per octant on the device (the M2L tile plan)
     // Levels 0..L-1 may split. All levels come from the sorted keys at once
     // (four launches, one read-back of the level offsets) when the leaf
     // capacity fits the window of the count pass; the box arrays are sized
     // for the worst case where that bound is modest. Otherwise, or when the
     // boxes overflow the arrays, the loop below builds one level at a time
     // and reads each level's size back.
+    for (;;) {  // twice when a guessed key depth was too shallow
     size_t bound = (size_t)nsys;
     for (int lv = 0; lv < L; ++lv) {
         const double full = (double)nsys * std::pow(8.0, lv);
@@ -1221,36 +1307,43 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     if (topo_levels && bound <= (size_t)8 * n + (1u << 20) && bound < (size_t)INT_MAX / 2)
         cap_boxes = std::max(cap_boxes, bound);
     box_arrays(cap_boxes, false);
-    int* d_total = pool.get<int>("total", 4);
-    // boxes started per body and the shallowest level of them (the
-    // reference-id pass below): written by the count pass, else counted there
-    int* cnt_start = pool.get<int>("cnt_start", n + 1);
-    int* minlev = pool.get<int>("minlev", n + 1);
-    bool starts_ready = false;
-    int l = 0;
-    bool done = false;
-    const int* d_level_dev = nullptr;  // the level offsets on the device
     if (topo_levels) {
         const int ntiles = (n + 1 + kTopoTile - 1) / kTopoTile;  // bodies 0..n (n: the ends of the last boxes)
-        int* lev = pool.get<int>("level_dev", L + 3);
+        // level offsets [0, L + 1], the overflow flag, then the counts
+        int* lev = pool.get<int>("level_dev", L + 3 + kTopoExtra);
         int* ovf = lev + L + 2;
+        int* extra = lev + L + 3;
         int* dl = pool.get<int>("topo_leaf_level", n + 1);
         int* tcnt = pool.get<int>("topo_tiles", (size_t)(L + 1) * ntiles);
         int* ltot = pool.get<int>("topo_totals", L + 1);
+        int* part = pool.get<int>("topo_partials", (size_t)kTopoExtra * ntiles);
         hipLaunchKernelGGL(k_topo_count, dim3(ntiles), dim3(kTopoTile), 0, s, n, cap, L, ntiles, d_roots, khi, klo,
                            dl, minlev, cnt_start, tcnt, ovf);
         hipLaunchKernelGGL(k_topo_scan, dim3(L + 1), dim3(kScanThreads), 0, s, ntiles, tcnt, ltot);
         hipLaunchKernelGGL(k_topo_emit, dim3(ntiles), dim3(kTopoTile), 0, s, n, L, ntiles, (int)cap_boxes, khi, klo,
                            dl, minlev, tcnt, ltot, lev, ovf, tv.seg_first, tv.level, tv.parent, tv.octant,
-                           tv.child_first, tv.nchild, tv.child_mask, tv.bsys);
+                           tv.child_first, tv.nchild, tv.child_mask, tv.bsys, part);
         hipLaunchKernelGGL(k_topo_ends, dim3(ntiles), dim3(kTopoTile), 0, s, n, L, ntiles, (int)cap_boxes, dl, minlev,
                            tcnt, ltot, tv.seg_first, tv.child_first, tv.octant, tv.seg_count, tv.nchild,
-                           tv.child_mask);
-        std::vector<int> hl(L + 3);
-        AFMM_CHECK(hipMemcpyAsync(hl.data(), lev, sizeof(int) * (L + 3), hipMemcpyDeviceToHost, s));
-        AFMM_CHECK(hipStreamSynchronize(s));
+                           tv.child_mask, part);
+        hipLaunchKernelGGL(k_topo_sums, dim3(kTopoExtra), dim3(kScanThreads), 0, s, ntiles, part, extra);
+        int* h_lev = pool.host<int>("h_level_dev", L + 3 + kTopoExtra);
+        AFMM_CHECK(hipMemcpyAsync(h_lev, lev, sizeof(int) * (L + 3 + kTopoExtra), hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));  // (with a guessed depth also the root cubes and hb)
+        const std::vector<int> hl(h_lev, h_lev + L + 3 + kTopoExtra);
+        if (guess) {
+            const int actual = roots_back();
+            if (actual < 0) return;
+            guess = false;
+            if (actual > L) {
+                L = actual;
+                sort_bodies();
+                continue;
+            }
+        }
         if (hl[L + 2] == 0) {
-            d_level_dev = lev;  // (not on the fallback below, which rebuilds the levels)
+            topo_extra.assign(hl.begin() + L + 3, hl.end());
+            d_oct_cnt = extra + kTopoOctCnt;
             meta.level_off.assign(hl.begin(), hl.begin() + L + 2);
             while (meta.level_off.size() > 2 &&
                    meta.level_off[meta.level_off.size() - 1] == meta.level_off[meta.level_off.size() - 2])
@@ -1260,6 +1353,9 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
             starts_ready = true;
         }
     }
+    break;
+    }
+    tv.tol = roots[0].tol;
     if (!done) {
     hipLaunchKernelGGL(k_init_roots, dim3((nsys + 255) / 256), dim3(256), 0, s, nsys, d_sys_off, tv.seg_first,
                        tv.seg_count, tv.level, tv.parent, tv.octant, tv.bsys);
@@ -1320,29 +1416,39 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     // ---- leaves, pass 2 (input order inside each leaf), gather
     int* flags = pool.get<int>("leaf_flag", nb);
     int* fscan = pool.get<int>("leaf_scan", nb);
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, d_total + 1, 1, 0);
-    hipLaunchKernelGGL(k_leaf_flags, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.seg_count, flags,
-                       d_total + 1);
-    scan_exclusive(flags, fscan, nb, d_total, scan_tmp, s);
-    // leaves before each level boundary (for the non-leaf ranges per level) and
-    // boxes per octant (for the M2L tiles) come back with the leaf count in
-    // one round trip
-    // (level offsets: the device copy of the device-driven loop, whose leading
-    // entries are meta.level_off, else uploaded)
+    // leaves before each level boundary (for the non-leaf ranges per level),
+    // boxes per octant (for the M2L tiles), the leaf count and the largest
+    // leaf: from the topology passes (read back with the level offsets), else
+    // gathered here in one more round trip
     const int nlev = (int)meta.level_off.size();
-    const int* d_lvl_idx = d_level_dev;
-    if (!d_lvl_idx) {
+    std::vector<int> got(nlev + 8 + 2);
+    if (!topo_extra.empty()) {
+        hipLaunchKernelGGL(k_leaf_flags, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.seg_count, flags,
+                           nullptr);
+        scan_exclusive(flags, fscan, nb, nullptr, scan_tmp, s);
+        int before = 0;
+        for (int lv = 0; lv < nlev; ++lv) {
+            got[lv] = before;  // leaves of the levels below boundary lv
+            if (lv <= kMaxDepth) before += topo_extra[kTopoLeafCnt + lv];
+        }
+        for (int q = 0; q < 8; ++q) got[nlev + q] = topo_extra[kTopoOctCnt + q];
+        got[nlev + 8] = before;
+        got[nlev + 9] = topo_extra[kTopoMaxLeaf];
+    } else {
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, d_total + 1, 1, 0);
+        hipLaunchKernelGGL(k_leaf_flags, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, tv.seg_count, flags,
+                           d_total + 1);
+        scan_exclusive(flags, fscan, nb, d_total, scan_tmp, s);
         int* up = pool.get<int>("lvl_idx", nlev);
         AFMM_CHECK(hipMemcpyAsync(up, meta.level_off.data(), sizeof(int) * nlev, hipMemcpyHostToDevice, s));
-        d_lvl_idx = up;
+        int* d_lvl_out = pool.get<int>("lvl_out", nlev + 8 + 2);
+        hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, up, nlev, nb, d_lvl_out, d_total, 2);
+        hipLaunchKernelGGL(k_octant_count, dim3(blocks(nb, kOctCountThreads * kOctCountPer)),
+                           dim3(kOctCountThreads), 0, s, nb, tv.octant, d_lvl_out + nlev);
+        d_oct_cnt = d_lvl_out + nlev;
+        AFMM_CHECK(hipMemcpyAsync(got.data(), d_lvl_out, sizeof(int) * (nlev + 8 + 2), hipMemcpyDeviceToHost, s));
+        AFMM_CHECK(hipStreamSynchronize(s));
     }
-    int* d_lvl_out = pool.get<int>("lvl_out", nlev + 8 + 2);
-    hipLaunchKernelGGL(k_gather_at, dim3(1), dim3(64), 0, s, fscan, d_lvl_idx, nlev, nb, d_lvl_out, d_total, 2);
-    hipLaunchKernelGGL(k_octant_count, dim3(blocks(nb, kOctCountThreads * kOctCountPer)), dim3(kOctCountThreads), 0,
-                       s, nb, tv.octant, d_lvl_out + nlev);
-    std::vector<int> got(nlev + 8 + 2);
-    AFMM_CHECK(hipMemcpyAsync(got.data(), d_lvl_out, sizeof(int) * (nlev + 8 + 2), hipMemcpyDeviceToHost, s));
-    AFMM_CHECK(hipStreamSynchronize(s));
     tv.nleaves = got[nlev + 8];
     const int max_leaf = got[nlev + 9];
     tv.leaf_ids = pool.get<int>("leaf_ids", tv.nleaves);
@@ -1411,7 +1517,6 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         tv.nonleaf_ids = pool.get<int>("nonleaf_ids", nb);
         hipLaunchKernelGGL(k_nonleaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.nonleaf_ids);
         const std::vector<int>& bounds = meta.level_off;
-        int* d_out = d_lvl_out;
         meta.nonleaf_off.resize(nlev);
         for (int l = 0; l < nlev; ++l)
             meta.nonleaf_off[l] = bounds[l] - (bounds[l] < nb ? got[l] : tv.nleaves);
@@ -1428,9 +1533,9 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         if (tot != nb) throw std::runtime_error("octant counts do not cover the boxes");
         tv.m2l_tiles256 = pool.get<int>("m2l_tiles256", tv.m2l_ntiles256 + 1);
         tv.m2l_tiles128 = pool.get<int>("m2l_tiles128", tv.m2l_ntiles128 + 1);
-        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_out + nlev, 256, tv.m2l_ntiles256,
+        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_oct_cnt, 256, tv.m2l_ntiles256,
                            tv.m2l_tiles256);
-        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_out + nlev, 128, tv.m2l_ntiles128,
+        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_oct_cnt, 128, tv.m2l_ntiles128,
                            tv.m2l_tiles128);
     }
 

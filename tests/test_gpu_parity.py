@@ -145,6 +145,41 @@ def test_errors_and_edge_cases(afmm):
     assert np.isfinite(acc).all()
 
 
+def test_key_depth_guess_across_evaluations(afmm):
+    """One solver evaluates systems whose key depth (set by the root cube and
+    the eps floor) goes up and down, so the depth guessed from the previous
+    tree is too shallow (the build sorts again), too deep (the same tree must
+    come out) or right; then a NaN input on the guessed path, which must fail
+    and leave the context usable. Every tree bit-exact against the oracle and
+    every force within 1e-12."""
+    rng = np.random.default_rng(4)
+    eps, cap, p = 0.01, 16, 4
+    small = rng.uniform(-0.5, 0.5, (5000, 3))
+    wide = np.concatenate([rng.uniform(-500, 500, (3000, 3)), rng.normal(0, 0.05, (3000, 3))])
+    s = solver(afmm, p, cap, eps)
+    orc = cabi.oracle()
+    for step, pos in enumerate((small, wide, small, wide, wide)):
+        m = rng.uniform(0.5, 1.5, len(pos))
+        print(f"key depth guess: step {step} n={len(m)}", flush=True)
+        acc = s.computeAccelerations(pos, m)
+        t, tr = s.tree(), orc.tree(pos, m, cap, eps)
+        for k in TREE_KEYS:
+            assert np.array_equal(t[k], tr[k]), (step, k)
+        ref, _ = orc.fmm(pos, m, 1.0, eps, p, cap)
+        mx, _ = cabi.rel_err(ref, acc)
+        assert mx <= TOL, (step, mx)
+    bad = small.copy()
+    bad[11, 2] = np.nan
+    with pytest.raises(afmm.AfmmError) as e:
+        s.computeAccelerations(bad, np.ones(len(bad)))
+    assert e.value.code == -4
+    m = np.ones(len(small))
+    acc = s.computeAccelerations(small, m)
+    ref, _ = orc.fmm(small, m, 1.0, eps, p, cap)
+    assert cabi.rel_err(ref, acc)[0] <= TOL
+    s.close()
+
+
 def test_many_more_boxes_than_bodies(afmm):
     """Near-duplicate pairs with leaf capacity 1 chain down to the depth cap:
     about ten boxes per body, so every scan over boxes runs past the sizes
