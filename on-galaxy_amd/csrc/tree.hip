@@ -664,14 +664,32 @@ __global__ void __launch_bounds__(kT) k_lo_keys(int nb, const int* level, const 
 }
 // M2L target order: (octant, level, BFS). A box's separated offsets depend on
 // its octant (not its level), so tiles cut inside one octant group share one
-// set of at most 189 offsets.
-__global__ void __launch_bounds__(kT) k_m2l_keys(int nb, const int* level, const int* octant, uint64_t* khi,
-                                                uint32_t* klo, int* val) {
-    const int b = blockIdx.x * kT + threadIdx.x;
-    if (b >= nb) return;
-    khi[b] = uint64_t(octant[b] * 32 + level[b]);  // level <= kMaxDepth < 32
-    klo[b] = 0;
-    val[b] = b;
+// set of at most 189 offsets. It regroups the (level, octant) order: the
+// group (l, o) keeps its run, moved from the start the (level, octant) sort
+// gave it (digit l*8+o of its scanned histogram, hist[d * nblocks]) to the
+// start of (o, l) among the groups ordered by octant first.
+constexpr int kLoDigits = 8 * (kMaxDepth + 1);  // (level, octant) keys; < 512 buckets
+__global__ void __launch_bounds__(kT) k_m2l_from_lo(int nb, const int* __restrict__ lo_order,
+                                                   const int* __restrict__ level, const int* __restrict__ octant,
+                                                   const int* __restrict__ hist, int nblocks, int* m2l_order) {
+    __shared__ int lo_start[kLoDigits + 1], m2l_start[kLoDigits];
+    for (int d = threadIdx.x; d <= kLoDigits; d += kT) lo_start[d] = hist[(size_t)d * nblocks];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int pos = 0;
+        for (int o = 0; o < 8; ++o)
+            for (int l = 0; l <= kMaxDepth; ++l) {
+                const int d = l * 8 + o;
+                m2l_start[d] = pos;
+                pos += lo_start[d + 1] - lo_start[d];
+            }
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= nb) return;
+    const int b = lo_order[i];
+    const int d = level[b] * 8 + octant[b];
+    m2l_order[m2l_start[d] + (i - lo_start[d])] = b;
 }
 // Boxes per octant, added into cnt[0..8) (zeroed before): block sums in LDS,
 // then eight integer atomics per block (order-free).
@@ -1506,14 +1524,14 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         lb.scan_tmp = pool.get<int>("lo_scan", scan_tmp_ints(512 * sort_blocks(nb) + nb) + 16);
         hipLaunchKernelGGL(k_lo_keys, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.level, tv.octant, lb.khi[0],
                            lb.klo[0], lb.val[0]);
-        const int r = radix_sort(lb, nb, 1, 1, s);
-        tv.lo_order = pool.get<int>("lo_order", nb);
-        AFMM_CHECK(hipMemcpyAsync(tv.lo_order, lb.val[r], sizeof(int) * nb, hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_m2l_keys, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.level, tv.octant, lb.khi[0],
-                           lb.klo[0], lb.val[0]);
-        const int r2 = radix_sort(lb, nb, 1, 1, s);
+        // (the sort's value buffer is the order: nothing sorts with lb again)
+        tv.lo_order = lb.val[radix_sort(lb, nb, 1, 1, s)];
         tv.m2l_order = pool.get<int>("m2l_order", nb);
-        AFMM_CHECK(hipMemcpyAsync(tv.m2l_order, lb.val[r2], sizeof(int) * nb, hipMemcpyDeviceToDevice, s));
+        if (nb > 1)  // (a single box: the sort ran no pass and left no histogram)
+            hipLaunchKernelGGL(k_m2l_from_lo, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.lo_order, tv.level,
+                               tv.octant, lb.hist, sort_blocks(nb), tv.m2l_order);
+        else
+            hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, tv.m2l_order, nb, 0);
         tv.nonleaf_ids = pool.get<int>("nonleaf_ids", nb);
         hipLaunchKernelGGL(k_nonleaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.nonleaf_ids);
         const std::vector<int>& bounds = meta.level_off;
