@@ -20,9 +20,12 @@ import cabi
 
 pytestmark = pytest.mark.gpu
 
-MIX_VS_FP64_RMS = 1e-6   # relative RMS, mixed vs FP64 GPU on the same tree
-MIX_VS_FP64_MAX = 1e-4   # relErr max (floor 1e-6 max|a|)
-MIX_EXTRA_RMS = 1e-6     # allowed growth of relative RMS vs FP64 direct
+# the <= 1e-7 relative RMS target for mixed precision against FP64 (measured
+# 2.0e-8 to 4.2e-8 on these Plummer spheres at p = 4, 8, 12; relErr max at most
+# 1.7e-6)
+MIX_VS_FP64_RMS = 1e-7   # relative RMS, mixed vs FP64 GPU on the same tree
+MIX_VS_FP64_MAX = 1e-5   # relErr max (floor 1e-6 max|a|)
+MIX_EXTRA_RMS = 1e-7     # allowed growth of relative RMS vs FP64 direct
 
 
 @pytest.fixture(scope="module")
