@@ -186,7 +186,9 @@ int sort_blocks(int n);
 // bits [0, 9), digits 1..7 are khi bits [9(g-1), 9g), digits 8 and 9 are klo
 // bits [9, 18) and [18, 27). Sorts digits first..first+count-1 in that order,
 // starting from buffer set `start`; returns the buffer set holding the result.
-int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, int start = 0);
+// lo = false: the klo words are all zero and stay where they are (digits 1..7
+// only; the result's klo is then the input's).
+int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, int start = 0, bool lo = true);
 
 // ---- tree.hip ----------------------------------------------------------------
 // Builds the adaptive octree and all interaction lists for n bodies given as

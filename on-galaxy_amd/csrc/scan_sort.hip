@@ -11,6 +11,7 @@
 // is deterministic and identical to a sequential stable counting sort.
 // ============================================================================
 #include <algorithm>
+#include <stdexcept>
 
 #include "afmm_internal.h"
 #include "block_scan.h"
@@ -82,6 +83,9 @@ __device__ __forceinline__ int digit_of(uint64_t khi, uint32_t klo, int g) {
     return int((khi >> (9 * (g - 1))) & 511u);
 }
 
+// LO = false: the low key words are all zero (one system, at most 21 levels)
+// and are neither read nor moved; digits then come from khi alone (g in 1..7).
+template <bool LO>
 __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* khi, const uint32_t* klo,
                                                             int n, int g, int* hist, int nblocks) {
     __shared__ int h[kBuckets];
@@ -90,7 +94,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* khi,
     const int base = blockIdx.x * kSortTile;
     for (int r = 0; r < kSortRounds; ++r) {
         const int i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[digit_of(khi[i], klo[i], g)], 1);
+        if (i < n) atomicAdd(&h[digit_of(khi[i], LO ? klo[i] : 0u, g)], 1);
     }
     __syncthreads();
     for (int d = threadIdx.x; d < kBuckets; d += kSortThreads) hist[d * nblocks + blockIdx.x] = h[d];
@@ -102,12 +106,13 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* khi,
 // direct scatter sends a wave's 64 writes to up to 64 runs: measured 2.3x
 // slower on passes with many distinct digits). Local digit starts come from
 // the tile's own histogram, global ones from the scanned histogram offs.
+template <bool LO>
 __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* khi, const uint32_t* klo,
                                                                const int* val, int n, int g,
                                                                const int* offs, int nblocks,
                                                                uint64_t* okhi, uint32_t* oklo, int* oval) {
     __shared__ uint64_t skh[kSortTile];
-    __shared__ uint32_t skl[kSortTile];
+    __shared__ uint32_t skl[LO ? kSortTile : 1];
     __shared__ int sval[kSortTile];
     __shared__ int run[kBuckets], lstart[kBuckets], gstart[kBuckets];
     __shared__ int cnt[kSortWaves][kBuckets];
@@ -125,7 +130,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* k
     // the tile's digit histogram, then its exclusive scan (two digits per thread)
     for (int r = 0; r < kSortRounds; ++r) {
         const int i = r * kSortThreads + threadIdx.x;
-        if (i < m) atomicAdd(&run[digit_of(khi[base + i], klo[base + i], g)], 1);
+        if (i < m) atomicAdd(&run[digit_of(khi[base + i], LO ? klo[base + i] : 0u, g)], 1);
     }
     __syncthreads();
     {
@@ -158,7 +163,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* k
         int v = 0, d = 0;
         if (act) {
             kh = khi[base + i];
-            kl = klo[base + i];
+            if constexpr (LO) kl = klo[base + i];
             v = val[base + i];
             d = digit_of(kh, kl, g);
         }
@@ -176,7 +181,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* k
             int pos = run[d] + rank;
             for (int w = 0; w < wave; ++w) pos += cnt[w][d];
             skh[pos] = kh;
-            skl[pos] = kl;
+            if constexpr (LO) skl[pos] = kl;
             sval[pos] = v;
         }
         __syncthreads();
@@ -193,11 +198,11 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* k
     // the staged tile in digit order: item j goes to its digit's global run
     for (int j = threadIdx.x; j < m; j += kSortThreads) {
         const uint64_t kh = skh[j];
-        const uint32_t kl = skl[j];
+        const uint32_t kl = LO ? skl[j] : 0u;
         const int d = digit_of(kh, kl, g);
         const int pos = gstart[d] + (j - lstart[d]);
         okhi[pos] = kh;
-        oklo[pos] = kl;
+        if constexpr (LO) oklo[pos] = kl;
         oval[pos] = sval[j];
     }
 }
@@ -222,17 +227,27 @@ void scan_exclusive_range(const int* d_in, int* d_out, const int* d_rng, int nma
 
 int sort_blocks(int n) { return (n + kSortTile - 1) / kSortTile; }
 
-int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, int start) {
+int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, int start, bool lo) {
     int cur = start;
     if (n <= 1) return cur;
     const int nblocks = sort_blocks(n);
+    if (!lo && (first_digit < 1 || first_digit + ndigits > 8))
+        throw std::runtime_error("radix_sort: digits outside khi without the low key words");
     for (int k = 0; k < ndigits; ++k) {
         const int g = first_digit + k;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, s, b.khi[cur], b.klo[cur],
-                           n, g, b.hist, nblocks);
+        if (lo)
+            hipLaunchKernelGGL(k_sort_hist<true>, dim3(nblocks), dim3(kSortThreads), 0, s, b.khi[cur], b.klo[cur],
+                               n, g, b.hist, nblocks);
+        else
+            hipLaunchKernelGGL(k_sort_hist<false>, dim3(nblocks), dim3(kSortThreads), 0, s, b.khi[cur], nullptr,
+                               n, g, b.hist, nblocks);
         scan_exclusive(b.hist, b.hist, kBuckets * nblocks, nullptr, b.scan_tmp, s);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, b.khi[cur], b.klo[cur],
-                           b.val[cur], n, g, b.hist, nblocks, b.khi[1 - cur], b.klo[1 - cur], b.val[1 - cur]);
+        if (lo)
+            hipLaunchKernelGGL(k_sort_scatter<true>, dim3(nblocks), dim3(kSortThreads), 0, s, b.khi[cur], b.klo[cur],
+                               b.val[cur], n, g, b.hist, nblocks, b.khi[1 - cur], b.klo[1 - cur], b.val[1 - cur]);
+        else
+            hipLaunchKernelGGL(k_sort_scatter<false>, dim3(nblocks), dim3(kSortThreads), 0, s, b.khi[cur], nullptr,
+                               b.val[cur], n, g, b.hist, nblocks, b.khi[1 - cur], nullptr, b.val[1 - cur]);
         cur = 1 - cur;
     }
     return cur;

@@ -1277,10 +1277,12 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         const int ndig = (L + 2) / 3;
         // system digits (klo bits 9..26) after the path digits: sorted by (system, path)
         const int sdig = nsys <= 1 ? 0 : (nsys <= 512 ? 1 : 2);
-        int r1 = radix_sort(sb, n, 8 - ndig, ndig, s);
+        // one system with at most 21 levels: every klo is zero and is not moved
+        const bool lo = nsys > 1 || L > 21;
+        int r1 = radix_sort(sb, n, 8 - ndig, ndig, s, 0, lo);
         if (sdig > 0) r1 = radix_sort(sb, n, 8, sdig, s, r1);
         khi = sb.khi[r1];
-        klo = sb.klo[r1];
+        klo = lo ? sb.klo[r1] : sb.klo[0];
         perm1 = sb.val[r1];
     };
     sort_bodies();
