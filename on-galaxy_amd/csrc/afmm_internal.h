@@ -141,7 +141,9 @@ struct TreeView {
     int* lo_order;       // [B] boxes sorted by (level, octant); level l is [level_off[l], level_off[l+1])
     int* m2l_order;      // [B] boxes sorted by (octant, level): the M2L target order
     // M2L target tiles over m2l_order, cut inside octant groups: tile k is
-    // ranks [m2l_tiles{256,128}[k], [k+1]), at most 256 (128) targets each
+    // ranks [m2l_tiles{256,128}[k], [k+1]), for workgroups of at most 256
+    // (128) targets; below a CU's worth of full tiles, tiles are cut smaller
+    // (a multiple of 64 targets) so that they spread over more CUs
     int *m2l_tiles256, *m2l_tiles128;
     int m2l_ntiles256 = 0, m2l_ntiles128 = 0;
     int* nonleaf_ids;    // [B - nleaves] non-leaf boxes in BFS order

@@ -29,6 +29,8 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include "afmm_internal.h"
 #include "block_scan.h"
@@ -723,6 +725,35 @@ __global__ void __launch_bounds__(kOctCountThreads) k_octant_count(int nb, const
 // split into the fewest near-equal tiles of at most T targets. The host
 // derives the tile count from the same counts with m2l_tile_count().
 __host__ __device__ inline int m2l_tiles_of(int cnt, int T) { return (cnt + T - 1) / T; }
+
+// Targets per M2L tile for workgroups of up to `tile` targets: the smallest
+// multiple of 64 (one 16-target column block per SIMD, so every SIMD of the
+// workgroup has the same share) whose tiles still fit one per CU, else `tile`.
+// A workgroup's time is set by its busiest SIMD's blocks times the offsets, so
+// below a CU's worth of full tiles smaller tiles on more CUs finish sooner.
+int m2l_tile_size(const int* per_octant, int tile, int cus) {
+    for (int t = 64; t < tile; t += 64) {
+        int k = 0;
+        for (int q = 0; q < 8; ++q)
+            if (per_octant[q] > 0) k += m2l_tiles_of(per_octant[q], t);
+        if (k <= cus) return t;
+    }
+    return tile;
+}
+
+// Compute units of the current device (cached per device).
+int device_cus() {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    int dev = 0;
+    AFMM_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    AFMM_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    return cache[dev] = n > 0 ? n : 1;
+}
 __global__ void __launch_bounds__(kT) k_m2l_tile_plan(const int* cnt, int T, int ntiles, int* tstart) {
     __shared__ int c[8], first_tile[9], first_box[9];
     if (threadIdx.x == 0) {
@@ -1543,19 +1574,22 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         // M2L tiles (tile starts built on the device from the same counts)
         int tot = 0;
         tv.m2l_ntiles256 = tv.m2l_ntiles128 = 0;
+        const int* per_octant = got.data() + nlev;
+        const int cus = device_cus();
+        const int t256 = m2l_tile_size(per_octant, 256, cus), t128 = m2l_tile_size(per_octant, 128, cus);
         for (int q = 0; q < 8; ++q) {
-            const int c = got[nlev + q];
+            const int c = per_octant[q];
             if (c <= 0) continue;
             tot += c;
-            tv.m2l_ntiles256 += m2l_tiles_of(c, 256);
-            tv.m2l_ntiles128 += m2l_tiles_of(c, 128);
+            tv.m2l_ntiles256 += m2l_tiles_of(c, t256);
+            tv.m2l_ntiles128 += m2l_tiles_of(c, t128);
         }
         if (tot != nb) throw std::runtime_error("octant counts do not cover the boxes");
         tv.m2l_tiles256 = pool.get<int>("m2l_tiles256", tv.m2l_ntiles256 + 1);
         tv.m2l_tiles128 = pool.get<int>("m2l_tiles128", tv.m2l_ntiles128 + 1);
-        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_oct_cnt, 256, tv.m2l_ntiles256,
+        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_oct_cnt, t256, tv.m2l_ntiles256,
                            tv.m2l_tiles256);
-        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_oct_cnt, 128, tv.m2l_ntiles128,
+        hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_oct_cnt, t128, tv.m2l_ntiles128,
                            tv.m2l_tiles128);
     }
 

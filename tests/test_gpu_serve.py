@@ -100,10 +100,11 @@ def test_budget_and_late_arrival(afmm):
     assert max(seen) <= int(1.5 * b_big)
     assert inf[i2]["t_admit"] >= inf[i1]["t_done"]
     assert inf[i3]["t_admit"] >= inf[i2]["t_admit"]
-    # a late arrival joins a running batch of its order: submit while a long job runs
+    # a late arrival joins a running batch of its order: submit while a long job
+    # runs (300 steps of 30k bodies: several hundred ms, far past the 50 ms wait)
     sv2 = afmm.Server()
     sv2.warm(6)
-    a = sv2.submit(*big[0], G=1.0, eps=0.01, dt=0.01, p=6, steps=30)
+    a = sv2.submit(*big[0], G=1.0, eps=0.01, dt=0.01, p=6, steps=300)
     time.sleep(0.05)
     b = sv2.submit(*small, G=1.0, eps=0.01, dt=0.01, p=6, steps=2)
     sv2.wait(b)
