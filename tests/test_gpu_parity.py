@@ -201,6 +201,29 @@ def test_many_more_boxes_than_bodies(afmm):
     assert mx <= TOL
 
 
+def test_level_loop_for_large_leaf_capacity(afmm):
+    """Leaf capacities above the count pass's window (3840 bodies) build the
+    tree one level at a time (each level's size read back) instead of all
+    levels at once from the sorted keys: tree bit-exact and forces within
+    1e-12 against the oracle."""
+    rng = np.random.default_rng(12)
+    pos = np.concatenate([rng.uniform(-1, 1, (14000, 3)), 0.5 + 0.01 * rng.standard_normal((6000, 3))])
+    m = rng.uniform(0.5, 1.5, len(pos))
+    cap = 4000
+    print(f"level loop: n={len(m)} cap={cap}", flush=True)
+    orc = cabi.oracle()
+    s = solver(afmm, 4, cap, 0.01)
+    acc = s.computeAccelerations(pos, m)
+    t, tr = s.tree(), orc.tree(pos, m, cap, 0.01)
+    assert len(t["leaves"]) > 1
+    for k in TREE_KEYS:
+        assert np.array_equal(t[k], tr[k]), k
+    ref, _ = orc.fmm(pos, m, 1.0, 0.01, 4, cap)
+    mx, _ = cabi.rel_err(ref, acc)
+    assert mx <= TOL
+    s.close()
+
+
 @pytest.mark.parametrize("ndup", [1500, 3000])
 def test_large_duplicate_leaf(afmm, ndup):
     """A cluster of identical bodies stops splitting at the depth cap and stays
