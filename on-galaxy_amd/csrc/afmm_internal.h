@@ -143,9 +143,11 @@ struct TreeView {
     // M2L target tiles over m2l_order, cut inside octant groups: tile k is
     // ranks [m2l_tiles{256,128}[k], [k+1]), for workgroups of at most 256
     // (128) targets; below a CU's worth of full tiles, tiles are cut smaller
-    // (a multiple of 64 targets) so that they spread over more CUs
+    // (a multiple of 64 targets, or 16 or 32 for the 256 plan) so that they
+    // spread over more CUs
     int *m2l_tiles256, *m2l_tiles128;
     int m2l_ntiles256 = 0, m2l_ntiles128 = 0;
+    int m2l_t256 = 256;  // targets per tile of the 256 plan (16 and 32: row-split workgroups)
     int* nonleaf_ids;    // [B - nleaves] non-leaf boxes in BFS order
     // lists
     int *coll, *ncoll;   // [27B], [B]

@@ -470,7 +470,10 @@ struct M2LTypes<true> {
 __device__ __forceinline__ double a_value(double a) { return a; }
 __device__ __forceinline__ double a_value(float2 a) { return (double)a.x + (double)a.y; }
 
-template <int P, bool MIX, int W>
+// RSX (0: the default row split of the order and form) lets more waves share a
+// column block when tiles are small (16 or 32 targets): each wave then runs a
+// shorter chain per offset, and every row keeps its accumulation order.
+template <int P, bool MIX, int W, int RSX = 0>
 __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const int* __restrict__ tstart,
                                                            const int* __restrict__ order,
                                                            const int* __restrict__ srcT,
@@ -485,7 +488,7 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
     // RS waves share each 16-target column block, each taking RBH of the RB
     // row blocks (the tail rows go to the last of them): RS = 2 when a whole
     // column block's accumulators would not fit four waves per SIMD.
-    constexpr int RS = m2l_row_split(P, MIX, W);
+    constexpr int RS = RSX ? RSX : m2l_row_split(P, MIX, W);
     constexpr int RBH = (RB + RS - 1) / RS;
     constexpr int TILE = W / RS * 16 * CB;
     constexpr int NT = W * 64;
@@ -1437,7 +1440,18 @@ void m2l_launch(Pool& pool, hipStream_t s, const TreeView& t, const FmmTables& t
         else if (form == kPrecMixed)
             hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
                                present, tab.m2l_hl, M, L);
-        else
+        else if constexpr (W == 16 && m2l_row_split(P, false, W) == 1) {
+            // tiles of 16 or 32 targets (small trees): four or two waves per block
+            if (t.m2l_t256 <= 16)
+                hipLaunchKernelGGL((k_m2l_mfma<P, false, W, 4>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                                   present, tab.m2l_sw, M, L);
+            else if (t.m2l_t256 <= 32)
+                hipLaunchKernelGGL((k_m2l_mfma<P, false, W, 2>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                                   present, tab.m2l_sw, M, L);
+            else
+                hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                                   present, tab.m2l_sw, M, L);
+        } else
             hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
                                present, tab.m2l_sw, M, L);
     };

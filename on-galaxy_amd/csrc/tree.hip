@@ -727,17 +727,24 @@ __global__ void __launch_bounds__(kOctCountThreads) k_octant_count(int nb, const
 __host__ __device__ inline int m2l_tiles_of(int cnt, int T) { return (cnt + T - 1) / T; }
 
 // Targets per M2L tile for workgroups of up to `tile` targets: the smallest
-// multiple of 64 (one 16-target column block per SIMD, so every SIMD of the
-// workgroup has the same share) whose tiles still fit one per CU, else `tile`.
-// A workgroup's time is set by its busiest SIMD's blocks times the offsets, so
-// below a CU's worth of full tiles smaller tiles on more CUs finish sooner.
+// size whose tiles still fit one per CU, else `tile`. A workgroup's time is
+// set by its busiest SIMD's chain of MFMAs per offset times the offsets, so
+// below a CU's worth of full tiles smaller tiles on more CUs finish sooner:
+// multiples of 64 targets (one 16-target column block per SIMD), and for the
+// 16-wave FP64 workgroups (the 256 plan) also 32 and 16 targets, whose blocks
+// the launch splits by rows over two and four waves (shorter chains).
 int m2l_tile_size(const int* per_octant, int tile, int cus) {
-    for (int t = 64; t < tile; t += 64) {
+    auto tiles = [&](int t) {
         int k = 0;
         for (int q = 0; q < 8; ++q)
             if (per_octant[q] > 0) k += m2l_tiles_of(per_octant[q], t);
-        if (k <= cus) return t;
-    }
+        return k;
+    };
+    if (tile == 256)
+        for (int t : {16, 32})
+            if (tiles(t) <= cus) return t;
+    for (int t = 64; t < tile; t += 64)
+        if (tiles(t) <= cus) return t;
     return tile;
 }
 
@@ -1577,6 +1584,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         const int* per_octant = got.data() + nlev;
         const int cus = device_cus();
         const int t256 = m2l_tile_size(per_octant, 256, cus), t128 = m2l_tile_size(per_octant, 128, cus);
+        tv.m2l_t256 = t256;
         for (int q = 0; q < 8; ++q) {
             const int c = per_octant[q];
             if (c <= 0) continue;

@@ -110,6 +110,25 @@ def test_row_split_m2l_bit_identical(afmm, prec_name, p):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("p", [4, 8])
+def test_small_tile_row_split_bit_identical(afmm, p):
+    """Small trees cut M2L into tiles of 16 or 32 targets, whose 16-target
+    blocks the 16-wave FP64 workgroups split by rows over four or two waves
+    (64 and more targets: one wave per block): the bits of the 8-wave
+    workgroups (tiles of 64 or more targets, one wave per block) at every
+    size."""
+    for n in (3000, 80000, 200000):
+        pos, m = bodies(n, seed=13)
+        print(f"row split by tile size: n={n} p={p}", flush=True)
+        out = []
+        for w in (8, 16):
+            s = afmm.AdaptiveFMM3D(G=1.0, eps=0.002, p=p, threshold=64)
+            s.set_shapes(m2l_waves=w)
+            out.append(s.computeAccelerations(pos, m))
+            s.close()
+        assert np.array_equal(out[0], out[1]), n
+
+
 def test_autotuner_returns_accepted_shapes(afmm):
     import importlib.util
     import os
