@@ -138,7 +138,7 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     };
     build_forest(c->pool, s, pos, mass, nsys, sys_off, eps, c->prm.leaf_cap, tv, meta, roots,
                  split ? TreeHook(on_core) : TreeHook(), split ? TreeHook(on_vlists) : TreeHook(),
-                 lists_beside ? c->side : nullptr, c->list_fork, &c->key_levels_hint);
+                 lists_beside ? c->side : nullptr, c->list_fork, &c->key_levels_hint, &c->tree_depth_hint);
     for (const RootInfo& r : roots)
         if (r.nonfinite) return fail(c, AFMM_ERR_NONFINITE, "position or mass is NaN or Inf");
     if (lists_beside) {

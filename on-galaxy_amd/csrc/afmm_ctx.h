@@ -25,6 +25,7 @@ struct afmm_ctx {
     afmm::TreeMeta meta;
     afmm::RootInfo root{};  // of system 0
     int key_levels_hint = -1;  // key depth of the last tree (the next build's guess)
+    int tree_depth_hint = -1;  // deepest level of the last tree (the levels the next sort orders)
     afmm::Shapes shapes{};  // kernel shapes (afmm_set_shapes), defaults resolved at create
     afmm::HostStager stage;  // pinned staging of host arrays (afmm_eval, afmm_eval_batch)
     int nsys = 1;

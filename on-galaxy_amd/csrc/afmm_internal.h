@@ -213,12 +213,16 @@ int radix_sort(SortBufs& b, int n, int first_digit, int ndigits, hipStream_t s, 
 // keeps the GPU busy meanwhile). levels_hint (optional, in and out): the key
 // depth of the previous build, taken as a guess that saves the root cubes'
 // own round trip (verified, and the sort redone if too shallow); set to this
-// build's depth.
+// build's depth. depth_hint (optional, in and out): the previous tree's
+// deepest level, taken as a guess of the levels the key sort must order
+// (verified by the topology passes, all levels sorted again if too shallow);
+// set to this tree's deepest level.
 using TreeHook = std::function<void(const TreeView&, const TreeMeta&)>;
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
                   std::vector<RootInfo>& roots, const TreeHook& on_core = {}, const TreeHook& on_vlists = {},
-                  hipStream_t ls = nullptr, hipEvent_t ls_ready = nullptr, int* levels_hint = nullptr);
+                  hipStream_t ls = nullptr, hipEvent_t ls_ready = nullptr, int* levels_hint = nullptr,
+                  int* depth_hint = nullptr);
 
 // ---- fmm.hip -----------------------------------------------------------------
 struct FmmTimes {

@@ -320,8 +320,10 @@ constexpr int kTopoItems = (kTopoTile + kTopoWinMax) / kTopoTile;
 static_assert(kTopoTile == 4 * kWave, "four waves per tile");
 // Counts the passes gather for the host with the level offsets: leaves per
 // level [0, kMaxDepth], boxes per octant, the largest leaf's body count.
+// Also whether some body's leaf lies below the levels the topology passes took
+// (nlev, the digits the sort ordered): then the build is redone with all digits.
 constexpr int kTopoLeafCnt = 0, kTopoOctCnt = kMaxDepth + 1, kTopoMaxLeaf = kTopoOctCnt + 8;
-constexpr int kTopoExtra = kTopoMaxLeaf + 1;
+constexpr int kTopoDeep = kTopoMaxLeaf + 1, kTopoExtra = kTopoDeep + 1;
 
 // Levels (digits) two keys of one system share; -1 for keys of two systems.
 // Level l's digit sits at khi bits [63-3l, 66-3l) (bit 63 is always 0) for
@@ -342,7 +344,8 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_count(int n, int cap, int nl
                                                           const uint64_t* __restrict__ khi,
                                                           const uint32_t* __restrict__ klo, int* __restrict__ dlev,
                                                           int* __restrict__ minlev, int* __restrict__ nstart,
-                                                          int* __restrict__ tcnt, int* __restrict__ ovf) {
+                                                          int* __restrict__ tcnt, int* __restrict__ ovf,
+                                                          int* __restrict__ part) {
     __shared__ int m[kTopoTile + kTopoWinMax];
     __shared__ int wc[kMaxDepth + 1][kTopoTile / kWave];
     const int tid = threadIdx.x, base = blockIdx.x * kTopoTile;
@@ -375,13 +378,16 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_count(int n, int cap, int nl
     __syncthreads();
     const int i = base + tid;
     int a = kMaxDepth, d = -1;  // a body past the end starts nothing
+    bool deep = false;
     if (i < n) {
         const int M = max(m[tid], m[tid + cap + 1 - P]);  // windows [tid, tid + cap]
         const uint64_t h = khi[i];
         const uint32_t lo = klo[i];
-        // (a guessed depth nlev below the system's own stops every body there:
-        // that build is only checked and then redone, but it stays consistent)
-        d = min(min(roots[lo >> 9].key_levels, nlev), M + 1);
+        // (nlev below the system's own depth, a guess, stops every body there:
+        // such a build is only checked and then redone, but it stays consistent)
+        const int full = min(roots[lo >> 9].key_levels, M + 1);
+        deep = full > nlev;
+        d = min(full, nlev);
         a = i > 0 ? common_levels(khi[i - 1], klo[i - 1], h, lo) : -1;
         dlev[i] = d;
         minlev[i] = a + 1;
@@ -395,6 +401,8 @@ __global__ void __launch_bounds__(kTopoTile) k_topo_count(int n, int cap, int nl
     __syncthreads();
     if (tid <= nlev) tcnt[(size_t)tid * ntiles + blockIdx.x] = wc[tid][0] + wc[tid][1] + wc[tid][2] + wc[tid][3];
     if (blockIdx.x == 0 && tid == 0) *ovf = 0;
+    const int anydeep = __syncthreads_or(deep ? 1 : 0);
+    if (tid == 0) part[(size_t)kTopoDeep * ntiles + blockIdx.x] = anydeep;
 }
 
 // Per level (one block each): exclusive scan of the tile counts in place, and
@@ -582,7 +590,7 @@ __global__ void __launch_bounds__(kScanThreads) k_topo_sums(int ntiles, const in
     const int k = blockIdx.x;
     const int* row = part + (size_t)k * ntiles;
     int v = 0;
-    if (k == kTopoMaxLeaf) {
+    if (k == kTopoMaxLeaf || k == kTopoDeep) {
         for (int t = threadIdx.x; t < ntiles; t += kScanThreads) v = max(v, row[t]);
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
@@ -1229,7 +1237,7 @@ int read_int(const int* d, hipStream_t s) {
 void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* d_mass, int nsys,
                   const int* h_sys_off, const double* h_eps_floor, int cap, TreeView& tv, TreeMeta& meta,
                   std::vector<RootInfo>& roots, const TreeHook& on_core, const TreeHook& on_vlists,
-                  hipStream_t ls, hipEvent_t ls_ready, int* levels_hint) {
+                  hipStream_t ls, hipEvent_t ls_ready, int* levels_hint, int* depth_hint) {
     const int n = h_sys_off[nsys];
     // ---- per-system root cubes: chunks of at most kBBoxChunk bodies of one system
     std::vector<int> hb;  // [cbeg | cend | cfirst(nsys+1) | sys_off(nsys+1)] as one upload
@@ -1309,10 +1317,19 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     const uint64_t* khi = nullptr;
     const uint32_t* klo = nullptr;
     const int* perm1 = nullptr;
+    // Levels the sort orders (three per 9-bit digit): every key level, or, with
+    // the depth of the context's previous tree as a guess, the levels down to
+    // it. The topology only looks that deep, the in-leaf order is the input
+    // order whatever the deeper digits, and a body whose leaf lies deeper
+    // sends the build round again with all levels.
+    const int dguess = depth_hint && *depth_hint > 0 ? *depth_hint : kMaxDepth;
+    int D = 0;
+    auto sorted_levels = [&]() { D = std::min(L, 3 * ((std::min(dguess, L) + 2) / 3)); };
+    sorted_levels();
     auto sort_bodies = [&]() {
         hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kT), 0, s, d_pos, n, nsys, d_sys_off, d_roots, sb.khi[0],
                            sb.klo[0], sb.val[0]);
-        const int ndig = (L + 2) / 3;
+        const int ndig = (D + 2) / 3;
         // system digits (klo bits 9..26) after the path digits: sorted by (system, path)
         const int sdig = nsys <= 1 ? 0 : (nsys <= 512 ? 1 : 2);
         // one system with at most 21 levels: every klo is zero and is not moved
@@ -1354,7 +1371,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     // for the worst case where that bound is modest. Otherwise, or when the
     // boxes overflow the arrays, the loop below builds one level at a time
     // and reads each level's size back.
-    for (;;) {  // twice when a guessed key depth was too shallow
+    for (;;) {  // again when a guessed key depth or sorted depth was too shallow
     size_t bound = (size_t)nsys;
     for (int lv = 0; lv < L; ++lv) {
         const double full = (double)nsys * std::pow(8.0, lv);
@@ -1367,42 +1384,48 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     box_arrays(cap_boxes, false);
     if (topo_levels) {
         const int ntiles = (n + 1 + kTopoTile - 1) / kTopoTile;  // bodies 0..n (n: the ends of the last boxes)
-        // level offsets [0, L + 1], the overflow flag, then the counts
-        int* lev = pool.get<int>("level_dev", L + 3 + kTopoExtra);
-        int* ovf = lev + L + 2;
-        int* extra = lev + L + 3;
+        // levels 0..D (the sorted ones): offsets [0, D + 1], the overflow flag, then the counts
+        int* lev = pool.get<int>("level_dev", D + 3 + kTopoExtra);
+        int* ovf = lev + D + 2;
+        int* extra = lev + D + 3;
         int* dl = pool.get<int>("topo_leaf_level", n + 1);
-        int* tcnt = pool.get<int>("topo_tiles", (size_t)(L + 1) * ntiles);
-        int* ltot = pool.get<int>("topo_totals", L + 1);
+        int* tcnt = pool.get<int>("topo_tiles", (size_t)(D + 1) * ntiles);
+        int* ltot = pool.get<int>("topo_totals", D + 1);
         int* part = pool.get<int>("topo_partials", (size_t)kTopoExtra * ntiles);
-        hipLaunchKernelGGL(k_topo_count, dim3(ntiles), dim3(kTopoTile), 0, s, n, cap, L, ntiles, d_roots, khi, klo,
-                           dl, minlev, cnt_start, tcnt, ovf);
-        hipLaunchKernelGGL(k_topo_scan, dim3(L + 1), dim3(kScanThreads), 0, s, ntiles, tcnt, ltot);
-        hipLaunchKernelGGL(k_topo_emit, dim3(ntiles), dim3(kTopoTile), 0, s, n, L, ntiles, (int)cap_boxes, khi, klo,
+        hipLaunchKernelGGL(k_topo_count, dim3(ntiles), dim3(kTopoTile), 0, s, n, cap, D, ntiles, d_roots, khi, klo,
+                           dl, minlev, cnt_start, tcnt, ovf, part);
+        hipLaunchKernelGGL(k_topo_scan, dim3(D + 1), dim3(kScanThreads), 0, s, ntiles, tcnt, ltot);
+        hipLaunchKernelGGL(k_topo_emit, dim3(ntiles), dim3(kTopoTile), 0, s, n, D, ntiles, (int)cap_boxes, khi, klo,
                            dl, minlev, tcnt, ltot, lev, ovf, tv.seg_first, tv.level, tv.parent, tv.octant,
                            tv.child_first, tv.nchild, tv.child_mask, tv.bsys, part);
-        hipLaunchKernelGGL(k_topo_ends, dim3(ntiles), dim3(kTopoTile), 0, s, n, L, ntiles, (int)cap_boxes, dl, minlev,
+        hipLaunchKernelGGL(k_topo_ends, dim3(ntiles), dim3(kTopoTile), 0, s, n, D, ntiles, (int)cap_boxes, dl, minlev,
                            tcnt, ltot, tv.seg_first, tv.child_first, tv.octant, tv.seg_count, tv.nchild,
                            tv.child_mask, part);
         hipLaunchKernelGGL(k_topo_sums, dim3(kTopoExtra), dim3(kScanThreads), 0, s, ntiles, part, extra);
-        int* h_lev = pool.host<int>("h_level_dev", L + 3 + kTopoExtra);
-        AFMM_CHECK(hipMemcpyAsync(h_lev, lev, sizeof(int) * (L + 3 + kTopoExtra), hipMemcpyDeviceToHost, s));
+        int* h_lev = pool.host<int>("h_level_dev", D + 3 + kTopoExtra);
+        AFMM_CHECK(hipMemcpyAsync(h_lev, lev, sizeof(int) * (D + 3 + kTopoExtra), hipMemcpyDeviceToHost, s));
         AFMM_CHECK(hipStreamSynchronize(s));  // (with a guessed depth also the root cubes and hb)
-        const std::vector<int> hl(h_lev, h_lev + L + 3 + kTopoExtra);
+        const std::vector<int> hl(h_lev, h_lev + D + 3 + kTopoExtra);
         if (guess) {
             const int actual = roots_back();
             if (actual < 0) return;
             guess = false;
             if (actual > L) {
                 L = actual;
+                sorted_levels();
                 sort_bodies();
                 continue;
             }
         }
-        if (hl[L + 2] == 0) {
-            topo_extra.assign(hl.begin() + L + 3, hl.end());
+        if (hl[D + 3 + kTopoDeep] && D < L) {  // a leaf below the sorted levels: all levels
+            D = L;
+            sort_bodies();
+            continue;
+        }
+        if (hl[D + 2] == 0) {
+            topo_extra.assign(hl.begin() + D + 3, hl.end());
             d_oct_cnt = extra + kTopoOctCnt;
-            meta.level_off.assign(hl.begin(), hl.begin() + L + 2);
+            meta.level_off.assign(hl.begin(), hl.begin() + D + 2);
             while (meta.level_off.size() > 2 &&
                    meta.level_off[meta.level_off.size() - 1] == meta.level_off[meta.level_off.size() - 2])
                 meta.level_off.pop_back();
@@ -1442,6 +1465,7 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     }
     }
     tv.max_level = l;
+    if (depth_hint) *depth_hint = l;
     const int nb = meta.level_off.back();
     tv.nb = nb;
     // scans over boxes from here on: chains of single-child boxes (near

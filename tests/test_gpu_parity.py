@@ -147,10 +147,11 @@ def test_errors_and_edge_cases(afmm):
 
 def test_key_depth_guess_across_evaluations(afmm):
     """One solver evaluates systems whose key depth (set by the root cube and
-    the eps floor) goes up and down, so the depth guessed from the previous
-    tree is too shallow (the build sorts again), too deep (the same tree must
-    come out) or right; then a NaN input on the guessed path, which must fail
-    and leave the context usable. Every tree bit-exact against the oracle and
+    the eps floor) and tree depth go up and down, so the depths guessed from
+    the previous tree (the key depth, and the levels the key sort orders) are
+    too shallow (the build sorts again), too deep (the same tree must come
+    out) or right; then a NaN input on the guessed path, which must fail and
+    leave the context usable. Every tree bit-exact against the oracle and
     every force within 1e-12."""
     rng = np.random.default_rng(4)
     eps, cap, p = 0.01, 16, 4
