@@ -851,18 +851,29 @@ constexpr int kLeafRankMax = 2048;  // per-leaf ranking is O(c^2 / 64) per lane
 // [f, f + c) is its input indices perm1[f .. f + c) ascending (what a stable
 // sort of all bodies by leaf start would give), here ranked inside the leaf by
 // one wave (indices are distinct). Also leaf_of[slot] = b.
+// The ranked body is gathered into its slot at once (positions, mass, input
+// index), in place of a separate pass over the permutation.
 __global__ void __launch_bounds__(kT) k_leaf_order(int nleaves, const int* leaf_ids, const int* seg_first,
                                                   const int* seg_count, const int* __restrict__ perm1,
-                                                  int* __restrict__ out, int* __restrict__ leaf_of) {
+                                                  const double* __restrict__ pos, const double* __restrict__ mass,
+                                                  double* __restrict__ x, double* __restrict__ y,
+                                                  double* __restrict__ z, double* __restrict__ m,
+                                                  int* __restrict__ orig, int* __restrict__ leaf_of) {
     const int li = blockIdx.x * (kT / 64) + (int)(threadIdx.x >> 6);
     if (li >= nleaves) return;
     const int b = leaf_ids[li];
     const int f = seg_first[b], c = seg_count[b];
     for (int k = (int)(threadIdx.x & 63); k < c; k += 64) {
         const int v = perm1[f + k];
+        const double px = pos[3 * (size_t)v], py = pos[3 * (size_t)v + 1], pz = pos[3 * (size_t)v + 2];
+        const double pm = mass[v];
         int r = 0;
         for (int j = 0; j < c; ++j) r += perm1[f + j] < v;
-        out[f + r] = v;
+        x[f + r] = px;
+        y[f + r] = py;
+        z[f + r] = pz;
+        m[f + r] = pm;
+        orig[f + r] = v;
         leaf_of[f + k] = b;
     }
 }
@@ -1536,14 +1547,18 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     tv.leaf_ids = pool.get<int>("leaf_ids", tv.nleaves);
     hipLaunchKernelGGL(k_leaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.leaf_ids);
     tv.leaf_of = pool.get<int>("leaf_of", n);
-    // input order inside the leaves: ranked per leaf when leaves are small (one
-    // kernel), else a stable radix sort of all bodies by leaf start
-    const int* leaf_perm;
+    // input order inside the leaves, and the bodies gathered into it: ranked
+    // per leaf when leaves are small (one kernel), else a stable radix sort of
+    // all bodies by leaf start and a gather
+    tv.x = pool.get<double>("bx", n);
+    tv.y = pool.get<double>("by", n);
+    tv.z = pool.get<double>("bz", n);
+    tv.m = pool.get<double>("bm", n);
+    tv.orig = pool.get<int>("orig", n);
     if (max_leaf <= kLeafRankMax) {
-        int* lp = pool.get<int>("leaf_perm", n);
         hipLaunchKernelGGL(k_leaf_order, dim3(blocks(64LL * tv.nleaves)), dim3(kT), 0, s, tv.nleaves, tv.leaf_ids,
-                           tv.seg_first, tv.seg_count, perm1, lp, tv.leaf_of);
-        leaf_perm = lp;
+                           tv.seg_first, tv.seg_count, perm1, d_pos, d_mass, tv.x, tv.y, tv.z, tv.m, tv.orig,
+                           tv.leaf_of);
     } else {
         SortBufs lk;  // buffers of their own: the pass-1 keys in sb are read later
         for (int k = 0; k < 2; ++k) {
@@ -1557,15 +1572,10 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
                            tv.seg_first, tv.seg_count, perm1, lk.khi[0], lk.klo[0], lk.val[0], tv.leaf_of);
         int bits = 1;
         while ((1LL << bits) <= n) ++bits;
-        leaf_perm = lk.val[radix_sort(lk, n, 1, (bits + 8) / 9, s)];
+        const int* leaf_perm = lk.val[radix_sort(lk, n, 1, (bits + 8) / 9, s)];
+        hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, n, leaf_perm, d_pos, d_mass, tv.x, tv.y,
+                           tv.z, tv.m, tv.orig);
     }
-    tv.x = pool.get<double>("bx", n);
-    tv.y = pool.get<double>("by", n);
-    tv.z = pool.get<double>("bz", n);
-    tv.m = pool.get<double>("bm", n);
-    tv.orig = pool.get<int>("orig", n);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(kT), 0, s, n, leaf_perm, d_pos, d_mass, tv.x, tv.y, tv.z,
-                       tv.m, tv.orig);
 
     // ---- geometry
     tv.cfp = pool.get<double>("cfp", 3 * (size_t)nb);
