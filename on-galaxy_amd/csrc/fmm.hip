@@ -383,24 +383,24 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
                                                int* __restrict__ present) {
     __shared__ int tab[kNumOffsets][kSrcTargets];
     __shared__ int box[kSrcTargets], cnt[kSrcTargets], tile_of[kSrcTargets];
-    const int r0 = blockIdx.x * kSrcTargets, tid = threadIdx.x;
+    // consecutive blocks on one XCD, so the two 64-byte halves of a 128-byte
+    // row segment are written through the same L2
+    const int r0 = xcd_item() * kSrcTargets, tid = threadIdx.x;
+    if (r0 >= nb) return;
     const int nt = min(kSrcTargets, nb - r0);
     for (int e = tid; e < kNumOffsets * kSrcTargets; e += kT) (&tab[0][0])[e] = -1;
     if (tid < nt) {
         box[tid] = order[r0 + tid];
         cnt[tid] = nv[box[tid]];
-    }
-    if (tid == 0) {  // the tile of each target: the last k with tstart[k] <= rank
+        // the tile of the target: the last k with tstart[k] <= rank (one
+        // search per target, side by side)
         int lo = 0, hi = ntiles - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (tstart[mid] <= r0) lo = mid;
+            if (tstart[mid] <= r0 + tid) lo = mid;
             else hi = mid - 1;
         }
-        for (int j = 0; j < nt; ++j) {
-            while (lo + 1 < ntiles && tstart[lo + 1] <= r0 + j) ++lo;
-            tile_of[j] = lo;
-        }
+        tile_of[tid] = lo;
     }
     __syncthreads();
     for (int w = tid; w < nt * kVSlots; w += kT) {
@@ -1372,7 +1372,8 @@ void m2l_prep(hipStream_t s, const TreeView& t, int np, int ntiles, const int* t
     const int nb = t.nb;
     hipLaunchKernelGGL(k_zero_int, dim3(blocks((long long)ntiles * kNumOffsets)), dim3(kT), 0, s, present,
                        (size_t)ntiles * kNumOffsets);
-    hipLaunchKernelGGL(k_m2l_src, dim3((nb + kSrcTargets - 1) / kSrcTargets), dim3(kT), 0, s, nb, np, t.m2l_order,
+    hipLaunchKernelGGL(k_m2l_src, dim3(xcd_grid((nb + kSrcTargets - 1) / kSrcTargets)), dim3(kT), 0, s, nb, np,
+                       t.m2l_order,
                        t.vl, t.nv, t.ic, srcT, tstart, ntiles, present);
 }
 
