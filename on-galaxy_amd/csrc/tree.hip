@@ -1586,7 +1586,21 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
     hipLaunchKernelGGL(k_geometry, dim3(blocks(nb)), dim3(kT), 0, s, nb, d_roots, tv.bsys, tv.seg_first, tv.level,
                        khi, klo, tv.cfp, tv.cdd, tv.half, tv.invh, tv.ic);
 
-    // ---- (level, octant) order; non-leaf boxes per level
+    // ---- non-leaf boxes per level (what the upward pass reads besides the
+    // bodies, leaves and geometry)
+    tv.nonleaf_ids = pool.get<int>("nonleaf_ids", nb);
+    hipLaunchKernelGGL(k_nonleaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.nonleaf_ids);
+    {
+        const std::vector<int>& bounds = meta.level_off;
+        meta.nonleaf_off.resize(nlev);
+        for (int l = 0; l < nlev; ++l)
+            meta.nonleaf_off[l] = bounds[l] - (bounds[l] < nb ? got[l] : tv.nleaves);
+    }
+
+    if (on_core) on_core(tv, meta);
+
+    // ---- (level, octant) order and the M2L target order and tiles (after the
+    // hook: the upward pass needs none of them)
     {
         SortBufs lb;
         for (int k = 0; k < 2; ++k) {
@@ -1606,12 +1620,6 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
                                tv.octant, lb.hist, sort_blocks(nb), tv.m2l_order);
         else
             hipLaunchKernelGGL(k_fill, dim3(1), dim3(kT), 0, s, tv.m2l_order, nb, 0);
-        tv.nonleaf_ids = pool.get<int>("nonleaf_ids", nb);
-        hipLaunchKernelGGL(k_nonleaf_ids, dim3(blocks(nb)), dim3(kT), 0, s, nb, tv.nchild, fscan, tv.nonleaf_ids);
-        const std::vector<int>& bounds = meta.level_off;
-        meta.nonleaf_off.resize(nlev);
-        for (int l = 0; l < nlev; ++l)
-            meta.nonleaf_off[l] = bounds[l] - (bounds[l] < nb ? got[l] : tv.nleaves);
         // M2L tiles (tile starts built on the device from the same counts)
         int tot = 0;
         tv.m2l_ntiles256 = tv.m2l_ntiles128 = 0;
@@ -1634,8 +1642,6 @@ void build_forest(Pool& pool, hipStream_t s, const double* d_pos, const double* 
         hipLaunchKernelGGL(k_m2l_tile_plan, dim3(1), dim3(kT), 0, s, d_oct_cnt, t128, tv.m2l_ntiles128,
                            tv.m2l_tiles128);
     }
-
-    if (on_core) on_core(tv, meta);
 
     // ---- colleagues, V, M2L source table
     tv.coll = pool.get<int>("coll", kCollSlots * (size_t)nb);
