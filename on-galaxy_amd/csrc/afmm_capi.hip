@@ -95,13 +95,15 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     // P2L, M2P and P2P run on the second stream beside it, after the upward
     // pass: their VALU work shares the SIMDs with the MFMA-bound M2L, and the
     // lists' size readback synchronises the second stream only. The
-    // automatic choice, from the 1M merger: beside for the bf16 (p = 8: 4.60
-    // -> 4.46 ms, p = 12: 15.9 -> 15.2 ms) and split-FP32 (7.75 -> 7.54 ms)
-    // forms and for FP64 from p = 14 (16-wave row-split M2L: 44.8 -> 41.9 ms
-    // at p = 14, 78.8 -> 73.8 ms at p = 16); FP64 in order at p = 8 (equal
-    // either way) and p = 12 (24.4 ms against 28.2 ms beside).
+    // automatic choice, from the merger at 16k-1M bodies: beside for the bf16
+    // (1M, p = 8: 4.34 -> 4.13 ms) and split-FP32 (7.75 -> 7.54 ms) forms, and
+    // for FP64 up to p = 8 (1M: 5.79 -> 5.67 ms at p = 8, 2.51 -> 2.45 ms at
+    // p = 4; 16k: 1.48 -> 1.26 ms at p = 8) and from p = 14 (16-wave
+    // row-split M2L: 44.8 -> 41.9 ms at p = 14, 78.8 -> 73.8 ms at p = 16);
+    // FP64 in order at p = 9-13, where the longer M2L loses more beside the
+    // VALU passes than they save at 1M (p = 12: 23.9 ms against 27.5 ms).
     const bool bf16_m2l = m2l_form(c->prm.precision) == kPrecMixedBf16;
-    const bool auto_beside = m2l_form(c->prm.precision) != kPrecFP64 || c->prm.p >= 14;
+    const bool auto_beside = m2l_form(c->prm.precision) != kPrecFP64 || c->prm.p <= 8 || c->prm.p >= 14;
     const bool lists_beside = split && !c->use_graph &&
                               (c->overlap_mode == AFMM_OVERLAP_BESIDE_M2L ||
                                (c->overlap_mode == AFMM_OVERLAP_AUTO && auto_beside));
