@@ -255,14 +255,18 @@ constexpr int m2p_min_waves() { return P <= 8 ? 4 : 1; }
 
 // ADD: far += the field (after L2P wrote far); otherwise far = the field, zero
 // for leaves without W (the separate array L2P adds later; the same sums).
+// order: the leaves by descending work (m2p_order); workgroup k takes leaf
+// order[k], which changes when a leaf runs, not what it sums.
 template <int P, bool POT, bool ADD>
 __global__ void __launch_bounds__(kM2PWaves * 64, m2p_min_waves<P>()) k_m2p(TreeView t, const double* __restrict__ M,
-                                                       double* __restrict__ far, double* __restrict__ fpot) {
+                                                       double* __restrict__ far, double* __restrict__ fpot,
+                                                       const int* __restrict__ order) {
     constexpr int S = nterms(P), L = kM2PWaves * 64, GM = m2p_groups<P>();
     __shared__ double mw[GM][S];
     __shared__ double red[4][L];
-    const int li = xcd_chunk_item();
-    if (li >= t.nleaves) return;
+    const int item = xcd_chunk_item();
+    if (item >= t.nleaves) return;
+    const int li = order[item];
     const int wa = t.w_off[li], nw = t.w_off[li + 1] - wa;
     const int b = t.leaf_ids[li];
     const int f = t.seg_first[b], nt = t.seg_count[b];
@@ -1349,6 +1353,45 @@ void launch_p2p(hipStream_t s, const TreeView& t, bool mixed, const double* far,
                            order);
 }
 
+// M2P's leaf order: a workgroup's work is its rounds, ceil(n / T) * ceil(w / G)
+// for a leaf of n bodies and w W entries with k_m2p's lane groups (T lanes,
+// G = L / T groups), and the leaves are sorted by descending rounds (capped at
+// 511), stable, so leaves of equal work keep tree order. On the 1M merger the
+// rounds run up to 73 against a mean of 5.2 over the leaves with W, and the
+// heaviest leaves, clustered in tree order, otherwise start late and finish
+// last on their XCD (in a greedy model of the dispatch: a makespan of 108
+// rounds in tree order, 76 heavy first, 70 ideal).
+__global__ void __launch_bounds__(kT) k_m2p_work(TreeView t, int groups, uint64_t* __restrict__ khi,
+                                                 uint32_t* __restrict__ klo, int* __restrict__ val) {
+    const int li = blockIdx.x * kT + threadIdx.x;
+    if (li >= t.nleaves) return;
+    constexpr int L = kM2PWaves * 64;
+    const int nw = t.w_off[li + 1] - t.w_off[li], nt = t.seg_count[t.leaf_ids[li]];
+    int r = 0;
+    if (nw > 0) {
+        const int T = nt > L ? L : max(nt, L / groups), G = L / T;
+        r = min((nt + T - 1) / T * ((nw + G - 1) / G), 511);
+    }
+    khi[li] = 0;
+    klo[li] = (uint32_t)(511 - r);
+    val[li] = li;
+}
+
+const int* m2p_order(Pool& pool, hipStream_t s, const TreeView& t, int groups, bool reserve_only) {
+    const int nl = t.nleaves;
+    SortBufs sb;
+    for (int k = 0; k < 2; ++k) {
+        sb.khi[k] = pool.get<uint64_t>(k ? "mw_khi1" : "mw_khi0", nl);
+        sb.klo[k] = pool.get<uint32_t>(k ? "mw_klo1" : "mw_klo0", nl);
+        sb.val[k] = pool.get<int>(k ? "mw_val1" : "mw_val0", nl);
+    }
+    sb.hist = pool.get<int>("mw_hist", 512 * (size_t)sort_blocks(nl));
+    sb.scan_tmp = pool.get<int>("mw_scan", scan_tmp_ints(512 * sort_blocks(nl) + nl) + 16);
+    if (reserve_only) return nullptr;
+    hipLaunchKernelGGL(k_m2p_work, dim3(blocks(nl)), dim3(kT), 0, s, t, groups, sb.khi[0], sb.klo[0], sb.val[0]);
+    return sb.val[radix_sort(sb, nl, 0, 1, s)];
+}
+
 // The persistent schedule's leaf order on stream s: work bins, one radix pass.
 const int* p2p_order(Pool& pool, hipStream_t s, const TreeView& t, bool reserve_only) {
     const int nl = t.nleaves;
@@ -1465,14 +1508,15 @@ void m2l_launch(Pool& pool, hipStream_t s, const TreeView& t, const FmmTables& t
 // stream s; either buffer is null when its lists are empty. They read only the
 // tree, the lists, the bodies and the multipoles, so they can run beside M2L.
 template <int P>
-void far_passes(hipStream_t s, const TreeView& t, const double* M, double* Lx, double* farw, double* fpotw,
-                hipEvent_t x_done = nullptr, hipEvent_t w_done = nullptr) {
+void far_passes(Pool& pool, hipStream_t s, const TreeView& t, const double* M, double* Lx, double* farw,
+                double* fpotw, hipEvent_t x_done = nullptr, hipEvent_t w_done = nullptr) {
     if (Lx) hipLaunchKernelGGL((k_accumulate<P, 2>), dim3(xcd_chunk_grid(t.nb)), dim3(64), 0, s, t, Lx);
     if (x_done) AFMM_CHECK(hipEventRecord(x_done, s));
     if (farw) {
+        const int* ord = m2p_order(pool, s, t, m2p_groups<P>(), false);
         const dim3 mg(xcd_chunk_grid(t.nleaves)), mb(kM2PWaves * 64);
-        if (fpotw) hipLaunchKernelGGL((k_m2p<P, true, false>), mg, mb, 0, s, t, M, farw, fpotw);
-        else hipLaunchKernelGGL((k_m2p<P, false, false>), mg, mb, 0, s, t, M, farw, fpotw);
+        if (fpotw) hipLaunchKernelGGL((k_m2p<P, true, false>), mg, mb, 0, s, t, M, farw, fpotw, ord);
+        else hipLaunchKernelGGL((k_m2p<P, false, false>), mg, mb, 0, s, t, M, farw, fpotw, ord);
     }
     if (w_done) AFMM_CHECK(hipEventRecord(w_done, s));
 }
@@ -1512,6 +1556,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     double* nr = split ? pool.get<double>("near", 3 * (size_t)t.n) : nullptr;
     double* npot = split && phi ? pool.get<double>("nearpot", t.n) : nullptr;
     if (sh.p2p_schedule) p2p_order(pool, s, t, true);  // its buffers too
+    if (t.n_w > 0) m2p_order(pool, s, t, m2p_groups<P>(), true);
     if (reserve_only) return;
     // P2P needs only the tree and the U lists: on the second stream it runs
     // beside the far-field passes. It is forked before M2L in FP64 (it shares
@@ -1555,9 +1600,10 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     if (fb) AFMM_CHECK(hipStreamWaitEvent(s, side.w_done, 0));
     hipLaunchKernelGGL(k_far<P>, dim3(blocks(t.n)), dim3(kT), 0, s, t, L, far, fpot, farw, fpotw);
     if (!fb && t.n_w > 0) {  // M2P in order: added to far after L2P
+        const int* ord = m2p_order(pool, s, t, m2p_groups<P>(), false);
         const dim3 mg(xcd_chunk_grid(t.nleaves)), mb(kM2PWaves * 64);
-        if (fpot) hipLaunchKernelGGL((k_m2p<P, true, true>), mg, mb, 0, s, t, M, far, fpot);
-        else hipLaunchKernelGGL((k_m2p<P, false, true>), mg, mb, 0, s, t, M, far, fpot);
+        if (fpot) hipLaunchKernelGGL((k_m2p<P, true, true>), mg, mb, 0, s, t, M, far, fpot, ord);
+        else hipLaunchKernelGGL((k_m2p<P, false, true>), mg, mb, 0, s, t, M, far, fpot, ord);
     }
     if (split) {
         AFMM_CHECK(hipStreamWaitEvent(s, side.join, 0));
@@ -1637,7 +1683,7 @@ void run_far_passes(Pool& pool, hipStream_t s, const TreeView& tv, int p, bool p
     double* fpotw = tv.n_w > 0 && pot ? pool.get<double>("farwpot", tv.n) : nullptr;
     switch (p) {
 #define AFMM_CASE(PP) \
-    case PP: far_passes<PP>(s, tv, M, Lx, farw, fpotw, x_done, w_done); return;
+    case PP: far_passes<PP>(pool, s, tv, M, Lx, farw, fpotw, x_done, w_done); return;
         AFMM_CASE(1) AFMM_CASE(2) AFMM_CASE(3) AFMM_CASE(4) AFMM_CASE(5) AFMM_CASE(6)
         AFMM_CASE(7) AFMM_CASE(8) AFMM_CASE(9) AFMM_CASE(10) AFMM_CASE(11) AFMM_CASE(12)
         AFMM_CASE(13) AFMM_CASE(14) AFMM_CASE(15) AFMM_CASE(16)
