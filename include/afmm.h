@@ -323,7 +323,8 @@ int afmm_get_shapes(const afmm_ctx* ctx, afmm_shapes* out);
  *   AFMM_OVERLAP_AUTO (the default): a second stream, shared by the process's
  *     contexts on that device, runs the upward pass beside the interaction-list
  *     build and P2P beside the far-field passes; with the bf16 or split-FP32
- *     M2L, and the FP64 one up to p = 8 and from p = 14, the U/W/X lists, P2L, M2P and P2P
+ *     M2L, and the FP64 one up to p = 8, from p = 14 and at p = 9-13 for up to
+ *     500,000 bodies, the U/W/X lists, P2L, M2P and P2P
  *     run there beside M2L as well (their VALU work shares the SIMDs with its
  *     MFMAs);
  *   AFMM_OVERLAP_BESIDE_M2L: the latter schedule in every form, with P2P

@@ -318,7 +318,8 @@ class AdaptiveFMM3D:
         """Two-stream overlap (afmm_set_overlap): True or OVERLAP_AUTO (the
         default: the upward pass beside the list build, P2P beside the
         far-field passes, and with the bf16 or split-FP32 M2L, or FP64 up to
-        p = 8 and from p = 14, the lists, P2L, M2P and P2P beside M2L),
+        p = 8, from p = 14 and at p = 9-13 for up to 500,000 bodies, the
+        lists, P2L, M2P and P2P beside M2L),
         OVERLAP_BESIDE_M2L,
         OVERLAP_IN_ORDER, or
         False / OVERLAP_OFF (one stream, in order). Results are the same bits

@@ -64,6 +64,10 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 // The evaluation of nsys systems (concatenated device arrays: pos AoS n x 3,
 // mass n; system q = bodies [sys_off[q], sys_off[q+1])) into acc (AoS, input
 // order) and optionally phi. G and eps per system (host arrays).
+// FP64 at p = 9-13: the most bodies for which the automatic schedule runs the
+// list, P2L, M2P and P2P passes beside M2L.
+constexpr int64_t kBesideMaxBodiesMidOrders = 500000;
+
 int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G, const double* eps, const double* pos,
                 const double* mass, double* acc, double* phi) {
     c->have_tree = false;
@@ -100,10 +104,13 @@ int afmm::eval_forest(afmm_ctx* c, int nsys, const int* sys_off, const double* G
     // for FP64 up to p = 8 (1M: 5.79 -> 5.67 ms at p = 8, 2.51 -> 2.45 ms at
     // p = 4; 16k: 1.48 -> 1.26 ms at p = 8) and from p = 14 (16-wave
     // row-split M2L: 44.8 -> 41.9 ms at p = 14, 78.8 -> 73.8 ms at p = 16);
-    // FP64 in order at p = 9-13, where the longer M2L loses more beside the
-    // VALU passes than they save at 1M (p = 12: 23.9 ms against 27.5 ms).
+    // FP64 at p = 9-13 beside up to 500k bodies (p = 12: 7.26 -> 7.00 ms at
+    // 16k, 13.7 -> 13.0 ms at 400k) and in order above, where the longer M2L
+    // loses more beside the VALU passes than they save (1M, p = 12: 23.9 ms
+    // against 27.5 ms beside).
     const bool bf16_m2l = m2l_form(c->prm.precision) == kPrecMixedBf16;
-    const bool auto_beside = m2l_form(c->prm.precision) != kPrecFP64 || c->prm.p <= 8 || c->prm.p >= 14;
+    const bool auto_beside = m2l_form(c->prm.precision) != kPrecFP64 || c->prm.p <= 8 || c->prm.p >= 14 ||
+                             n <= kBesideMaxBodiesMidOrders;
     const bool lists_beside = split && !c->use_graph &&
                               (c->overlap_mode == AFMM_OVERLAP_BESIDE_M2L ||
                                (c->overlap_mode == AFMM_OVERLAP_AUTO && auto_beside));
