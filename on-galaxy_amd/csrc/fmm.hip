@@ -370,32 +370,49 @@ __global__ void __launch_bounds__(kT) k_zero_int(int* a, size_t n) {
 // operators are all kept in slot order).
 __constant__ M2LSlots kM2LSlots = make_m2l_slots();
 
-// srcT[slot * np + r] = the source box of target order[r] at the offset of
-// that slot, -1 where it has none, and present[tile][slot] = 1 when some
-// target of the tile has a source at that slot (present zeroed before). One
-// block per 16 consecutive targets of the M2L order: their V lists fill a
-// slots x 16 table in LDS (initially -1), which is then written out row by
-// row, 64 contiguous bytes per offset, so the whole table is written once and
-// coalesced (no separate fill of -1). Presence: one integer atomic per
-// (slot, run of the block's targets in one tile) with a source there
-// (order-free; a block's targets span one tile or a few small ones).
+// Source-table rows: a target at integer coordinate 2 P + c (c = 0 or 1 per
+// axis, the parities par = 4 cx + 2 cy + cz) meets its V-list sources, the
+// children of its parent's colleagues P - 1 .. P + 1 that are not adjacent,
+// only at offsets d = ic(target) - ic(source) in [c - 3, c + 2]: 6 of the 7
+// values per axis, 216 of the 343 offsets. Row of offset index oi (of
+// offset_index) for parities par, or -1 where the target cannot meet it.
+constexpr int kSrcRows = 216;
+__device__ __forceinline__ int src_row(int oi, int par) {
+    const int ux = oi / 49 - (par >> 2), uy = (oi / 7) % 7 - ((par >> 1) & 1), uz = oi % 7 - (par & 1);
+    return (unsigned)ux < 6u && (unsigned)uy < 6u && (unsigned)uz < 6u ? (ux * 6 + uy) * 6 + uz : -1;
+}
+__device__ __forceinline__ int ic_parity(const int* __restrict__ ic, int b) {
+    return ((ic[3 * (size_t)b] & 1) << 2) | ((ic[3 * (size_t)b + 1] & 1) << 1) | (ic[3 * (size_t)b + 2] & 1);
+}
+
+// srcT[row * np + r] = the source box of target order[r] at the offset of that
+// row (src_row for the target's parities), -1 where it has none, and
+// present[tile][slot] = 1 when some target of the tile has a source at that
+// slot (present zeroed before). One block per 16 consecutive targets of the
+// M2L order: their V lists fill a rows x 16 table in LDS (initially -1), which
+// is then written out row by row, 64 contiguous bytes per row, so the whole
+// table is written once and coalesced (no separate fill of -1). Presence: one
+// integer atomic per (slot, run of the block's targets in one tile) with a
+// source there (order-free; a block's targets span one tile or a few small
+// ones).
 constexpr int kSrcTargets = 16;
 __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __restrict__ order,
                                                const int* __restrict__ vl, const int* __restrict__ nv,
                                                const int* __restrict__ ic, int* __restrict__ srcT,
                                                const int* __restrict__ tstart, int ntiles,
                                                int* __restrict__ present) {
-    __shared__ int tab[kNumOffsets][kSrcTargets];
-    __shared__ int box[kSrcTargets], cnt[kSrcTargets], tile_of[kSrcTargets];
+    __shared__ int tab[kSrcRows][kSrcTargets];
+    __shared__ int box[kSrcTargets], cnt[kSrcTargets], tile_of[kSrcTargets], par[kSrcTargets];
     // consecutive blocks on one XCD, so the two 64-byte halves of a 128-byte
     // row segment are written through the same L2
     const int r0 = xcd_item() * kSrcTargets, tid = threadIdx.x;
     if (r0 >= nb) return;
     const int nt = min(kSrcTargets, nb - r0);
-    for (int e = tid; e < kNumOffsets * kSrcTargets; e += kT) (&tab[0][0])[e] = -1;
+    for (int e = tid; e < kSrcRows * kSrcTargets; e += kT) (&tab[0][0])[e] = -1;
     if (tid < nt) {
         box[tid] = order[r0 + tid];
         cnt[tid] = nv[box[tid]];
+        par[tid] = ic_parity(ic, box[tid]);
         // the tile of the target: the last k with tstart[k] <= rank (one
         // search per target, side by side)
         int lo = 0, hi = ntiles - 1;
@@ -413,17 +430,20 @@ __global__ void __launch_bounds__(kT) k_m2l_src(int nb, int np, const int* __res
         const int b = box[j], c = vl[kVSlots * (size_t)b + e];
         const int o = offset_index(ic[3 * (size_t)b] - ic[3 * (size_t)c], ic[3 * (size_t)b + 1] - ic[3 * (size_t)c + 1],
                                    ic[3 * (size_t)b + 2] - ic[3 * (size_t)c + 2]);
-        tab[kM2LSlots.slot_of[o]][j] = c;
+        const int row = src_row(o, par[j]);
+        if (row >= 0) tab[row][j] = c;  // (always: V sources lie in the target's range)
     }
     __syncthreads();
-    for (int e = tid; e < kNumOffsets * kSrcTargets; e += kT) {
+    for (int e = tid; e < kSrcRows * kSrcTargets; e += kT) {
         const int k = e / kSrcTargets, j = e - k * kSrcTargets;
         if (j < nt) srcT[(size_t)k * np + r0 + j] = tab[k][j];
     }
     for (int k = tid; k < kNumOffsets; k += kT) {
+        const int oi = kM2LSlots.offset_of[k];
         bool any = false;
         for (int j = 0; j < nt; ++j) {
-            any |= tab[k][j] >= 0;
+            const int row = src_row(oi, par[j]);
+            any |= row >= 0 && tab[row][j] >= 0;
             if (j + 1 == nt || tile_of[j + 1] != tile_of[j]) {  // end of a run in one tile
                 if (any) atomicOr(&present[(size_t)tile_of[j] * kNumOffsets + k], 1);
                 any = false;
@@ -480,6 +500,7 @@ __device__ __forceinline__ double a_value(float2 a) { return (double)a.x + (doub
 template <int P, bool MIX, int W, int RSX = 0>
 __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const int* __restrict__ tstart,
                                                            const int* __restrict__ order,
+                                                           const int* __restrict__ ic,
                                                            const int* __restrict__ srcT,
                                                            const int* __restrict__ present_g,
                                                            const typename M2LTypes<MIX>::A* __restrict__ Asw,
@@ -510,12 +531,13 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
     const int tile = xcd_item();
     if (tile >= ntiles) return;
     const int base = tstart[tile], end = tstart[tile + 1];  // end - base <= TILE
-    int rank[CB];
+    int rank[CB], par[CB];
     bool valid[CB];
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
         rank[cb] = base + (grp * CB + cb) * 16 + j;
         valid[cb] = rank[cb] < end;
+        par[cb] = valid[cb] ? ic_parity(ic, order[rank[cb]]) : 0;
     }
     Acc acc[CB][RBH];
     // MIX: each offset's products are summed in FP32 from zero and folded into
@@ -569,7 +591,8 @@ __global__ void __launch_bounds__(W * 64) k_m2l_mfma(int np, int ntiles, const i
             bool any = false;
 #pragma unroll
             for (int cb = 0; cb < CB; ++cb) {
-                src[cb] = (o < kNumOffsets && valid[cb]) ? srcT[(size_t)o * np + rank[cb]] : -1;
+                const int row = o < kNumOffsets && valid[cb] ? src_row(kM2LSlots.offset_of[o], par[cb]) : -1;
+                src[cb] = row >= 0 ? srcT[(size_t)row * np + rank[cb]] : -1;
                 any |= src[cb] >= 0;
             }
             return __ballot(any) != 0ull;
@@ -786,6 +809,7 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast
 template <int P, int W>
 __global__ void __launch_bounds__(W * 64) k_m2l_bf16(int np, int ntiles, const int* __restrict__ tstart,
                                                            const int* __restrict__ order,
+                                                           const int* __restrict__ ic,
                                                            const int* __restrict__ srcT,
                                                            const int* __restrict__ present_g,
                                                            const uint4* __restrict__ Abf,
@@ -804,6 +828,7 @@ __global__ void __launch_bounds__(W * 64) k_m2l_bf16(int np, int ntiles, const i
     const int base = tstart[tile], end = tstart[tile + 1];  // end - base <= 16 W
     const int rank = base + wave * 16 + j;
     const bool valid = rank < end;
+    const int par = valid ? ic_parity(ic, order[rank]) : 0;
     // each offset's products are summed in FP32 from zero and folded into FP64
     // totals when the offset is done (FP32 rounding never compounds across offsets)
     flt4 acc[RB];
@@ -835,7 +860,8 @@ __global__ void __launch_bounds__(W * 64) k_m2l_bf16(int np, int ntiles, const i
             }
         };
         auto load_src = [&](int o, int& src) {
-            src = (o < kNumOffsets && valid) ? srcT[(size_t)o * np + rank] : -1;
+            const int row = o < kNumOffsets && valid ? src_row(kM2LSlots.offset_of[o], par) : -1;
+            src = row >= 0 ? srcT[(size_t)row * np + rank] : -1;
             return __ballot(src >= 0) != 0ull;
         };
         int o = next_present(-1);
@@ -1408,9 +1434,10 @@ const int* p2p_order(Pool& pool, hipStream_t s, const TreeView& t, bool reserve_
     return sb.val[radix_sort(sb, nl, 0, 1, s)];
 }
 
-// M2L preparation: the transposed source table srcT[slot * np + rank] (rank:
-// the target's position in the M2L order; -1 where it has no source at that
-// offset) and the per-tile offset presence table.
+// M2L preparation: the transposed source table srcT[row * np + rank] (rank:
+// the target's position in the M2L order, row: src_row of the offset for the
+// target's parities; -1 where it has no source at that offset) and the
+// per-tile offset presence table.
 void m2l_prep(hipStream_t s, const TreeView& t, int np, int ntiles, const int* tstart, int* srcT, int* present) {
     const int nb = t.nb;
     hipLaunchKernelGGL(k_zero_int, dim3(blocks((long long)ntiles * kNumOffsets)), dim3(kT), 0, s, present,
@@ -1430,7 +1457,7 @@ void m2l_prep_for(Pool& pool, hipStream_t s, const TreeView& t, int prec, const 
     if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
-    int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
+    int* srcT = pool.get<int>("m2l_srcT", (size_t)kSrcRows * np);
     int* present = pool.get<int>("m2l_present", (size_t)ntiles * kNumOffsets);
     m2l_prep(s, t, np, ntiles, tstart, srcT, present);
 }
@@ -1466,7 +1493,7 @@ void m2l_launch(Pool& pool, hipStream_t s, const TreeView& t, const FmmTables& t
     if (TILE != 256 && TILE != 128) throw std::runtime_error("M2L tile size without a plan");
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
-    const int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
+    const int* srcT = pool.get<int>("m2l_srcT", (size_t)kSrcRows * np);
     const int* present = pool.get<int>("m2l_present", (size_t)ntiles * kNumOffsets);
     uint4* Mbf = form == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
                                         : nullptr;
@@ -1479,24 +1506,24 @@ void m2l_launch(Pool& pool, hipStream_t s, const TreeView& t, const FmmTables& t
         constexpr int W = decltype(wtag)::value;
         const dim3 g(xcd_grid(ntiles)), b(W * 64);
         if (form == kPrecMixedBf16)
-            hipLaunchKernelGGL((k_m2l_bf16<P, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT, present,
+            hipLaunchKernelGGL((k_m2l_bf16<P, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, t.ic, srcT, present,
                                tab.m2l_bf, Mbf, L);
         else if (form == kPrecMixed)
-            hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+            hipLaunchKernelGGL((k_m2l_mfma<P, true, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, t.ic, srcT,
                                present, tab.m2l_hl, M, L);
         else if constexpr (W == 16 && m2l_row_split(P, false, W) == 1) {
             // tiles of 16 or 32 targets (small trees): four or two waves per block
             if (t.m2l_t256 <= 16)
-                hipLaunchKernelGGL((k_m2l_mfma<P, false, W, 4>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                hipLaunchKernelGGL((k_m2l_mfma<P, false, W, 4>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, t.ic, srcT,
                                    present, tab.m2l_sw, M, L);
             else if (t.m2l_t256 <= 32)
-                hipLaunchKernelGGL((k_m2l_mfma<P, false, W, 2>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                hipLaunchKernelGGL((k_m2l_mfma<P, false, W, 2>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, t.ic, srcT,
                                    present, tab.m2l_sw, M, L);
             else
-                hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+                hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, t.ic, srcT,
                                    present, tab.m2l_sw, M, L);
         } else
-            hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, srcT,
+            hipLaunchKernelGGL((k_m2l_mfma<P, false, W>), g, b, 0, s, np, ntiles, tstart, t.m2l_order, t.ic, srcT,
                                present, tab.m2l_sw, M, L);
     };
     if (sh.m2l_waves == 16) launch_m2l(std::integral_constant<int, 16>{});
@@ -1541,7 +1568,7 @@ void run(Pool& pool, hipStream_t s, const TreeView& t, const TreeMeta& meta, con
     const int* tstart = TILE == 256 ? t.m2l_tiles256 : t.m2l_tiles128;
     const int ntiles = TILE == 256 ? t.m2l_ntiles256 : t.m2l_ntiles128;
     const int np = nb;
-    int* srcT = pool.get<int>("m2l_srcT", (size_t)kNumOffsets * np);
+    int* srcT = pool.get<int>("m2l_srcT", (size_t)kSrcRows * np);
     int* present = pool.get<int>("m2l_present", (size_t)ntiles * kNumOffsets);
     uint4* Mbf = form == kPrecMixedBf16 ? pool.get<uint4>("m2l_Mbf", (size_t)nb * m2l_bf_kb(S) * kBfPieces * 4)
                                         : nullptr;
